@@ -1,0 +1,207 @@
+/*
+ * qsi_gpu.h — C ABI of the MI355X (gfx950) matrix-free ring-exchange H·v and
+ * Lanczos library.
+ *
+ * The reference computes H·v on the host with OpenMP (LazyOpSum, push form with
+ * atomics) and drives it from Spectra's SymEigsSolver (`mfsparse`) or its own
+ * 3-term Lanczos (`mfeig0`).  This library keeps the same objects — a sorted
+ * basis of 128-bit Z-basis states, a list of permutation-matrix ("PMR") terms
+ * with coefficients, an operator that applies y = Hx — but every vector and
+ * basis lives in GPU memory and every pass over the basis is a HIP kernel.
+ *
+ * Conventions
+ *   - All functions return 0 on success and a negative QSI_E* code on failure;
+ *     qsi_last_error() gives a message (thread local).  Nothing throws across
+ *     this boundary.
+ *   - A "state" is a pair of uint64 words [lo, hi] (bit i = spin i, 1 = up),
+ *     exactly the HDF5 (D,2) layout of the reference's basis files
+ *     (components/common/include/basis_io_h5.hpp:30-70,90-141).
+ *   - Vector arguments named d_* are device pointers; h_* are host pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *     device work is stream ordered; an entry point returns once its work is
+ *     enqueued unless its comment says it synchronises.
+ */
+#ifndef QSI_GPU_H
+#define QSI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *qsi_stream_t;
+typedef struct qsi_basis_s *qsi_basis_t;
+typedef struct qsi_op_s *qsi_op_t;
+
+enum {
+  QSI_OK = 0,
+  QSI_E_INVALID = -1,   /* bad argument (null pointer, bad size, unsorted basis...) */
+  QSI_E_NOMEM = -2,     /* device or host allocation failed                          */
+  QSI_E_HIP = -3,       /* a HIP runtime call failed                                 */
+  QSI_E_NOTFOUND = -4,  /* a connected state is missing from the basis              */
+  QSI_E_NOCONV = -5,    /* an iterative solver hit its iteration limit               */
+  QSI_E_UNSUPPORTED = -6
+};
+
+/*
+ * One operator term in the reference's SymbolicPMROperator representation
+ * (components/operator/include/operator.hpp:343-347): it maps |s> to
+ *   sign * (-1)^popcount(~s' & Z) * |s'>,  s' = s ^ X,
+ * when (s & down) == 0 and (s & up) == up, and to 0 otherwise
+ * (applyState, operator.hpp:232-254).  `coeff` is the SymbolicOpSum
+ * coefficient (operator.hpp:489-495).  Words are [lo, hi].
+ * A term with X == 0 is diagonal.
+ */
+typedef struct {
+  double coeff;
+  uint64_t X[2];
+  uint64_t Z[2];
+  uint64_t up[2];
+  uint64_t down[2];
+  int32_t sign;
+  int32_t slot; /* coefficient slot for block applies with per-column
+                   coefficients (qsi_op_apply_block); -1 = use `coeff`   */
+} qsi_term_t;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *qsi_last_error(void);
+/* Version string, e.g. "qsi-gpu 0.1 gfx950". */
+const char *qsi_version(void);
+/* Number of visible GPUs (0 if none).  Does not fail. */
+int qsi_device_count(void);
+
+/* ---- basis ---------------------------------------------------------------
+ * Replaces ZBasisInterp / ZBasisBST (components/operator/include/operator.hpp:
+ * 153-204; load_from_file + find_bounds, components/operator/src/operator.cpp:
+ * 73-86,138-159).  The host array (D rows of [lo,hi], strictly increasing as
+ * 128-bit integers) is copied to the device once; a top-bit bucket table
+ * (2^K+1 uint32 ranks) is built on the device.  n_spins bounds the state bits
+ * (1..128).  bucket_bits = 0 picks K automatically.  Synchronises. */
+int qsi_basis_create(const uint64_t *h_lohi, int64_t dim, int n_spins,
+                     int bucket_bits, qsi_basis_t *out);
+int qsi_basis_destroy(qsi_basis_t b);
+int64_t qsi_basis_dim(qsi_basis_t b);
+int qsi_basis_n_spins(qsi_basis_t b);
+int qsi_basis_bucket_bits(qsi_basis_t b);
+/* Device-side state -> rank lookup of n states (test hook and building block;
+ * replaces ZBasisInterp::search, operator.cpp:38-70).  d_lohi: n rows of
+ * [lo,hi] in device memory; d_rank receives the rank or -1 if absent. */
+int qsi_basis_search(qsi_basis_t b, const uint64_t *d_lohi, int64_t n,
+                     int64_t *d_rank, qsi_stream_t stream);
+
+/* ---- operator ------------------------------------------------------------
+ * Replaces LazyOpSum<double, ZBasisInterp> (components/operator/include/
+ * operator_matrix.hpp:10-118).  Borrows the basis (kept alive by a
+ * reference count), copies the terms.  n_slots > 0 declares coefficient slots
+ * for qsi_op_apply_block. */
+int qsi_op_create(qsi_basis_t b, const qsi_term_t *terms, int n_terms,
+                  int n_slots, qsi_op_t *out);
+int qsi_op_destroy(qsi_op_t op);
+int64_t qsi_op_dim(qsi_op_t op);
+int qsi_op_n_offdiag(qsi_op_t op);
+int qsi_op_n_diag(qsi_op_t op);
+
+/* y = H x (accumulate = 0) or y += H x (accumulate = 1).
+ * accumulate = 0 is LazyOpSum::evaluate / LazyOpSumProd::perform_op
+ * (operator_matrix.hpp:32-36,129-131); accumulate = 1 is evaluate_add
+ * (components/operator/src/operator_matrix.cpp:108-112) and the
+ * projED::RealApplyFn contract (components/linalg_routines/include/
+ * common_bits.hpp:18-21).  Off-diagonal contributions with |x_j| <= 1e-15
+ * are skipped (APPLY_TOL, operator_matrix.hpp:113).  x and y must not alias. */
+int qsi_op_apply(qsi_op_t op, const double *d_x, double *d_y, int accumulate,
+                 qsi_stream_t stream);
+
+/* Same contract with host vectors (copies in and out; synchronises).  This is
+ * the literal drop-in for LazyOpSumProd::perform_op under a host eigensolver;
+ * it includes PCIe traffic. */
+int qsi_op_apply_host(qsi_op_t op, const double *h_x, double *h_y,
+                      int accumulate);
+
+/* Block apply: Y[:, r] (+)= H_r X[:, r] for r < R, X and Y row-major D x R
+ * with leading dimensions ldx, ldy >= R.  If d_coeffs is non-null it holds
+ * n_slots x R doubles (slot-major): term t with slot k >= 0 uses
+ * d_coeffs[k*R + r] for column r instead of its own coeff. */
+int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
+                       double *d_Y, int64_t ldy, int R, const double *d_coeffs,
+                       int accumulate, qsi_stream_t stream);
+
+/* Connectivity materialiser (the matrix-free equivalent of
+ * LazyOpSum::toSparseMatrix, operator_matrix.hpp:79-105): writes the nonzero
+ * off-diagonal entries H[row, col] = val in row-major order plus the diagonal
+ * value of every row.  Call with d_rows == NULL to get the count in *nnz.
+ * Otherwise the arrays must hold *nnz entries and d_diag D doubles.
+ * Synchronises. */
+int qsi_op_connectivity(qsi_op_t op, int64_t *nnz, int64_t *d_rows,
+                        int64_t *d_cols, double *d_vals, double *d_diag);
+
+/* ---- Lanczos / eigensolvers ---------------------------------------------- */
+
+/* Settings of the reference's in-house Lanczos (lanczos::Settings,
+ * components/linalg_routines/include/lanczos.hpp:9-20). */
+typedef struct {
+  int64_t krylov_dim;
+  double abs_tol;
+  double rel_tol;
+  int64_t min_iterations;
+  int64_t max_iterations;
+  uint32_t x0_seed;
+  int calc_eigenvector;
+  int verify_eigenvector;
+} qsi_lanczos_settings_t;
+
+typedef struct {
+  int eigval_converged;
+  int eigvec_converged;
+  int64_t n_iterations;
+  double eigval_error;
+  double eigvec_error;
+} qsi_lanczos_result_t;
+
+/* lanczos::eigval0 (components/linalg_routines/src/lanczos.cpp:184-235) with
+ * the vector recurrence on the GPU in FP64.  The start vector is the
+ * reference's: std::mt19937(x0_seed) + std::normal_distribution<double>,
+ * normalised (lanczos.cpp:55-62,80-81).  If calc_eigenvector, d_eigvec (D
+ * doubles, device) receives the Ritz vector.  Synchronises. */
+int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t *settings,
+                        double *eigval, double *d_eigvec,
+                        qsi_lanczos_result_t *result);
+
+/* Lowest-nev eigenpairs with a thick-restart Lanczos (full
+ * reorthogonalisation, ncv-dimensional search space), the GPU counterpart of
+ * compute_spectrum_iterative<LazyOpSumProd> -> Spectra::SymEigsSolver
+ * (include/matrix_diag_bits.hpp:51-95; SortRule::SmallestAlge).  tol is the
+ * Spectra-style relative residual tolerance.  d_v0 (D doubles, device) is the
+ * start vector or NULL for a seeded Gaussian start.  h_evals receives nev
+ * ascending eigenvalues, d_evecs (D x n_evecs column-major, may be NULL) the
+ * eigenvectors; *nconv the number of converged pairs.  Synchronises. */
+int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
+              const double *d_v0, uint64_t seed, double *h_evals,
+              double *d_evecs, int n_evecs, int *nconv, int *n_applies);
+
+/* ---- vector kernels (streaming FP64, deterministic reductions) ----------- */
+int qsi_vec_dot(const double *d_a, const double *d_b, int64_t n, double *h_out,
+                qsi_stream_t stream); /* synchronises */
+int qsi_vec_axpy(double alpha, const double *d_x, double *d_y, int64_t n,
+                 qsi_stream_t stream);
+int qsi_vec_scal(double alpha, double *d_x, int64_t n, qsi_stream_t stream);
+
+/* ---- basis enumeration (host) --------------------------------------------
+ * Ice-rule depth-first enumeration (components/build_basis/src/pyro_tree.cpp:
+ * 20-240; gen_spinon_basis.cpp:16-156): spins are visited in the order `order`
+ * (n_spins entries; the reference's default is its greedy ordering), each
+ * tetrahedron (4 member spins) must hold 2 up spins, and up to
+ * n_spinon_pairs forced violations are allowed as in fork_state_impl.  The
+ * sorted states are returned in a malloc'd array (*out_lohi, free with
+ * qsi_free); *out_dim receives D. */
+int qsi_enumerate_basis(int n_spins, int n_tetra, const int32_t *tetra_members,
+                        const int32_t *order, int n_spinon_pairs,
+                        int n_threads, uint64_t **out_lohi, int64_t *out_dim);
+void qsi_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSI_GPU_H */

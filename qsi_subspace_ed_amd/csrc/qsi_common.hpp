@@ -1,0 +1,153 @@
+// Shared device/host helpers for the qsi HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/qsi_gpu.h"
+
+namespace qsi {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* get_error();
+
+#define QSI_CHECK_HIP(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      ::qsi::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+      return QSI_E_HIP;                                                       \
+    }                                                                         \
+  } while (0)
+
+#define QSI_REQUIRE(cond, msg)                                                \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      ::qsi::set_error(msg);                                                  \
+      return QSI_E_INVALID;                                                   \
+    }                                                                         \
+  } while (0)
+
+// Checks the launch that was just enqueued.
+#define QSI_CHECK_LAUNCH() QSI_CHECK_HIP(hipGetLastError())
+
+// ---- launch geometry --------------------------------------------------------
+// MI355X: 256 CUs in 8 XCDs.  Workgroups are handed to XCDs round-robin by
+// workgroup id, so workgroup g runs on XCD g % 8.  xcd_remap() turns the
+// hardware id into a logical id such that each XCD works on one contiguous
+// range of logical ids (rows that are close in the sorted basis share flip
+// targets and x entries, so they should share an L2).
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t g, uint32_t n) {
+  const uint32_t per = n / kNumXcd;      // full rounds
+  const uint32_t rem = n % kNumXcd;
+  const uint32_t xcd = g % kNumXcd;
+  const uint32_t k = g / kNumXcd;
+  // XCD x owns logical ids [start(x), start(x) + cnt(x)), cnt = per + (x < rem)
+  const uint32_t start = xcd * per + (xcd < rem ? xcd : rem);
+  const uint32_t cnt = per + (xcd < rem ? 1u : 0u);
+  return k < cnt ? start + k : g;  // k >= cnt cannot happen when n % 8 handled
+}
+
+// ---- basis states -----------------------------------------------------------
+// A state of NW 64-bit words, word 0 = spins 0..63.  The device basis stores
+// NW words per state, interleaved ([lo] or [lo, hi]).
+template <int NW>
+struct State {
+  uint64_t w[NW];
+};
+
+template <int NW>
+__device__ __forceinline__ State<NW> load_state(const uint64_t* __restrict__ s,
+                                                int64_t i) {
+  State<NW> r;
+  if constexpr (NW == 1) {
+    r.w[0] = __builtin_nontemporal_load(s + i) ;
+  } else {
+    const ulonglong2 v = reinterpret_cast<const ulonglong2*>(s)[i];
+    r.w[0] = v.x;
+    r.w[1] = v.y;
+  }
+  return r;
+}
+
+// Plain (cached) load; used for lookups where reuse across lanes matters.
+template <int NW>
+__device__ __forceinline__ State<NW> load_state_cached(
+    const uint64_t* __restrict__ s, int64_t i) {
+  State<NW> r;
+  if constexpr (NW == 1) {
+    r.w[0] = s[i];
+  } else {
+    const ulonglong2 v = reinterpret_cast<const ulonglong2*>(s)[i];
+    r.w[0] = v.x;
+    r.w[1] = v.y;
+  }
+  return r;
+}
+
+template <int NW>
+__device__ __forceinline__ bool st_le(const State<NW>& a, const State<NW>& b) {
+  if constexpr (NW == 1) {
+    return a.w[0] <= b.w[0];
+  } else {
+    return (a.w[1] < b.w[1]) || (a.w[1] == b.w[1] && a.w[0] <= b.w[0]);
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ bool st_eq(const State<NW>& a, const State<NW>& b) {
+  if constexpr (NW == 1) {
+    return a.w[0] == b.w[0];
+  } else {
+    return a.w[0] == b.w[0] && a.w[1] == b.w[1];
+  }
+}
+
+// Top `kbits` of an n_spins-bit state: (state >> (n_spins - kbits)).
+template <int NW>
+__device__ __forceinline__ uint32_t bucket_key(const State<NW>& s, int shift) {
+  if constexpr (NW == 1) {
+    return static_cast<uint32_t>(s.w[0] >> shift);
+  } else {
+    if (shift >= 64) return static_cast<uint32_t>(s.w[1] >> (shift - 64));
+    if (shift == 0) return static_cast<uint32_t>(s.w[0]);
+    return static_cast<uint32_t>((s.w[1] << (64 - shift)) | (s.w[0] >> shift));
+  }
+}
+
+// Device view of a basis: states + top-bit bucket table.
+struct BasisView {
+  const uint64_t* states;  // D * NW words
+  const uint32_t* table;   // 2^K + 1 entries: table[k] = #states with key < k
+  int64_t dim;
+  int shift;               // n_spins - K
+  int kbits;
+};
+
+// Rank of `key` in the basis, or -1.  The bucket [table[k], table[k+1]) is
+// searched with a branch-free lower-bound bisection.
+template <int NW>
+__device__ __forceinline__ int64_t basis_lookup(const BasisView& b,
+                                                const State<NW>& key) {
+  const uint32_t k = bucket_key<NW>(key, b.shift);
+  if (k >= (1u << b.kbits)) return -1;
+  uint32_t lo = b.table[k];
+  uint32_t n = b.table[k + 1] - lo;
+  if (n == 0) return -1;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    const State<NW> m = load_state_cached<NW>(b.states, lo + half);
+    lo = st_le<NW>(m, key) ? lo + half : lo;
+    n -= half;
+  }
+  const State<NW> f = load_state_cached<NW>(b.states, lo);
+  return st_eq<NW>(f, key) ? static_cast<int64_t>(lo) : -1;
+}
+
+}  // namespace qsi

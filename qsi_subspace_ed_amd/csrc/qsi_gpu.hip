@@ -1,0 +1,1106 @@
+// Device side of the qsi library: device basis + top-bit bucket table,
+// matrix-free pull apply (single vector and block), connectivity
+// materialiser and streaming FP64 vector kernels.  gfx950 only.
+//
+// Reference behaviour being replaced (file:line in the reference tree):
+//   ZBasisInterp::find_bounds / search   components/operator/src/operator.cpp:38-86
+//   SymbolicPMROperator::applyState      components/operator/include/operator.hpp:232-254
+//   LazyOpSum::evaluate_add_off_diag_omp components/operator/src/operator_matrix.cpp:28-60
+//   LazyOpSum::evaluate_add_diagonal     components/operator/src/operator_matrix.cpp:65-82
+//   LazyOpSum::toSparseMatrix            components/operator/include/operator_matrix.hpp:79-105
+//
+// The reference pushes: for each term and each column j it finds
+// J = rank(O s_j) and atomically adds c*sign*x_j to y_J.  Here every output
+// row i pulls instead: for each term whose precondition holds on the
+// pre-image s_i ^ X, it gathers x at rank(s_i ^ X).  Both compute
+// y_i = sum_j H_ij x_j for the same H (no symmetry is assumed), but the pull
+// form writes each y_i exactly once, so it needs no atomics and its sum order
+// is fixed (term order, then diagonal terms) — results are deterministic.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "qsi_internal.hpp"
+#include "qsi_reduce.hpp"
+
+namespace qsi {
+
+constexpr int kApplyBlock = 256;
+constexpr double kApplyTol = 1e-15;   // LazyOpSum::APPLY_TOL
+constexpr double kSparseTol = 1e-14;  // toSparseMatrix default tol
+
+// ============================================================================
+// Basis: bucket table
+// ============================================================================
+
+// table[k] = first rank whose top-K key is >= k, for k in [0, 2^K].
+template <int NW>
+__global__ void k_build_table(const uint64_t* __restrict__ states, int64_t dim,
+                              int shift, uint32_t nkeys,
+                              uint32_t* __restrict__ table) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k > nkeys) return;
+  int64_t lo = 0, n = dim;
+  while (n > 0) {
+    const int64_t half = n >> 1;
+    const State<NW> s = load_state_cached<NW>(states, lo + half);
+    if (static_cast<int64_t>(bucket_key<NW>(s, shift)) < k) {
+      lo = lo + half + 1;
+      n = n - half - 1;
+    } else {
+      n = half;
+    }
+  }
+  table[k] = static_cast<uint32_t>(lo);
+}
+
+template <int NW>
+__global__ void k_search(BasisView b, const uint64_t* __restrict__ q, int64_t n,
+                         int64_t* __restrict__ rank) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State<NW> s;
+  // queries always come as [lo, hi] pairs
+  s.w[0] = q[2 * i];
+  if constexpr (NW == 2) s.w[1] = q[2 * i + 1];
+  bool ok = true;
+  if constexpr (NW == 1) ok = (q[2 * i + 1] == 0);
+  rank[i] = ok ? basis_lookup<NW>(b, s) : -1;
+}
+
+// ============================================================================
+// Apply: one lane per output row
+// ============================================================================
+
+template <int NW>
+__device__ __forceinline__ bool term_ok(const OffTerm& t, const State<NW>& s) {
+  if constexpr (NW == 1) {
+    return (s.w[0] & t.m[0]) == t.v[0];
+  } else {
+    return ((s.w[0] & t.m[0]) == t.v[0]) & ((s.w[1] & t.m[1]) == t.v[1]);
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ int parity_notz(const State<NW>& s,
+                                           const uint64_t* Z) {
+  int p = __popcll(~s.w[0] & Z[0]);
+  if constexpr (NW == 2) p += __popcll(~s.w[1] & Z[1]);
+  return p & 1;
+}
+
+template <int NW>
+__device__ __forceinline__ int popc_and(const State<NW>& s, const uint64_t* Z) {
+  int p = __popcll(s.w[0] & Z[0]);
+  if constexpr (NW == 2) p += __popcll(s.w[1] & Z[1]);
+  return p;
+}
+
+template <int NW>
+__device__ __forceinline__ State<NW> xor_state(const State<NW>& s,
+                                               const uint64_t* X) {
+  State<NW> r;
+  r.w[0] = s.w[0] ^ X[0];
+  if constexpr (NW == 2) r.w[1] = s.w[1] ^ X[1];
+  return r;
+}
+
+// Diagonal value d(s) = sum_t c_t * (+-1 or field count); x-independent.
+template <int NW>
+__device__ __forceinline__ double diag_accumulate(
+    double acc, double xi, const State<NW>& s, const DiagTerm* __restrict__ dg,
+    int n_diag) {
+  for (int t = 0; t < n_diag; ++t) {
+    const DiagTerm& d = dg[t];
+    double f;
+    if (d.kind == 0) {
+      f = d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z));
+    } else {
+      int nz = __popcll(d.Z[0]);
+      if constexpr (NW == 2) nz += __popcll(d.Z[1]);
+      f = static_cast<double>(2 * popc_and<NW>(s, d.Z) - nz);
+    }
+    // reference: dy = c * x[i] * applyState(psi)
+    acc += d.c * xi * f;
+  }
+  return acc;
+}
+
+// y[i] = yscale * y_in[i] + sum_t H_{i,t} x[...] + d(s_i) x[i]
+// If DOT, partials[block] = sum over the block's rows of x[i] * y[i].
+template <int NW, bool HAS_Z, bool DOT>
+__global__ __launch_bounds__(kApplyBlock) void k_apply(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  double prod = 0.0;
+  if (i < b.dim) {
+    const State<NW> s = load_state_cached<NW>(b.states, i);
+    const double xi = x[i];
+    double acc = y_in ? yscale * y_in[i] : 0.0;
+    for (int t0 = 0; t0 < n_off; t0 += 64) {
+      const int tn = min(64, n_off - t0);
+      // phase 1: which terms connect this row (uniform loop over terms)
+      uint64_t mask = 0;
+      for (int t = 0; t < tn; ++t) {
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
+      }
+      // phase 2: gather through the rank lookup, in term order
+      while (mask) {
+        const int t = t0 + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const OffTerm& T = off[t];
+        const State<NW> sj = xor_state<NW>(s, T.X);
+        const int64_t J = basis_lookup<NW>(b, sj);
+        if (J >= 0) {
+          const double xj = x[J];
+          if (fabs(xj) > kApplyTol) {
+            double c = T.c * T.sgn;
+            if constexpr (HAS_Z) {
+              if (parity_notz<NW>(s, T.Z)) c = -c;
+            }
+            acc += c * xj;
+          }
+        }
+      }
+    }
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
+}
+
+// ============================================================================
+// Block apply: a wavefront owns one row at a time, lanes own columns.
+// Lanes first test the row's terms in parallel (one term per lane, results
+// combined with a wavefront ballot), then each connected row rank J is found
+// once and the R-wide row X[J, :] is read with one coalesced load per lane.
+// ============================================================================
+
+template <int NW>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_block(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ X,
+    int64_t ldx, double* __restrict__ Y, int64_t ldy, int R,
+    const double* __restrict__ coeffs, int accumulate) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t wave_global =
+      (static_cast<int64_t>(xcd_remap(blockIdx.x, gridDim.x)) * kApplyBlock +
+       threadIdx.x) /
+      kWave;
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * kApplyBlock / kWave;
+  // rows per wave pass: lanes are split into groups of G >= R lanes
+  int G = 64;
+  while (G / 2 >= R && G > 1) G >>= 1;
+  const int groups = kWave / G;
+  const int grp = lane / G;
+  const int col = lane % G;
+  const bool col_ok = col < R;
+
+  for (int64_t r0 = wave_global * groups; r0 < b.dim; r0 += n_waves * groups) {
+    const int64_t i = r0 + grp;
+    const bool row_ok = i < b.dim;
+    State<NW> s;
+    if (row_ok) s = load_state_cached<NW>(b.states, i);
+    else s = load_state_cached<NW>(b.states, 0);
+    double acc = 0.0;
+    if (accumulate && row_ok && col_ok) acc = Y[i * ldy + col];
+    for (int t0 = 0; t0 < n_off; t0 += G) {
+      // lane `col` tests term t0 + col for its group's row
+      const int t = t0 + col;
+      const bool ok = row_ok && t < n_off && term_ok<NW>(off[t], s);
+      uint64_t bal = __ballot(ok);
+      uint64_t mine = (G == 64) ? bal : ((bal >> (grp * G)) & ((1ull << G) - 1));
+      // each set bit: one lookup, done redundantly by the group's lanes
+      // (same address -> one memory transaction per group)
+      while (mine) {
+        const int tt = t0 + __builtin_ctzll(mine);
+        mine &= mine - 1;
+        const OffTerm& T = off[tt];
+        const State<NW> sj = xor_state<NW>(s, T.X);
+        const int64_t J = basis_lookup<NW>(b, sj);
+        if (J >= 0 && col_ok) {
+          const double xj = X[J * ldx + col];
+          if (fabs(xj) > kApplyTol) {
+            double c = (coeffs && T.slot >= 0) ? coeffs[T.slot * R + col] : T.c;
+            c *= T.sgn;
+            if (parity_notz<NW>(s, T.Z)) c = -c;
+            acc += c * xj;
+          }
+        }
+      }
+    }
+    if (row_ok && col_ok) {
+      const double xi = X[i * ldx + col];
+      for (int t = 0; t < n_diag; ++t) {
+        const DiagTerm& d = dg[t];
+        double f;
+        if (d.kind == 0) {
+          f = d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z));
+        } else {
+          int nz = __popcll(d.Z[0]);
+          if constexpr (NW == 2) nz += __popcll(d.Z[1]);
+          f = static_cast<double>(2 * popc_and<NW>(s, d.Z) - nz);
+        }
+        const double c = (coeffs && d.slot >= 0) ? coeffs[d.slot * R + col] : d.c;
+        acc += c * xi * f;
+      }
+      Y[i * ldy + col] = acc;
+    }
+  }
+}
+
+// ============================================================================
+// Connectivity (row-major triplets of the off-diagonal part + diagonal)
+// ============================================================================
+
+template <int NW>
+__global__ void k_conn(BasisView b, const OffTerm* __restrict__ off, int n_off,
+                       const DiagTerm* __restrict__ dg, int n_diag,
+                       const int64_t* __restrict__ offsets,
+                       int64_t* __restrict__ counts, int64_t* __restrict__ rows,
+                       int64_t* __restrict__ cols, double* __restrict__ vals,
+                       double* __restrict__ diag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.dim) return;
+  const State<NW> s = load_state_cached<NW>(b.states, i);
+  int64_t pos = offsets ? offsets[i] : 0;
+  int64_t cnt = 0;
+  for (int t = 0; t < n_off; ++t) {
+    const OffTerm& T = off[t];
+    if (!term_ok<NW>(T, s)) continue;
+    const int64_t J = basis_lookup<NW>(b, xor_state<NW>(s, T.X));
+    if (J < 0) continue;
+    double c = T.c * T.sgn;
+    if (parity_notz<NW>(s, T.Z)) c = -c;
+    if (!(fabs(c) > kSparseTol)) continue;
+    if (offsets) {
+      rows[pos] = i;
+      cols[pos] = J;
+      vals[pos] = c;
+      ++pos;
+    }
+    ++cnt;
+  }
+  if (counts) counts[i] = cnt;
+  if (diag) diag[i] = diag_accumulate<NW>(0.0, 1.0, s, dg, n_diag);
+}
+
+// ============================================================================
+// Streaming FP64 vector kernels
+// ============================================================================
+
+constexpr int kVecBlock = 256;
+
+inline int vec_blocks(int64_t n) {
+  int64_t nb = (n + kVecBlock * 4 - 1) / (kVecBlock * 4);
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nb, 8192)));
+}
+
+__global__ __launch_bounds__(kVecBlock) void k_dot_partial(
+    const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+    double* __restrict__ partials) {
+  double v = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
+    v += a[i] * b[i];
+  v = block_sum<kVecBlock>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kVecBlock) void k_axpy(double alpha,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ y,
+                                                    int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
+    y[i] += alpha * x[i];
+}
+
+__global__ __launch_bounds__(kVecBlock) void k_scal(double alpha,
+                                                    double* __restrict__ x,
+                                                    int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
+    x[i] *= alpha;
+}
+
+// u -= (*alpha) v, partials[b] = sum u^2 over the workgroup's share
+__global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
+    const double* __restrict__ alpha, const double* __restrict__ v,
+    double* __restrict__ u, int64_t n, double* __restrict__ partials) {
+  const double a = alpha[0];
+  double acc = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    const double w = u[i] - a * v[i];
+    u[i] = w;
+    acc += w * w;
+  }
+  acc = block_sum<kVecBlock>(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// x *= s[0] (s on the device)
+__global__ __launch_bounds__(kVecBlock) void k_scal_dev(
+    const double* __restrict__ s, double* __restrict__ x, int64_t n) {
+  const double a = s[0];
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
+    x[i] *= a;
+}
+
+// ---- tall-skinny kernels for full reorthogonalisation ----------------------
+// V is column-major with leading dimension ldv (column j at V + j*ldv).
+
+constexpr int kMaxCols = 64;
+
+// partials[blk * m + j] = sum over the workgroup's rows of V[:, j] * w
+__global__ __launch_bounds__(kVecBlock) void k_multidot(
+    const double* __restrict__ V, int64_t ldv, int m, const double* __restrict__ w,
+    int64_t n, double* __restrict__ partials) {
+  __shared__ double red[kVecBlock / kWave][16];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  for (int j0 = 0; j0 < m; j0 += 16) {
+    const int jn = min(16, m - j0);
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+      const double wi = w[i];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j < jn) acc[j] += V[(j0 + j) * ldv + i] * wi;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const double v = wave_sum(acc[j]);
+      if (lane == 0) red[wid][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < jn) {
+      double r = 0.0;
+      for (int k = 0; k < kVecBlock / kWave; ++k) r += red[k][threadIdx.x];
+      partials[static_cast<int64_t>(blockIdx.x) * m + j0 + threadIdx.x] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// out[j] = sum_b partials[b * m + j], one workgroup per column j, fixed order
+__global__ __launch_bounds__(256) void k_finalize_cols(
+    const double* __restrict__ partials, int nb, int m, double* __restrict__ out,
+    double* __restrict__ acc_out) {
+  const int j = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) v += partials[static_cast<int64_t>(b) * m + j];
+  v = block_sum<256>(v);
+  if (threadIdx.x == 0) {
+    out[j] = v;
+    if (acc_out) acc_out[j] += v;
+  }
+}
+
+// w -= V[:, 0:m] h
+__global__ __launch_bounds__(kVecBlock) void k_gemv_sub(
+    const double* __restrict__ V, int64_t ldv, int m, const double* __restrict__ h,
+    double* __restrict__ w, int64_t n) {
+  __shared__ double hs[kMaxCols];
+  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += V[j * ldv + i] * hs[j];
+    w[i] -= acc;
+  }
+}
+
+// out[:, c] = V[:, 0:m] Y[:, c] for c < k (Y column-major m x k, on device);
+// output columns in chunks of 8 so V is read ceil(k/8) times.
+__global__ __launch_bounds__(kVecBlock) void k_combine(
+    const double* __restrict__ V, int64_t ldv, int m, const double* __restrict__ Y,
+    int k, double* __restrict__ out, int64_t ldo, int64_t n) {
+  __shared__ double ys[kMaxCols * kMaxCols];
+  for (int t = threadIdx.x; t < m * k; t += kVecBlock) ys[t] = Y[t];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    for (int c0 = 0; c0 < k; c0 += 8) {
+      double acc[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double vj = V[j * ldv + i];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          if (c0 + c < k) acc[c] += vj * ys[(c0 + c) * m + j];
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c0 + c < k) out[(c0 + c) * ldo + i] = acc[c];
+    }
+  }
+}
+
+// x[i] = s_dev[0] * y[i]
+__global__ __launch_bounds__(kVecBlock) void k_copy_scaled(
+    const double* __restrict__ s, const double* __restrict__ y,
+    double* __restrict__ x, int64_t n) {
+  const double a = s[0];
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
+    x[i] = a * y[i];
+}
+
+// y += s_dev[0] * c * x
+__global__ __launch_bounds__(kVecBlock) void k_axpy_dev(
+    const double* __restrict__ s, double c, const double* __restrict__ x,
+    double* __restrict__ y, int64_t n) {
+  const double a = c * s[0];
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
+    y[i] += a * x[i];
+}
+
+// ============================================================================
+// Host helpers
+// ============================================================================
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* get_error() { return g_err.c_str(); }
+
+void basis_retain(qsi_basis_s* b) { b->refs.fetch_add(1); }
+void basis_release(qsi_basis_s* b) {
+  if (b->refs.fetch_sub(1) == 1) {
+    if (b->d_states) (void)hipFree(b->d_states);
+    if (b->d_table) (void)hipFree(b->d_table);
+    delete b;
+  }
+}
+
+int64_t apply_num_blocks(const qsi_op_s* op) {
+  return (op->basis->dim + kApplyBlock - 1) / kApplyBlock;
+}
+
+template <int NW, bool HAS_Z, bool DOT>
+static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
+                           const double* y_in, double yscale,
+                           const double* yscale_dev, double* partials,
+                           hipStream_t s) {
+  const int64_t nb = apply_num_blocks(op);
+  hipLaunchKernelGGL((k_apply<NW, HAS_Z, DOT>), dim3(static_cast<uint32_t>(nb)),
+                     dim3(kApplyBlock), 0, s, op->basis->view(), op->d_off,
+                     op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                     yscale_dev, partials);
+}
+
+int launch_apply(const qsi_op_s* op, const double* x, double* y,
+                 const double* y_in, double yscale, double* dot_partials,
+                 double* dot_out, hipStream_t s) {
+  return launch_apply_dev(op, x, y, y_in, yscale, nullptr, dot_partials,
+                          dot_out, s);
+}
+
+int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
+                     const double* y_in, double yscale,
+                     const double* yscale_dev, double* dot_partials,
+                     double* dot_out, hipStream_t s) {
+  const bool dot = dot_partials != nullptr;
+  const int nw = op->basis->nw;
+  const bool hz = op->has_z;
+#define QSI_LAUNCH(NW_, HZ_, D_) \
+  launch_apply_t<NW_, HZ_, D_>(op, x, y, y_in, yscale, yscale_dev, dot_partials, s)
+  if (nw == 1) {
+    if (hz) { if (dot) QSI_LAUNCH(1, true, true); else QSI_LAUNCH(1, true, false); }
+    else    { if (dot) QSI_LAUNCH(1, false, true); else QSI_LAUNCH(1, false, false); }
+  } else {
+    if (hz) { if (dot) QSI_LAUNCH(2, true, true); else QSI_LAUNCH(2, true, false); }
+    else    { if (dot) QSI_LAUNCH(2, false, true); else QSI_LAUNCH(2, false, false); }
+  }
+#undef QSI_LAUNCH
+  QSI_CHECK_LAUNCH();
+  if (dot) {
+    hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s,
+                       dot_partials, apply_num_blocks(op), dot_out, 0);
+    QSI_CHECK_LAUNCH();
+  }
+  return QSI_OK;
+}
+
+int launch_dot(const double* a, const double* b, int64_t n, double* partials,
+               double* out, hipStream_t s) {
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kVecBlock), 0, s, a, b, n,
+                     partials);
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials,
+                     static_cast<int64_t>(nb), out, 0);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_axpy(double alpha, const double* x, double* y, int64_t n,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, alpha,
+                     x, y, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_scal(double alpha, double* x, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scal, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, alpha,
+                     x, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+// u -= alpha_dev * v; nrm_out[0] = ||u||, nrm_out[1] = 1/||u||
+int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
+                     int64_t n, double* partials, double* nrm_out,
+                     hipStream_t s) {
+  const int nb = vec_blocks(n);
+  hipLaunchKernelGGL(k_axpy_dev_norm, dim3(nb), dim3(kVecBlock), 0, s,
+                     alpha_dev, v, u, n, partials);
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials,
+                     static_cast<int64_t>(nb), nrm_out, 1);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_scal_dev(const double* s_dev, double* x, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scal_dev, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s,
+                     s_dev, x, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int max_vec_partials() { return 8192; }
+
+int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
+                    int64_t n, double* partials, double* out, double* acc_out,
+                    hipStream_t s) {
+  if (m <= 0) return QSI_OK;
+  if (m > kMaxCols) {
+    set_error("launch_multidot: too many columns");
+    return QSI_E_INVALID;
+  }
+  const int nb = std::min(vec_blocks(n), max_vec_partials() / kMaxCols);
+  hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, w,
+                     n, partials);
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize_cols, dim3(m), dim3(256), 0, s, partials, nb, m,
+                     out, acc_out);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_gemv_sub(const double* V, int64_t ldv, int m, const double* h,
+                    double* w, int64_t n, hipStream_t s) {
+  if (m <= 0) return QSI_OK;
+  hipLaunchKernelGGL(k_gemv_sub, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, V,
+                     ldv, m, h, w, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_combine(const double* V, int64_t ldv, int m, const double* Y, int k,
+                   double* out, int64_t ldo, int64_t n, hipStream_t s) {
+  if (m > kMaxCols || k > kMaxCols) {
+    set_error("launch_combine: too many columns");
+    return QSI_E_INVALID;
+  }
+  hipLaunchKernelGGL(k_combine, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, V,
+                     ldv, m, Y, k, out, ldo, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_copy_scaled(const double* s_dev, const double* y, double* x,
+                       int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_scaled, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s,
+                     s_dev, y, x, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int launch_axpy_dev(const double* s_dev, double c, const double* x, double* y,
+                    int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_dev, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s,
+                     s_dev, c, x, y, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+}  // namespace qsi
+
+// ============================================================================
+// C ABI: library, basis, operator, apply, connectivity, vector ops
+// ============================================================================
+
+using namespace qsi;
+
+extern "C" {
+
+const char* qsi_last_error(void) { return get_error(); }
+
+const char* qsi_version(void) { return "qsi-gpu 0.1 gfx950"; }
+
+int qsi_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+static int auto_bucket_bits(int64_t dim, int n_spins) {
+  int lg = 0;
+  while ((int64_t{1} << lg) < dim) ++lg;
+  int k = lg + 2;
+  k = std::min(k, 28);
+  k = std::min(k, n_spins);
+  return std::max(k, 1);
+}
+
+int qsi_basis_create(const uint64_t* h_lohi, int64_t dim, int n_spins,
+                     int bucket_bits, qsi_basis_t* out) {
+  QSI_REQUIRE(out != nullptr, "qsi_basis_create: out is null");
+  *out = nullptr;
+  QSI_REQUIRE(h_lohi != nullptr || dim == 0, "qsi_basis_create: states are null");
+  QSI_REQUIRE(dim >= 0 && dim < (int64_t{1} << 32) - 1,
+              "qsi_basis_create: dim must be in [0, 2^32-1)");
+  QSI_REQUIRE(n_spins >= 1 && n_spins <= 128,
+              "qsi_basis_create: n_spins must be in [1, 128]");
+  QSI_REQUIRE(bucket_bits >= 0 && bucket_bits <= 30 && bucket_bits <= n_spins,
+              "qsi_basis_create: bucket_bits must be in [0, min(30, n_spins)]");
+  const int nw = n_spins <= 64 ? 1 : 2;
+  // validate: in range and strictly increasing
+  for (int64_t i = 0; i < dim; ++i) {
+    const uint64_t lo = h_lohi[2 * i], hi = h_lohi[2 * i + 1];
+    bool in_range;
+    if (n_spins <= 64) in_range = hi == 0 && (n_spins == 64 || (lo >> n_spins) == 0);
+    else in_range = (n_spins == 128) || (hi >> (n_spins - 64)) == 0;
+    if (!in_range) {
+      set_error("qsi_basis_create: state " + std::to_string(i) +
+                " has bits at or above n_spins");
+      return QSI_E_INVALID;
+    }
+    if (i > 0) {
+      const uint64_t plo = h_lohi[2 * i - 2], phi = h_lohi[2 * i - 1];
+      if (!(phi < hi || (phi == hi && plo < lo))) {
+        set_error("qsi_basis_create: states are not strictly increasing at row " +
+                  std::to_string(i));
+        return QSI_E_INVALID;
+      }
+    }
+  }
+  auto* b = new (std::nothrow) qsi_basis_s();
+  if (!b) {
+    set_error("qsi_basis_create: host allocation failed");
+    return QSI_E_NOMEM;
+  }
+  b->dim = dim;
+  b->n_spins = n_spins;
+  b->nw = nw;
+  b->kbits = bucket_bits ? bucket_bits : auto_bucket_bits(dim, n_spins);
+  b->shift = n_spins - b->kbits;
+  (void)hipGetDevice(&b->device);
+  const size_t words = static_cast<size_t>(std::max<int64_t>(dim, 1)) * nw;
+  const size_t nkeys = size_t{1} << b->kbits;
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(std::string("qsi_basis_create: ") + what + ": " + hipGetErrorString(e));
+    basis_release(b);
+    return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
+  };
+  hipError_t e = hipMalloc(&b->d_states, words * sizeof(uint64_t));
+  if (e != hipSuccess) return fail(e, "hipMalloc(states)");
+  e = hipMalloc(&b->d_table, (nkeys + 1) * sizeof(uint32_t));
+  if (e != hipSuccess) return fail(e, "hipMalloc(table)");
+  if (dim > 0) {
+    if (nw == 2) {
+      e = hipMemcpy(b->d_states, h_lohi, words * sizeof(uint64_t),
+                    hipMemcpyHostToDevice);
+    } else {
+      std::vector<uint64_t> lo(static_cast<size_t>(dim));
+      for (int64_t i = 0; i < dim; ++i) lo[i] = h_lohi[2 * i];
+      e = hipMemcpy(b->d_states, lo.data(), words * sizeof(uint64_t),
+                    hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return fail(e, "hipMemcpy(states)");
+  } else {
+    e = hipMemset(b->d_states, 0, words * sizeof(uint64_t));
+    if (e != hipSuccess) return fail(e, "hipMemset(states)");
+  }
+  const uint32_t nk = static_cast<uint32_t>(nkeys);
+  const uint32_t grid = static_cast<uint32_t>((nkeys + 1 + 255) / 256);
+  if (nw == 1)
+    hipLaunchKernelGGL(k_build_table<1>, dim3(grid), dim3(256), 0, 0,
+                       b->d_states, dim, b->shift, nk, b->d_table);
+  else
+    hipLaunchKernelGGL(k_build_table<2>, dim3(grid), dim3(256), 0, 0,
+                       b->d_states, dim, b->shift, nk, b->d_table);
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(e, "k_build_table launch");
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(e, "k_build_table");
+  *out = b;
+  return QSI_OK;
+}
+
+int qsi_basis_destroy(qsi_basis_t b) {
+  if (b) basis_release(b);
+  return QSI_OK;
+}
+
+int64_t qsi_basis_dim(qsi_basis_t b) { return b ? b->dim : -1; }
+int qsi_basis_n_spins(qsi_basis_t b) { return b ? b->n_spins : -1; }
+int qsi_basis_bucket_bits(qsi_basis_t b) { return b ? b->kbits : -1; }
+
+int qsi_basis_search(qsi_basis_t b, const uint64_t* d_lohi, int64_t n,
+                     int64_t* d_rank, qsi_stream_t stream) {
+  QSI_REQUIRE(b != nullptr, "qsi_basis_search: null basis");
+  QSI_REQUIRE(n >= 0, "qsi_basis_search: n < 0");
+  if (n == 0) return QSI_OK;
+  QSI_REQUIRE(d_lohi && d_rank, "qsi_basis_search: null pointer");
+  const uint32_t grid = static_cast<uint32_t>((n + 255) / 256);
+  auto s = static_cast<hipStream_t>(stream);
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_search<1>, dim3(grid), dim3(256), 0, s, b->view(),
+                       d_lohi, n, d_rank);
+  else
+    hipLaunchKernelGGL(k_search<2>, dim3(grid), dim3(256), 0, s, b->view(),
+                       d_lohi, n, d_rank);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
+                  int n_slots, qsi_op_t* out) {
+  QSI_REQUIRE(out != nullptr, "qsi_op_create: out is null");
+  *out = nullptr;
+  QSI_REQUIRE(b != nullptr, "qsi_op_create: null basis");
+  QSI_REQUIRE(n_terms >= 0 && (terms != nullptr || n_terms == 0),
+              "qsi_op_create: bad terms");
+  QSI_REQUIRE(n_slots >= 0, "qsi_op_create: n_slots < 0");
+  const int nw = b->nw;
+  auto* op = new (std::nothrow) qsi_op_s();
+  if (!op) {
+    set_error("qsi_op_create: host allocation failed");
+    return QSI_E_NOMEM;
+  }
+  // merged single-site Z terms: coefficient -> mask (kept in first-seen order)
+  std::vector<DiagTerm> fields;
+  for (int t = 0; t < n_terms; ++t) {
+    const qsi_term_t& T = terms[t];
+    const uint64_t bad_hi = (nw == 1) ? (T.X[1] | T.Z[1] | T.up[1] | T.down[1]) : 0;
+    if (bad_hi) {
+      delete op;
+      set_error("qsi_op_create: term " + std::to_string(t) +
+                " acts on spins >= 64 but the basis has <= 64 spins");
+      return QSI_E_INVALID;
+    }
+    if (T.slot >= 0 && T.slot >= n_slots) {
+      delete op;
+      set_error("qsi_op_create: term slot out of range");
+      return QSI_E_INVALID;
+    }
+    // a term whose sign is 0 or whose up/down masks overlap is identically 0
+    if (T.sign == 0 || (T.up[0] & T.down[0]) || (T.up[1] & T.down[1])) continue;
+    if ((T.X[0] | T.X[1]) == 0) {
+      // diagonal.  A diagonal term with up/down projectors is a Z string
+      // times projectors; only pure Z strings occur on the hot path.
+      if ((T.up[0] | T.up[1] | T.down[0] | T.down[1]) != 0) {
+        delete op;
+        set_error("qsi_op_create: diagonal terms with projector masks are not supported");
+        return QSI_E_UNSUPPORTED;
+      }
+      const int nz = __builtin_popcountll(T.Z[0]) + __builtin_popcountll(T.Z[1]);
+      if (nz == 1 && T.slot < 0) {
+        // single-site Z: c * sign * Z_i; merge equal effective coefficients
+        const double ce = T.coeff * T.sign;
+        bool merged = false;
+        for (auto& f : fields) {
+          if (f.c == ce) {
+            f.Z[0] |= T.Z[0];
+            f.Z[1] |= T.Z[1];
+            merged = true;
+            break;
+          }
+        }
+        if (!merged) {
+          DiagTerm d{};
+          d.Z[0] = T.Z[0];
+          d.Z[1] = T.Z[1];
+          d.c = ce;
+          d.sgn = 1.0;
+          d.kind = 1;
+          d.slot = -1;
+          fields.push_back(d);
+        }
+        continue;
+      }
+      DiagTerm d{};
+      d.Z[0] = T.Z[0];
+      d.Z[1] = T.Z[1];
+      d.c = T.coeff;
+      d.sgn = static_cast<double>(T.sign);
+      d.kind = 0;
+      d.slot = T.slot;
+      op->h_diag.push_back(d);
+    } else {
+      OffTerm o{};
+      for (int w = 0; w < 2; ++w) {
+        o.m[w] = T.up[w] | T.down[w];
+        o.v[w] = T.up[w] ^ (T.X[w] & o.m[w]);
+        o.X[w] = T.X[w];
+        o.Z[w] = T.Z[w];
+      }
+      o.c = T.coeff;
+      o.sgn = static_cast<double>(T.sign);
+      o.slot = T.slot;
+      if (o.Z[0] | o.Z[1]) op->has_z = true;
+      op->h_off.push_back(o);
+    }
+  }
+  for (auto& f : fields) op->h_diag.push_back(f);
+  op->n_off = static_cast<int>(op->h_off.size());
+  op->n_diag = static_cast<int>(op->h_diag.size());
+  op->n_slots = n_slots;
+  hipError_t e = hipMalloc(&op->d_off, std::max<size_t>(1, op->h_off.size()) * sizeof(OffTerm));
+  if (e == hipSuccess)
+    e = hipMalloc(&op->d_diag, std::max<size_t>(1, op->h_diag.size()) * sizeof(DiagTerm));
+  if (e == hipSuccess && op->n_off)
+    e = hipMemcpy(op->d_off, op->h_off.data(), op->h_off.size() * sizeof(OffTerm),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && op->n_diag)
+    e = hipMemcpy(op->d_diag, op->h_diag.data(), op->h_diag.size() * sizeof(DiagTerm),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (op->d_off) (void)hipFree(op->d_off);
+    if (op->d_diag) (void)hipFree(op->d_diag);
+    delete op;
+    set_error(std::string("qsi_op_create: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
+  }
+  basis_retain(b);
+  op->basis = b;
+  *out = op;
+  return QSI_OK;
+}
+
+int qsi_op_destroy(qsi_op_t op) {
+  if (!op) return QSI_OK;
+  if (op->d_off) (void)hipFree(op->d_off);
+  if (op->d_diag) (void)hipFree(op->d_diag);
+  if (op->basis) basis_release(op->basis);
+  delete op;
+  return QSI_OK;
+}
+
+int64_t qsi_op_dim(qsi_op_t op) { return op ? op->basis->dim : -1; }
+int qsi_op_n_offdiag(qsi_op_t op) { return op ? op->n_off : -1; }
+int qsi_op_n_diag(qsi_op_t op) { return op ? op->n_diag : -1; }
+
+int qsi_op_apply(qsi_op_t op, const double* d_x, double* d_y, int accumulate,
+                 qsi_stream_t stream) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_apply: null operator");
+  if (op->basis->dim == 0) return QSI_OK;
+  QSI_REQUIRE(d_x && d_y, "qsi_op_apply: null vector");
+  QSI_REQUIRE(d_x != d_y, "qsi_op_apply: x and y must not alias");
+  return launch_apply(op, d_x, d_y, accumulate ? d_y : nullptr, 1.0, nullptr,
+                      nullptr, static_cast<hipStream_t>(stream));
+}
+
+int qsi_op_apply_host(qsi_op_t op, const double* h_x, double* h_y,
+                      int accumulate) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_apply_host: null operator");
+  const int64_t D = op->basis->dim;
+  if (D == 0) return QSI_OK;
+  QSI_REQUIRE(h_x && h_y, "qsi_op_apply_host: null vector");
+  double *dx = nullptr, *dy = nullptr;
+  const size_t bytes = static_cast<size_t>(D) * sizeof(double);
+  QSI_CHECK_HIP(hipMalloc(&dx, bytes));
+  hipError_t e = hipMalloc(&dy, bytes);
+  if (e == hipSuccess) e = hipMemcpy(dx, h_x, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess && accumulate) e = hipMemcpy(dy, h_y, bytes, hipMemcpyHostToDevice);
+  int rc = QSI_OK;
+  if (e == hipSuccess) {
+    rc = launch_apply(op, dx, dy, accumulate ? dy : nullptr, 1.0, nullptr,
+                      nullptr, nullptr);
+    if (rc == QSI_OK) e = hipMemcpy(h_y, dy, bytes, hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_op_apply_host: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  return rc;
+}
+
+int qsi_op_apply_block(qsi_op_t op, const double* d_X, int64_t ldx, double* d_Y,
+                       int64_t ldy, int R, const double* d_coeffs,
+                       int accumulate, qsi_stream_t stream) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_apply_block: null operator");
+  QSI_REQUIRE(R >= 1 && R <= 64, "qsi_op_apply_block: R must be in [1, 64]");
+  QSI_REQUIRE(ldx >= R && ldy >= R, "qsi_op_apply_block: ld < R");
+  QSI_REQUIRE(d_coeffs == nullptr || op->n_slots > 0,
+              "qsi_op_apply_block: coefficients given but the operator has no slots");
+  if (op->basis->dim == 0) return QSI_OK;
+  QSI_REQUIRE(d_X && d_Y, "qsi_op_apply_block: null matrix");
+  QSI_REQUIRE(d_X != d_Y, "qsi_op_apply_block: X and Y must not alias");
+  // waves needed: one wave handles 64/G rows per pass
+  int G = 64;
+  while (G / 2 >= R && G > 1) G >>= 1;
+  const int64_t rows_per_wave = 64 / G;
+  const int64_t waves = (op->basis->dim + rows_per_wave - 1) / rows_per_wave;
+  int64_t nb = (waves + (kApplyBlock / kWave) - 1) / (kApplyBlock / kWave);
+  nb = std::min<int64_t>(nb, 65536);
+  auto s = static_cast<hipStream_t>(stream);
+  if (op->basis->nw == 1)
+    hipLaunchKernelGGL(k_apply_block<1>, dim3(static_cast<uint32_t>(nb)),
+                       dim3(kApplyBlock), 0, s, op->basis->view(), op->d_off,
+                       op->n_off, op->d_diag, op->n_diag, d_X, ldx, d_Y, ldy, R,
+                       d_coeffs, accumulate);
+  else
+    hipLaunchKernelGGL(k_apply_block<2>, dim3(static_cast<uint32_t>(nb)),
+                       dim3(kApplyBlock), 0, s, op->basis->view(), op->d_off,
+                       op->n_off, op->d_diag, op->n_diag, d_X, ldx, d_Y, ldy, R,
+                       d_coeffs, accumulate);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
+int qsi_op_connectivity(qsi_op_t op, int64_t* nnz, int64_t* d_rows,
+                        int64_t* d_cols, double* d_vals, double* d_diag) {
+  QSI_REQUIRE(op != nullptr && nnz != nullptr, "qsi_op_connectivity: null argument");
+  const int64_t D = op->basis->dim;
+  if (D == 0) {
+    *nnz = 0;
+    return QSI_OK;
+  }
+  int64_t *counts = nullptr, *offsets = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  const uint32_t grid = static_cast<uint32_t>((D + 255) / 256);
+  auto cleanup = [&]() {
+    if (counts) (void)hipFree(counts);
+    if (offsets) (void)hipFree(offsets);
+    if (tmp) (void)hipFree(tmp);
+  };
+  hipError_t e = hipMalloc(&counts, static_cast<size_t>(D + 1) * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(&offsets, static_cast<size_t>(D + 1) * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemset(counts, 0, static_cast<size_t>(D + 1) * sizeof(int64_t));
+  if (e != hipSuccess) {
+    cleanup();
+    set_error(std::string("qsi_op_connectivity: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  auto view = op->basis->view();
+  if (op->basis->nw == 1)
+    hipLaunchKernelGGL(k_conn<1>, dim3(grid), dim3(256), 0, 0, view, op->d_off,
+                       op->n_off, op->d_diag, op->n_diag, nullptr, counts,
+                       nullptr, nullptr, nullptr, d_rows ? d_diag : nullptr);
+  else
+    hipLaunchKernelGGL(k_conn<2>, dim3(grid), dim3(256), 0, 0, view, op->d_off,
+                       op->n_off, op->d_diag, op->n_diag, nullptr, counts,
+                       nullptr, nullptr, nullptr, d_rows ? d_diag : nullptr);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, offsets, D + 1);
+  if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
+  if (e == hipSuccess)
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, offsets, D + 1);
+  int64_t total = 0;
+  if (e == hipSuccess)
+    e = hipMemcpy(&total, offsets + D, sizeof(int64_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    cleanup();
+    set_error(std::string("qsi_op_connectivity: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  if (d_rows == nullptr) {
+    *nnz = total;
+    cleanup();
+    return QSI_OK;
+  }
+  if (*nnz < total || !d_cols || !d_vals || !d_diag) {
+    cleanup();
+    set_error("qsi_op_connectivity: output arrays too small or null");
+    return QSI_E_INVALID;
+  }
+  if (op->basis->nw == 1)
+    hipLaunchKernelGGL(k_conn<1>, dim3(grid), dim3(256), 0, 0, view, op->d_off,
+                       op->n_off, op->d_diag, op->n_diag, offsets, nullptr,
+                       d_rows, d_cols, d_vals, d_diag);
+  else
+    hipLaunchKernelGGL(k_conn<2>, dim3(grid), dim3(256), 0, 0, view, op->d_off,
+                       op->n_off, op->d_diag, op->n_diag, offsets, nullptr,
+                       d_rows, d_cols, d_vals, d_diag);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  cleanup();
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_op_connectivity: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  *nnz = total;
+  return QSI_OK;
+}
+
+int qsi_vec_dot(const double* d_a, const double* d_b, int64_t n, double* h_out,
+                qsi_stream_t stream) {
+  QSI_REQUIRE(h_out != nullptr, "qsi_vec_dot: null output");
+  if (n <= 0) {
+    *h_out = 0.0;
+    return QSI_OK;
+  }
+  QSI_REQUIRE(d_a && d_b, "qsi_vec_dot: null vector");
+  auto s = static_cast<hipStream_t>(stream);
+  double* buf = nullptr;
+  QSI_CHECK_HIP(hipMalloc(&buf, (max_vec_partials() + 2) * sizeof(double)));
+  int rc = launch_dot(d_a, d_b, n, buf + 2, buf, s);
+  hipError_t e = hipSuccess;
+  if (rc == QSI_OK) e = hipMemcpyAsync(h_out, buf, sizeof(double), hipMemcpyDeviceToHost, s);
+  if (rc == QSI_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_vec_dot: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  return rc;
+}
+
+int qsi_vec_axpy(double alpha, const double* d_x, double* d_y, int64_t n,
+                 qsi_stream_t stream) {
+  if (n <= 0) return QSI_OK;
+  QSI_REQUIRE(d_x && d_y, "qsi_vec_axpy: null vector");
+  return launch_axpy(alpha, d_x, d_y, n, static_cast<hipStream_t>(stream));
+}
+
+int qsi_vec_scal(double alpha, double* d_x, int64_t n, qsi_stream_t stream) {
+  if (n <= 0) return QSI_OK;
+  QSI_REQUIRE(d_x, "qsi_vec_scal: null vector");
+  return launch_scal(alpha, d_x, n, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

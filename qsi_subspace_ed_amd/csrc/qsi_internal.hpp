@@ -1,0 +1,120 @@
+// Internal objects behind the C ABI handles.
+#pragma once
+
+#include <atomic>
+#include <vector>
+
+#include "qsi_common.hpp"
+
+namespace qsi {
+
+// Off-diagonal term in pull form.  For a row state s the term contributes
+//   c * sgn * (-1)^popcount(~s & Z) * x[rank(s ^ X)]
+// when (s & m) == v.  With s' = s ^ X the pre-image, this is the reference's
+// precondition (s' & down) == 0 && (s' & up) == up rewritten on s:
+//   m = up | down,  v = up ^ (X & m).
+struct OffTerm {
+  uint64_t m[2];
+  uint64_t v[2];
+  uint64_t X[2];
+  uint64_t Z[2];
+  double c;     // coefficient (single-vector apply)
+  double sgn;   // +-1 operator sign
+  int32_t slot; // coefficient slot for block applies (-1: use c)
+  int32_t pad;
+};
+
+// Diagonal term.  kind 0: c * sgn * (-1)^popcount(~s & Z)  (a Z string);
+// kind 1: c * (2 popcount(s & Z) - popcount(Z))  (a sum of single-site Z
+// terms that share one coefficient, merged on the host).
+struct DiagTerm {
+  uint64_t Z[2];
+  double c;
+  double sgn;
+  int32_t kind;
+  int32_t slot;
+};
+
+}  // namespace qsi
+
+struct qsi_basis_s {
+  int64_t dim = 0;
+  int n_spins = 0;
+  int nw = 1;
+  int kbits = 0;
+  int shift = 0;
+  int device = 0;
+  uint64_t* d_states = nullptr;  // dim * nw words
+  uint32_t* d_table = nullptr;   // 2^kbits + 1
+  std::atomic<int> refs{1};
+
+  qsi::BasisView view() const {
+    qsi::BasisView v;
+    v.states = d_states;
+    v.table = d_table;
+    v.dim = dim;
+    v.shift = shift;
+    v.kbits = kbits;
+    return v;
+  }
+};
+
+struct qsi_op_s {
+  qsi_basis_s* basis = nullptr;
+  int n_off = 0;
+  int n_diag = 0;
+  int n_slots = 0;
+  bool has_z = false;  // any off-diagonal term with a Z string
+  qsi::OffTerm* d_off = nullptr;
+  qsi::DiagTerm* d_diag = nullptr;
+  std::vector<qsi::OffTerm> h_off;
+  std::vector<qsi::DiagTerm> h_diag;
+};
+
+namespace qsi {
+
+void basis_retain(qsi_basis_s* b);
+void basis_release(qsi_basis_s* b);
+
+// Launch helpers shared by the solvers (all stream ordered).
+// y = yscale * y_in + H x  (y_in may be null or alias y); if dot_out is
+// non-null, *dot_out (device) = sum_i x[i] * y[i] after the update.
+int launch_apply(const qsi_op_s* op, const double* x, double* y,
+                 const double* y_in, double yscale, double* dot_partials,
+                 double* dot_out, hipStream_t s);
+// as launch_apply, with the y_in scale multiplied by yscale_dev[0] (device)
+int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
+                     const double* y_in, double yscale,
+                     const double* yscale_dev, double* dot_partials,
+                     double* dot_out, hipStream_t s);
+int64_t apply_num_blocks(const qsi_op_s* op);
+
+// streaming vector kernels; partial buffers need max_vec_partials() doubles
+int max_vec_partials();
+int launch_dot(const double* a, const double* b, int64_t n, double* partials,
+               double* out, hipStream_t s);
+int launch_axpy(double alpha, const double* x, double* y, int64_t n,
+                hipStream_t s);
+int launch_scal(double alpha, double* x, int64_t n, hipStream_t s);
+// u -= alpha_dev[0] v; nrm_out[0] = ||u||, nrm_out[1] = 1/||u||
+int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
+                     int64_t n, double* partials, double* nrm_out,
+                     hipStream_t s);
+int launch_scal_dev(const double* s_dev, double* x, int64_t n, hipStream_t s);
+// out[j] = V[:, j] . w for j < m (V column-major, ld ldv); acc_out += out
+int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
+                    int64_t n, double* partials, double* out, double* acc_out,
+                    hipStream_t s);
+// w -= V[:, 0:m] h
+int launch_gemv_sub(const double* V, int64_t ldv, int m, const double* h,
+                    double* w, int64_t n, hipStream_t s);
+// out[:, 0:k] = V[:, 0:m] Y  (Y m x k column-major, device)
+int launch_combine(const double* V, int64_t ldv, int m, const double* Y, int k,
+                   double* out, int64_t ldo, int64_t n, hipStream_t s);
+int launch_copy_scaled(const double* s_dev, const double* y, double* x,
+                       int64_t n, hipStream_t s);
+// y += c * s_dev[0] * x
+int launch_axpy_dev(const double* s_dev, double c, const double* x, double* y,
+                    int64_t n, hipStream_t s);
+
+}  // namespace qsi

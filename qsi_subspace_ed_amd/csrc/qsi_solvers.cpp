@@ -1,0 +1,574 @@
+// Krylov drivers (host control, GPU vectors) and the host basis enumerator.
+//
+//   qsi_lanczos_eigval0  lanczos::eigval0 / lanczos_iterate
+//                        (components/linalg_routines/src/lanczos.cpp:14-235)
+//   qsi_eigsh            thick-restart Lanczos standing in for
+//                        Spectra::SymEigsSolver behind `mfsparse`
+//                        (include/matrix_diag_bits.hpp:51-95)
+//   qsi_enumerate_basis  gen_spinon_basis DFS
+//                        (components/build_basis/src/pyro_tree.cpp:20-240)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "qsi_hostla.hpp"
+#include "qsi_internal.hpp"
+
+namespace qsi {
+namespace {
+
+// RAII device buffer
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 8)); }
+  double* d() const { return static_cast<double*>(p); }
+};
+
+#define QSI_TRY(expr)            \
+  do {                           \
+    const int _rc = (expr);      \
+    if (_rc != QSI_OK) return _rc; \
+  } while (0)
+
+// The reference's start vector: std::mt19937(seed), N(0,1) entries, then
+// divided by the 2-norm (lanczos.cpp:55-62, 80-81).
+std::vector<double> random_unit(int64_t n, uint32_t seed) {
+  std::vector<double> v(static_cast<size_t>(n));
+  std::mt19937 rng(seed);
+  std::normal_distribution<double> dist(0.0, 1.0);
+  for (auto& x : v) x = dist(rng);
+  double s = 0.0;
+  for (double x : v) s += x * x;
+  const double nrm = std::sqrt(s);
+  for (auto& x : v) x /= nrm;
+  return v;
+}
+
+// One Lanczos step on the device.  On entry v holds v_j (normalised), u holds
+// v_{j-1} (ignored when first).  On exit u = beta_j^{-1} (H v_j - alpha_j v_j
+// - beta_{j-1} v_{j-1}) — the next vector — and the caller swaps v and u.
+// alpha_j -> ab[0], beta_j -> ab[1], 1/beta_j -> ab[2]; beta_{j-1} is read
+// from prev_beta (device) when !first.
+int lanczos_step(const qsi_op_s* op, const double* v, double* u, bool first,
+                 const double* prev_beta, double* ab, double* partials,
+                 hipStream_t s) {
+  const int64_t D = op->basis->dim;
+  // u = -beta_{j-1} u + H v;  alpha = <v, u>     (lanczos.cpp:117-126)
+  QSI_TRY(launch_apply_dev(op, v, u, first ? nullptr : u, -1.0,
+                           first ? nullptr : prev_beta, partials, ab, s));
+  // u -= alpha v;  beta = ||u||                   (lanczos.cpp:129-132)
+  QSI_TRY(launch_axpy_norm(ab, v, u, D, partials, ab + 1, s));
+  // v_{j+1} = u / beta                            (lanczos.cpp:136-139)
+  QSI_TRY(launch_scal_dev(ab + 2, u, D, s));
+  return QSI_OK;
+}
+
+struct ConvState {
+  double eigval = std::numeric_limits<double>::max();
+  double error = 0.0;
+  bool converged = false;
+};
+
+// check_lanczos_convergence (lanczos.cpp:14-53) on the leading n entries.
+void check_convergence(const std::vector<double>& a, const std::vector<double>& b,
+                       size_t n, const qsi_lanczos_settings_t& st, ConvState& c) {
+  const double old = c.eigval;
+  c.eigval = hostla::tridiag_lowest(a, b, n);
+  c.error = std::fabs(c.eigval - old);
+  const double rel = c.error / std::max(std::fabs(c.eigval), 1e-10);
+  c.converged = (c.error < st.abs_tol || rel < st.rel_tol);
+}
+
+}  // namespace
+}  // namespace qsi
+
+using namespace qsi;
+
+extern "C" {
+
+int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t* st,
+                        double* eigval, double* d_eigvec,
+                        qsi_lanczos_result_t* result) {
+  QSI_REQUIRE(op && st && eigval, "qsi_lanczos_eigval0: null argument");
+  const int64_t D = op->basis->dim;
+  QSI_REQUIRE(D > 0, "qsi_lanczos_eigval0: v called with size 0");
+  QSI_REQUIRE(!st->calc_eigenvector || d_eigvec,
+              "qsi_lanczos_eigval0: calc_eigenvector needs d_eigvec");
+  QSI_REQUIRE(st->max_iterations >= 0 && st->min_iterations >= 0,
+              "qsi_lanczos_eigval0: negative iteration bounds");
+  const int64_t max_it = st->max_iterations;
+  const size_t bytes = static_cast<size_t>(D) * sizeof(double);
+  DevBuf bv, bu, bpart, bab;
+  QSI_CHECK_HIP(bv.alloc(bytes));
+  QSI_CHECK_HIP(bu.alloc(bytes));
+  QSI_CHECK_HIP(bpart.alloc(max_vec_partials() * sizeof(double)));
+  const int64_t chunk = 64;
+  // per step: alpha, beta, 1/beta
+  QSI_CHECK_HIP(bab.alloc(static_cast<size_t>(3 * std::max<int64_t>(max_it, 1)) * sizeof(double)));
+  hipStream_t s = nullptr;
+  const std::vector<double> v0 = random_unit(D, st->x0_seed);
+
+  // ---- pass 1: the recurrence; stops at max_iterations or beta < abs_tol
+  // (lanczos.cpp:111-160; pass 1 never stops on convergence)
+  std::vector<double> alphas, betas;
+  std::vector<double> hab;
+  int64_t n_loop = max_it;  // value of the loop counter at exit
+  {
+    double* v = bv.d();
+    double* u = bu.d();
+    QSI_CHECK_HIP(hipMemcpy(v, v0.data(), bytes, hipMemcpyHostToDevice));
+    bool stop = false;
+    for (int64_t it0 = 0; it0 < max_it && !stop; it0 += chunk) {
+      const int64_t it1 = std::min(max_it, it0 + chunk);
+      for (int64_t it = it0; it < it1; ++it) {
+        QSI_TRY(lanczos_step(op, v, u, it == 0, bab.d() + 3 * (it - 1) + 1,
+                             bab.d() + 3 * it, bpart.d(), s));
+        std::swap(v, u);
+      }
+      hab.resize(static_cast<size_t>(3 * (it1 - it0)));
+      QSI_CHECK_HIP(hipMemcpyAsync(hab.data(), bab.d() + 3 * it0,
+                                   hab.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+      QSI_CHECK_HIP(hipStreamSynchronize(s));
+      for (int64_t it = it0; it < it1; ++it) {
+        const double a = hab[3 * (it - it0)], b = hab[3 * (it - it0) + 1];
+        if (!(b >= st->abs_tol)) {  // beta < abs_tol (or NaN): break
+          n_loop = it;
+          stop = true;
+          break;
+        }
+        alphas.push_back(a);
+        betas.push_back(b);
+      }
+    }
+  }
+  const size_t n1 = alphas.size();
+  qsi_lanczos_result_t res{};
+  // tridiagonalise_one after pass 1 (lanczos.cpp:209-211)
+  std::vector<double> ritz;
+  if (n1 > 0) {
+    *eigval = hostla::tridiag_lowest(alphas, betas, n1);
+    ritz = hostla::tridiag_eigvec(alphas, betas, n1, *eigval);
+  } else {
+    // the Krylov space is one-dimensional: v0 is an eigenvector with the
+    // Rayleigh quotient alpha_0 (the reference reads an empty vector here)
+    *eigval = hab.empty() ? 0.0 : hab[0];
+  }
+
+  if (!st->calc_eigenvector) {
+    // pass-1 Result: the last convergence check (lanczos.cpp:149-159)
+    res.n_iterations = n_loop;
+    const int64_t mi = st->min_iterations;
+    if (n1 > 0 && static_cast<int64_t>(n1) - 1 >= mi) {
+      ConvState c;
+      if (static_cast<int64_t>(n1) - 2 >= mi) check_convergence(alphas, betas, n1 - 1, *st, c);
+      check_convergence(alphas, betas, n1, *st, c);
+      res.eigval_converged = c.converged;
+      res.eigval_error = c.error;
+    }
+    if (result) *result = res;
+    return QSI_OK;
+  }
+
+  // ---- pass 2: same start vector, accumulate eigvec += ritz[j] v_j until
+  // the per-step check converges (lanczos.cpp:112-115,155-158).  The device
+  // recurrence is deterministic, so pass 2 reproduces pass 1's coefficients
+  // and its stopping iteration is found on the host first.
+  ConvState c;
+  int64_t stop_it = -1;  // iteration at which pass 2 breaks on convergence
+  for (size_t k = 0; k < n1; ++k) {
+    if (static_cast<int64_t>(k) >= st->min_iterations) {
+      check_convergence(alphas, betas, k + 1, *st, c);
+      if (c.converged) {
+        stop_it = static_cast<int64_t>(k);
+        break;
+      }
+    }
+  }
+  // last iteration whose top-of-loop accumulation runs
+  const int64_t last_acc = stop_it >= 0 ? stop_it : std::min<int64_t>(n_loop, max_it - 1);
+  {
+    double* v = bv.d();
+    double* u = bu.d();
+    QSI_CHECK_HIP(hipMemcpy(v, v0.data(), bytes, hipMemcpyHostToDevice));
+    QSI_CHECK_HIP(hipMemsetAsync(d_eigvec, 0, bytes, s));
+    for (int64_t it = 0; it <= last_acc; ++it) {
+      if (it < static_cast<int64_t>(ritz.size()))
+        QSI_TRY(launch_axpy(ritz[it], v, d_eigvec, D, s));
+      if (it == last_acc) break;
+      QSI_TRY(lanczos_step(op, v, u, it == 0, bab.d() + 3 * (it - 1) + 1,
+                           bab.d() + 3 * it, bpart.d(), s));
+      std::swap(v, u);
+    }
+    QSI_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  res.eigval_converged = c.converged;
+  res.eigvec_converged = stop_it >= 0;
+  res.n_iterations = stop_it >= 0 ? stop_it : n_loop;
+  res.eigval_error = c.error;
+  if (st->verify_eigenvector) {
+    // u = H eigvec - eigval * eigvec; error = ||u|| (lanczos.cpp:163-172)
+    double* u = bu.d();
+    QSI_TRY(launch_apply(op, d_eigvec, u, nullptr, 0.0, nullptr, nullptr, s));
+    QSI_TRY(launch_axpy(-c.eigval, d_eigvec, u, D, s));
+    DevBuf out;
+    QSI_CHECK_HIP(out.alloc(2 * sizeof(double)));
+    QSI_TRY(launch_dot(u, u, D, bpart.d(), out.d(), s));
+    double nn = 0.0;
+    QSI_CHECK_HIP(hipMemcpy(&nn, out.d(), sizeof(double), hipMemcpyDeviceToHost));
+    res.eigvec_error = std::sqrt(nn);
+  }
+  if (result) *result = res;
+  return QSI_OK;
+}
+
+int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
+              const double* d_v0, uint64_t seed, double* h_evals,
+              double* d_evecs, int n_evecs, int* nconv_out, int* n_applies) {
+  QSI_REQUIRE(op && h_evals, "qsi_eigsh: null argument");
+  const int64_t D = op->basis->dim;
+  QSI_REQUIRE(D > 0, "qsi_eigsh: empty basis");
+  QSI_REQUIRE(nev >= 1 && ncv >= 2, "qsi_eigsh: need nev >= 1, ncv >= 2");
+  QSI_REQUIRE(ncv <= 63, "qsi_eigsh: ncv must be <= 63");
+  QSI_REQUIRE(n_evecs >= 0 && (n_evecs == 0 || d_evecs), "qsi_eigsh: bad eigenvector output");
+  // Spectra's sizing (matrix_diag_bits.hpp:64-66): ncv <= n, nev <= ncv - 1
+  ncv = static_cast<int>(std::min<int64_t>(ncv, D));
+  nev = std::min(nev, std::max(ncv - 1, 1));
+  n_evecs = std::min(n_evecs, nev);
+  const int64_t ld = (D + 63) / 64 * 64;
+  hipStream_t s = nullptr;
+  DevBuf bV, bW, bw, bpart, bh, bscal;
+  QSI_CHECK_HIP(bV.alloc(static_cast<size_t>(ld) * (ncv + 1) * sizeof(double)));
+  QSI_CHECK_HIP(bW.alloc(static_cast<size_t>(ld) * (ncv + 1) * sizeof(double)));
+  QSI_CHECK_HIP(bw.alloc(static_cast<size_t>(D) * sizeof(double)));
+  QSI_CHECK_HIP(bpart.alloc(max_vec_partials() * sizeof(double)));
+  QSI_CHECK_HIP(bh.alloc(3 * 64 * sizeof(double)));
+  QSI_CHECK_HIP(bscal.alloc(4 * sizeof(double)));
+  double* V = bV.d();
+  double* W = bW.d();
+  double* w = bw.d();
+  double* h = bh.d();
+  double* h2 = bh.d() + 64;
+  int applies = 0;
+
+  // start vector
+  if (d_v0) {
+    QSI_CHECK_HIP(hipMemcpyAsync(V, d_v0, static_cast<size_t>(D) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+  } else {
+    const std::vector<double> v0 = random_unit(D, static_cast<uint32_t>(seed));
+    QSI_CHECK_HIP(hipMemcpy(V, v0.data(), static_cast<size_t>(D) * sizeof(double),
+                            hipMemcpyHostToDevice));
+  }
+  double nrm2 = 0.0;
+  QSI_TRY(launch_dot(V, V, D, bpart.d(), bscal.d(), s));
+  QSI_CHECK_HIP(hipMemcpy(&nrm2, bscal.d(), sizeof(double), hipMemcpyDeviceToHost));
+  QSI_REQUIRE(nrm2 > 0.0, "qsi_eigsh: zero start vector");
+  QSI_TRY(launch_scal(1.0 / std::sqrt(nrm2), V, D, s));
+
+  std::vector<double> T(static_cast<size_t>(ncv) * ncv, 0.0);
+  auto Tat = [&](int i, int j) -> double& { return T[static_cast<size_t>(j) * ncv + i]; };
+  std::vector<double> theta, Y;
+  int k = 0;       // kept (locked + thick) vectors
+  int nconv = 0;
+  int m = ncv;     // current basis size
+  double beta_last = 0.0;
+  bool exhausted = false;
+  std::vector<double> hh(64);
+  for (int restart = 0;; ++restart) {
+    m = ncv;
+    for (int j = k; j < ncv; ++j) {
+      double* vj = V + static_cast<int64_t>(j) * ld;
+      QSI_TRY(launch_apply(op, vj, w, nullptr, 0.0, nullptr, nullptr, s));
+      ++applies;
+      // two classical Gram-Schmidt passes against v_0..v_j
+      QSI_TRY(launch_multidot(V, ld, j + 1, w, D, bpart.d(), h, nullptr, s));
+      QSI_TRY(launch_gemv_sub(V, ld, j + 1, h, w, D, s));
+      QSI_TRY(launch_multidot(V, ld, j + 1, w, D, bpart.d(), h2, h, s));
+      QSI_TRY(launch_gemv_sub(V, ld, j + 1, h2, w, D, s));
+      QSI_TRY(launch_dot(w, w, D, bpart.d(), bscal.d(), s));
+      QSI_CHECK_HIP(hipMemcpyAsync(hh.data(), h, (j + 1) * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+      double bb = 0.0;
+      QSI_CHECK_HIP(hipMemcpyAsync(&bb, bscal.d(), sizeof(double), hipMemcpyDeviceToHost, s));
+      QSI_CHECK_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i <= j; ++i) {
+        if (i < k && j == k) {
+          Tat(i, j) = Tat(j, i) = hh[i];  // coupling to the kept Ritz vectors
+        } else if (i >= k) {
+          Tat(i, j) = Tat(j, i) = hh[i];
+        }
+      }
+      const double beta = std::sqrt(std::max(bb, 0.0));
+      double tnorm = 0.0;
+      for (int i = 0; i <= j; ++i) tnorm = std::max(tnorm, std::fabs(Tat(i, j)));
+      if (beta <= 1e-13 * std::max(tnorm, 1e-300)) {
+        // invariant subspace: the projected matrix is exact
+        m = j + 1;
+        beta_last = 0.0;
+        exhausted = true;
+        break;
+      }
+      beta_last = beta;
+      double* vn = V + static_cast<int64_t>(j + 1) * ld;
+      QSI_CHECK_HIP(hipMemcpyAsync(vn, w, static_cast<size_t>(D) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+      QSI_TRY(launch_scal(1.0 / beta, vn, D, s));
+    }
+    // Rayleigh-Ritz on the m x m projected matrix
+    std::vector<double> Tm(static_cast<size_t>(m) * m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) Tm[static_cast<size_t>(j) * m + i] = Tat(i, j);
+    hostla::sym_eig(Tm, m, theta, Y);
+    const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+    const int want = std::min(nev, m);
+    nconv = 0;
+    for (int i = 0; i < want; ++i) {
+      const double resid = beta_last * std::fabs(Y[static_cast<size_t>(i) * m + (m - 1)]);
+      if (resid < tol * std::max(eps23, std::fabs(theta[i]))) ++nconv;
+      else break;  // count the leading converged run
+    }
+    if (nconv >= want || exhausted || restart >= max_restarts) break;
+    // thick restart: keep the k lowest Ritz vectors plus the residual vector
+    int kn = nev + std::min(nconv, (m - nev) / 2);
+    kn = std::max(1, std::min(kn, m - 1));
+    std::vector<double> Yk(Y.begin(), Y.begin() + static_cast<size_t>(kn) * m);
+    DevBuf bY;
+    QSI_CHECK_HIP(bY.alloc(Yk.size() * sizeof(double)));
+    QSI_CHECK_HIP(hipMemcpyAsync(bY.d(), Yk.data(), Yk.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, s));
+    QSI_TRY(launch_combine(V, ld, m, bY.d(), kn, W, ld, D, s));
+    QSI_CHECK_HIP(hipMemcpyAsync(W + static_cast<int64_t>(kn) * ld,
+                                 V + static_cast<int64_t>(m) * ld,
+                                 static_cast<size_t>(D) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+    QSI_CHECK_HIP(hipStreamSynchronize(s));
+    std::swap(V, W);
+    std::fill(T.begin(), T.end(), 0.0);
+    for (int i = 0; i < kn; ++i) Tat(i, i) = theta[i];
+    k = kn;
+  }
+  const int nout = std::min(nev, m);
+  for (int i = 0; i < nev; ++i)
+    h_evals[i] = i < nout ? theta[i] : std::numeric_limits<double>::quiet_NaN();
+  if (n_evecs > 0) {
+    const int ne = std::min(n_evecs, m);
+    std::vector<double> Yk(Y.begin(), Y.begin() + static_cast<size_t>(ne) * m);
+    DevBuf bY;
+    QSI_CHECK_HIP(bY.alloc(Yk.size() * sizeof(double)));
+    QSI_CHECK_HIP(hipMemcpyAsync(bY.d(), Yk.data(), Yk.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, s));
+    QSI_TRY(launch_combine(V, ld, m, bY.d(), ne, d_evecs, D, D, s));
+    QSI_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  // hand the original buffer ownership back so both are freed
+  if (V != bV.d()) std::swap(bV.p, bW.p);
+  if (nconv_out) *nconv_out = nconv;
+  if (n_applies) *n_applies = applies;
+  if (nconv < std::min(nev, m)) {
+    set_error("qsi_eigsh: not all requested eigenpairs converged");
+    return QSI_E_NOCONV;
+  }
+  return QSI_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Basis enumeration
+// ---------------------------------------------------------------------------
+
+namespace {
+
+using u128 = unsigned __int128;
+
+struct EnumLattice {
+  int n = 0;
+  std::vector<u128> tmask;           // tetra bitmask in visiting order
+  std::vector<int> tmin, tmax, tsize;
+  std::vector<int> nb0, nb1;         // first two tetras of each (visited) spin
+  unsigned pairs = 0;
+};
+
+struct Node {
+  u128 state;
+  unsigned spin;
+  unsigned pairs;
+};
+
+// possible_spin_states (pyro_tree.cpp:20-81): bit 0 = down allowed, bit 1 =
+// up allowed, judged on the spin's two tetrahedra with the spins visited so
+// far.
+inline int possible(const EnumLattice& L, const Node& c) {
+  int res = 3;
+  const u128 known = (static_cast<u128>(1) << c.spin) - 1;
+  for (int ud = 0; ud < 2; ++ud) {
+    const u128 st = ud ? (c.state | (static_cast<u128>(1) << c.spin)) : c.state;
+    for (int k = 0; k < 2; ++k) {
+      const int t = k == 0 ? L.nb0[c.spin] : L.nb1[c.spin];
+      const u128 tm = L.tmask[t];
+      const int q = __builtin_popcountll(static_cast<uint64_t>(st & tm)) +
+                    __builtin_popcountll(static_cast<uint64_t>((st & tm) >> 64));
+      const u128 km = tm & known;
+      const int nk = __builtin_popcountll(static_cast<uint64_t>(km)) +
+                     __builtin_popcountll(static_cast<uint64_t>(km >> 64)) + 1;
+      const int nu = L.tsize[t] - nk;
+      if (q + nu < L.tmin[t] || q > L.tmax[t]) {
+        res &= ~(1 << ud);
+        break;
+      }
+    }
+  }
+  return res;
+}
+
+// fork_state_impl (pyro_tree.cpp:168-189)
+template <typename Push>
+inline void fork(const EnumLattice& L, const Node& c, Push&& push) {
+  const int poss = possible(L, c);
+  const bool may_pair = c.pairs < L.pairs;
+  const u128 up = static_cast<u128>(1) << c.spin;
+  if (poss & 1) push(Node{c.state, c.spin + 1, c.pairs});
+  else if (may_pair) push(Node{c.state, c.spin + 1, c.pairs + 1});
+  if (poss & 2) push(Node{c.state | up, c.spin + 1, c.pairs});
+  else if (may_pair) push(Node{c.state | up, c.spin + 1, c.pairs + 1});
+}
+
+void dfs(const EnumLattice& L, Node root, std::vector<u128>& out) {
+  std::vector<Node> stack;
+  stack.push_back(root);
+  while (!stack.empty()) {
+    const Node c = stack.back();
+    stack.pop_back();
+    if (static_cast<int>(c.spin) == L.n) {
+      out.push_back(c.state);
+      continue;
+    }
+    fork(L, c, [&](const Node& x) { stack.push_back(x); });
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsi_enumerate_basis(int n_spins, int n_tetra, const int32_t* tetra_members,
+                        const int32_t* order, int n_spinon_pairs, int n_threads,
+                        uint64_t** out_lohi, int64_t* out_dim) {
+  QSI_REQUIRE(out_lohi && out_dim, "qsi_enumerate_basis: null output");
+  *out_lohi = nullptr;
+  *out_dim = 0;
+  QSI_REQUIRE(n_spins >= 1 && n_spins <= 128, "qsi_enumerate_basis: n_spins must be in [1,128]");
+  QSI_REQUIRE(n_tetra >= 1 && tetra_members && order, "qsi_enumerate_basis: null lattice");
+  QSI_REQUIRE(n_spinon_pairs >= 0, "qsi_enumerate_basis: n_spinon_pairs < 0");
+  // iperm: original spin id -> visiting position (lattice::apply_permutation)
+  std::vector<int> iperm(n_spins, -1);
+  for (int k = 0; k < n_spins; ++k) {
+    const int sid = order[k];
+    QSI_REQUIRE(sid >= 0 && sid < n_spins && iperm[sid] < 0,
+                "qsi_enumerate_basis: order is not a permutation");
+    iperm[sid] = k;
+  }
+  EnumLattice L;
+  L.n = n_spins;
+  L.pairs = static_cast<unsigned>(n_spinon_pairs);
+  L.tmask.resize(n_tetra);
+  L.tmin.resize(n_tetra);
+  L.tmax.resize(n_tetra);
+  L.tsize.resize(n_tetra);
+  std::vector<std::vector<int>> nbs(n_spins);
+  for (int t = 0; t < n_tetra; ++t) {
+    u128 m = 0;
+    for (int q = 0; q < 4; ++q) {
+      const int sid = tetra_members[4 * t + q];
+      QSI_REQUIRE(sid >= 0 && sid < n_spins, "qsi_enumerate_basis: tetra member out of range");
+      m |= static_cast<u128>(1) << iperm[sid];
+      nbs[iperm[sid]].push_back(t);
+    }
+    L.tmask[t] = m;
+    L.tsize[t] = 4;
+    L.tmin[t] = 4 / 2;        // tetra default rules (tetra_graph_io.hpp:45-46)
+    L.tmax[t] = (4 + 1) / 2;
+  }
+  L.nb0.resize(n_spins);
+  L.nb1.resize(n_spins);
+  for (int i = 0; i < n_spins; ++i) {
+    QSI_REQUIRE(nbs[i].size() >= 2, "qsi_enumerate_basis: every spin needs two tetrahedra");
+    L.nb0[i] = nbs[i][0];
+    L.nb1[i] = nbs[i][1];
+  }
+  // breadth-first frontier, then independent depth-first subtrees
+  std::vector<Node> front{Node{0, 0, 0}};
+  const size_t target = static_cast<size_t>(std::max(1, n_threads)) * 64;
+  while (front.size() < target) {
+    std::vector<Node> next;
+    bool any = false;
+    for (const Node& c : front) {
+      if (static_cast<int>(c.spin) == L.n) {
+        next.push_back(c);
+        continue;
+      }
+      any = true;
+      fork(L, c, [&](const Node& x) { next.push_back(x); });
+    }
+    front.swap(next);
+    if (!any) break;
+  }
+  const int nt = std::max(1, n_threads);
+  std::vector<std::vector<u128>> parts(nt);
+  std::atomic<size_t> next_node{0};
+  auto worker = [&](int tid) {
+    for (;;) {
+      const size_t i = next_node.fetch_add(1);
+      if (i >= front.size()) break;
+      dfs(L, front[i], parts[tid]);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
+  worker(0);
+  for (auto& th : pool) th.join();
+  size_t total = 0;
+  for (auto& p : parts) total += p.size();
+  std::vector<u128> all;
+  all.reserve(total);
+  for (auto& p : parts) {
+    all.insert(all.end(), p.begin(), p.end());
+    std::vector<u128>().swap(p);
+  }
+  // undo the visiting permutation: bit k -> bit order[k] (bittools permute)
+  for (auto& st : all) {
+    u128 y = 0;
+    for (int kk = 0; kk < n_spins; ++kk)
+      if ((st >> kk) & 1) y |= static_cast<u128>(1) << order[kk];
+    st = y;
+  }
+  std::sort(all.begin(), all.end());
+  auto* outp = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, all.size()) * 16));
+  if (!outp) {
+    set_error("qsi_enumerate_basis: host allocation failed");
+    return QSI_E_NOMEM;
+  }
+  for (size_t i = 0; i < all.size(); ++i) {
+    outp[2 * i] = static_cast<uint64_t>(all[i]);
+    outp[2 * i + 1] = static_cast<uint64_t>(all[i] >> 64);
+  }
+  *out_lohi = outp;
+  *out_dim = static_cast<int64_t>(all.size());
+  return QSI_OK;
+}
+
+void qsi_free(void* p) { std::free(p); }
+
+}  // extern "C"

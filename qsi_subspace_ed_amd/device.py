@@ -1,0 +1,321 @@
+"""Python face of the GPU library: device basis, the LazyOpSum-shaped operator
+and the Krylov solvers, all calling the C ABI (include/qsi_gpu.h).
+
+Device vectors are torch tensors on `cuda:0` (float64); torch is only used for
+device memory and the current stream.  The reference's interfaces mirrored:
+  LazyOpSum  evaluate / evaluate_add / rows / cols / toSparseMatrix
+             (components/operator/include/operator_matrix.hpp:10-118)
+  LazyOpSumProd::perform_op      (operator_matrix.hpp:120-139)
+  lanczos::eigval0 + Settings    (components/linalg_routines/include/lanczos.hpp)
+  compute_spectrum_iterative     (include/matrix_diag_bits.hpp:51-95)
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _require_cuda_f64(t, n=None, name="vector"):
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+            and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous float64 CUDA tensor")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{name} has {t.numel()} entries, expected {n}")
+
+
+def as_lohi(states):
+    """(D, 2) uint64 [lo, hi] array from an array of states (uint64 (D,) or
+    (D, 2), or a list of Python ints)."""
+    a = np.asarray(states)
+    if a.dtype == object or (a.ndim == 1 and a.dtype != np.uint64):
+        v = [int(x) for x in states]
+        out = np.empty((len(v), 2), dtype=np.uint64)
+        out[:, 0] = [x & ((1 << 64) - 1) for x in v]
+        out[:, 1] = [x >> 64 for x in v]
+        return out
+    if a.ndim == 1:
+        out = np.zeros((a.shape[0], 2), dtype=np.uint64)
+        out[:, 0] = a
+        return out
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+class GpuBasis:
+    """Sorted Z-basis resident in GPU memory with its top-bit bucket table
+    (replaces ZBasisInterp, components/operator/include/operator.hpp:197-204)."""
+
+    def __init__(self, states, n_spins, bucket_bits=0):
+        lib = _lib.load()
+        lohi = as_lohi(states)
+        self._states_host = lohi
+        h = C.c_void_p()
+        check(lib.qsi_basis_create(lohi.ctypes.data_as(C.c_void_p), lohi.shape[0],
+                                   int(n_spins), int(bucket_bits), C.byref(h)),
+              "qsi_basis_create")
+        self._h = h
+        self.n_spins = int(n_spins)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def dim(self):
+        return int(_lib.load().qsi_basis_dim(self._h))
+
+    def bucket_bits(self):
+        return int(_lib.load().qsi_basis_bucket_bits(self._h))
+
+    def states(self):
+        return self._states_host
+
+    def search(self, queries_lohi):
+        """Device lookup: int64 ranks (-1 where absent) of (n, 2) uint64 states
+        given as a CUDA int64 tensor view or numpy array."""
+        torch = _torch()
+        q = as_lohi(queries_lohi)
+        dq = torch.from_numpy(q.view(np.int64)).cuda()
+        out = torch.empty(q.shape[0], dtype=torch.int64, device="cuda")
+        check(_lib.load().qsi_basis_search(self._h, _ptr(dq), q.shape[0], _ptr(out), _stream()),
+              "qsi_basis_search")
+        return out.cpu().numpy()
+
+    def close(self):
+        if self._h:
+            _lib.load().qsi_basis_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Settings:
+    """lanczos::Settings (components/linalg_routines/include/lanczos.hpp:9-20)."""
+    krylov_dim: int = 10
+    abs_tol: float = 1e-8
+    rel_tol: float = 1e-8
+    min_iterations: int = 30
+    max_iterations: int = 5000
+    convergence_check_interval: int = 5
+    verbosity: int = 0
+    x0_seed: int = 10
+    calc_eigenvector: bool = False
+    verify_eigenvector: bool = False
+
+    def to_c(self):
+        s = _lib.LanczosSettings()
+        s.krylov_dim = self.krylov_dim
+        s.abs_tol = self.abs_tol
+        s.rel_tol = self.rel_tol
+        s.min_iterations = self.min_iterations
+        s.max_iterations = self.max_iterations
+        s.x0_seed = self.x0_seed & 0xFFFFFFFF
+        s.calc_eigenvector = int(self.calc_eigenvector)
+        s.verify_eigenvector = int(self.verify_eigenvector)
+        return s
+
+
+@dataclass
+class Result:
+    eigval_converged: bool
+    eigvec_converged: bool
+    n_iterations: int
+    eigval_error: float
+    eigvec_error: float
+
+
+class GpuOp:
+    """y = H x over a GpuBasis (replaces LazyOpSum<double, ZBasisInterp> and
+    LazyOpSumProd).  Borrows the basis, copies the terms."""
+
+    def __init__(self, basis: GpuBasis, H, n_slots=None):
+        lib = _lib.load()
+        arr, n = H.terms()
+        ns = H.n_slots() if n_slots is None else int(n_slots)
+        h = C.c_void_p()
+        check(lib.qsi_op_create(basis.handle, arr, n, ns, C.byref(h)), "qsi_op_create")
+        self._h = h
+        self.basis = basis
+        self.n_slots = ns
+        self._n = basis.dim()
+
+    # -- Eigen / Spectra glue
+    def rows(self):
+        return self._n
+
+    def cols(self):
+        return self._n
+
+    def evaluate(self, x, y):
+        """y = H x (LazyOpSum::evaluate)."""
+        self._apply(x, y, 0)
+
+    def evaluate_add(self, x, y):
+        """y += H x (LazyOpSum::evaluate_add; the RealApplyFn contract)."""
+        self._apply(x, y, 1)
+
+    def perform_op(self, x_in, y_out):
+        """Spectra MatOp: y_out = H x_in (LazyOpSumProd::perform_op)."""
+        self._apply(x_in, y_out, 0)
+
+    def _apply(self, x, y, acc):
+        _require_cuda_f64(x, self._n, "x")
+        _require_cuda_f64(y, self._n, "y")
+        check(_lib.load().qsi_op_apply(self._h, _ptr(x), _ptr(y), acc, _stream()), "qsi_op_apply")
+
+    def evaluate_host(self, x, y=None, accumulate=False):
+        """Host-vector version (copies through PCIe)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if y is None:
+            y = np.zeros(self._n)
+        check(_lib.load().qsi_op_apply_host(self._h, x.ctypes.data_as(C.c_void_p),
+                                            y.ctypes.data_as(C.c_void_p), int(accumulate)),
+              "qsi_op_apply_host")
+        return y
+
+    def apply_block(self, X, Y, coeffs=None, accumulate=False):
+        """Y[:, r] (+)= H_r X[:, r]; X, Y CUDA float64 (D, R) row-major.
+        coeffs: CUDA float64 (n_slots, R) per-column slot coefficients."""
+        torch = _torch()
+        if X.dim() != 2 or Y.shape != X.shape:
+            raise ValueError("X and Y must be (D, R) with the same shape")
+        D, R = X.shape
+        if D != self._n:
+            raise ValueError("row count mismatch")
+        for t, nm in ((X, "X"), (Y, "Y")):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.stride(1) == 1):
+                raise TypeError(f"{nm} must be row-major float64 CUDA")
+        cp = None
+        if coeffs is not None:
+            if tuple(coeffs.shape) != (self.n_slots, R):
+                raise ValueError("coeffs must be (n_slots, R)")
+            coeffs = coeffs.contiguous()
+            cp = _ptr(coeffs)
+        check(_lib.load().qsi_op_apply_block(self._h, _ptr(X), X.stride(0), _ptr(Y), Y.stride(0),
+                                             R, cp, int(accumulate), _stream()),
+              "qsi_op_apply_block")
+
+    def connectivity(self):
+        """(rows, cols, vals, diag) numpy arrays: off-diagonal H[row, col] in
+        row-major order and the diagonal."""
+        torch = _torch()
+        lib = _lib.load()
+        torch.cuda.synchronize()
+        nnz = C.c_int64(0)
+        check(lib.qsi_op_connectivity(self._h, C.byref(nnz), None, None, None, None),
+              "qsi_op_connectivity")
+        n = max(1, nnz.value)
+        r = torch.empty(n, dtype=torch.int64, device="cuda")
+        c = torch.empty(n, dtype=torch.int64, device="cuda")
+        v = torch.empty(n, dtype=torch.float64, device="cuda")
+        d = torch.empty(max(1, self._n), dtype=torch.float64, device="cuda")
+        check(lib.qsi_op_connectivity(self._h, C.byref(nnz), _ptr(r), _ptr(c), _ptr(v), _ptr(d)),
+              "qsi_op_connectivity")
+        k = nnz.value
+        return (r[:k].cpu().numpy(), c[:k].cpu().numpy(), v[:k].cpu().numpy(),
+                d[:self._n].cpu().numpy())
+
+    # -- solvers
+    def lanczos_eigval0(self, settings: Settings):
+        """lanczos::eigval0 on the GPU: (E0, eigvec tensor or None, Result)."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        ev = C.c_double(0.0)
+        vec = torch.zeros(self._n, dtype=torch.float64, device="cuda") \
+            if settings.calc_eigenvector else None
+        res = _lib.LanczosResult()
+        st = settings.to_c()
+        check(_lib.load().qsi_lanczos_eigval0(self._h, C.byref(st), C.byref(ev),
+                                              _ptr(vec) if vec is not None else None,
+                                              C.byref(res)), "qsi_lanczos_eigval0")
+        return ev.value, vec, Result(bool(res.eigval_converged), bool(res.eigvec_converged),
+                                     int(res.n_iterations), float(res.eigval_error),
+                                     float(res.eigvec_error))
+
+    def eigsh(self, nev=5, ncv=15, tol=1e-10, max_restarts=1000, v0=None, seed=0,
+              n_evecs=0, raise_on_noconv=True):
+        """Lowest nev eigenpairs (SmallestAlge) by thick-restart Lanczos.
+        Returns (evals numpy, evecs CUDA (D, n_evecs) column-major view or
+        None, nconv, n_applies)."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        ev = (C.c_double * nev)()
+        vecs = None
+        if n_evecs:
+            vecs = torch.zeros((n_evecs, self._n), dtype=torch.float64, device="cuda")
+        if v0 is not None:
+            _require_cuda_f64(v0, self._n, "v0")
+        nconv = C.c_int(0)
+        napp = C.c_int(0)
+        rc = _lib.load().qsi_eigsh(self._h, nev, ncv, tol, max_restarts,
+                                   _ptr(v0) if v0 is not None else None, seed, ev,
+                                   _ptr(vecs) if vecs is not None else None, n_evecs,
+                                   C.byref(nconv), C.byref(napp))
+        if rc != _lib.QSI_E_NOCONV or raise_on_noconv:
+            check(rc, "qsi_eigsh")
+        out = np.array(list(ev))
+        return out, (vecs.T if vecs is not None else None), nconv.value, napp.value
+
+    def close(self):
+        if self._h:
+            _lib.load().qsi_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def enumerate_basis(jdata, n_spinon_pairs=0, order="greedy", n_threads=None):
+    """Sorted ice-rule basis of a lattice as a (D, 2) uint64 [lo, hi] array
+    (gen_spinon_basis, components/build_basis/gen_spinon_basis.cpp:16-156)."""
+    from .lattice import greedy_spin_ordering
+    W = len(jdata["atoms"])
+    tet = np.ascontiguousarray([t["member_spin_idx"] for t in jdata["tetrahedra"]],
+                               dtype=np.int32)
+    if order == "greedy":
+        perm = greedy_spin_ordering(jdata, 0)
+    elif order == "none":
+        perm = list(range(W))
+    else:
+        perm = list(order)
+    perm = np.ascontiguousarray(perm, dtype=np.int32)
+    if n_threads is None:
+        n_threads = min(16, os.cpu_count() or 1)
+    out = C.POINTER(C.c_uint64)()
+    dim = C.c_int64(0)
+    lib = _lib.load()
+    check(lib.qsi_enumerate_basis(W, tet.shape[0], tet.ctypes.data_as(C.c_void_p),
+                                  perm.ctypes.data_as(C.c_void_p), int(n_spinon_pairs),
+                                  int(n_threads), C.byref(out), C.byref(dim)),
+          "qsi_enumerate_basis")
+    try:
+        n = dim.value
+        arr = np.ctypeslib.as_array(out, shape=(max(n, 1) * 2,))[: n * 2].copy().reshape(n, 2)
+    finally:
+        lib.qsi_free(C.cast(out, C.c_void_p))
+    return arr

@@ -1,0 +1,205 @@
+"""GPU parity of the device basis lookup, the matrix-free apply (single vector
+and block) and the connectivity materialiser against the oracle (the
+reference's push algorithm on the CPU) on the fixture bases.
+
+Tolerances: ranks and (row, col) connectivity bit-exact; matrix elements
+exact (products of a coefficient and +-1); H·v within 1e-13 relative 2-norm.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_basis
+from oracle import pyoracle as orc
+import qsi_subspace_ed_amd as q
+
+pytestmark = pytest.mark.gpu
+
+HV_TOL = 1e-13
+
+
+def _ham(jd, W, kind):
+    if kind == "ring":
+        return q.build_hamiltonian_g(jd, [-1.0, -0.55, -0.3, -0.8])
+    if kind == "ring+field":
+        H = q.build_hamiltonian_g(jd, [-0.9, -0.25, -0.6, -0.45])
+        q.add_local_field(H, jd, [0.07, -0.02, 0.05])
+        return H
+    if kind == "jpm":
+        return q.build_hamiltonian_jpm(jd, 0.02, [0.1, 0.1, 0.1])
+    raise ValueError(kind)
+
+
+def _rand_x(D, seed, tiny=True):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(D)
+    if tiny:  # exercise the |x| <= 1e-15 skip
+        x[rng.integers(0, D, size=max(1, D // 50))] = 3e-16
+    return x
+
+
+@pytest.mark.parametrize("W", [8, 32, 48])
+@pytest.mark.parametrize("kbits", [0, 1, 5])
+def test_search_ranks(W, kbits, cuda):
+    b = load_basis(W)
+    basis = q.GpuBasis(b, W, bucket_bits=min(kbits, W))
+    assert np.array_equal(basis.search(b), np.arange(b.shape[0]))
+    rng = np.random.default_rng(W)
+    members = set(int(x) for x in b[:, 0])
+    cand = rng.integers(0, 1 << W if W < 64 else (1 << 63), size=4000, dtype=np.uint64)
+    cand = np.array([c for c in cand if int(c) not in members][:2000], dtype=np.uint64)
+    qs = np.zeros((cand.shape[0], 2), dtype=np.uint64)
+    qs[:, 0] = cand
+    assert np.all(basis.search(qs) == -1)
+    qs[:, 1] = 1  # hi word set: outside a <= 64-spin basis
+    assert np.all(basis.search(qs[:10]) == -1)
+
+
+@pytest.mark.parametrize("W,kind", [(8, "ring"), (32, "ring"), (32, "ring+field"),
+                                    (48, "ring+field"), (48, "jpm")])
+def test_apply_parity(W, kind, lattices, cuda):
+    torch = cuda
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    H = _ham(jd, W, kind)
+    op = q.GpuOp(q.GpuBasis(b, W), H)
+    ob = orc.Basis(b)
+    x = _rand_x(D, W)
+    y0 = np.random.default_rng(W + 1).standard_normal(D)
+    xd = torch.from_numpy(x).cuda()
+    # evaluate: y = H x
+    yd = torch.full((D,), 7.0, dtype=torch.float64, device="cuda")
+    op.evaluate(xd, yd)
+    ref = orc.evaluate_add(ob, H, x, np.zeros(D))
+    got = yd.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= HV_TOL * np.linalg.norm(ref)
+    # evaluate_add: y += H x
+    yd = torch.from_numpy(y0.copy()).cuda()
+    op.evaluate_add(xd, yd)
+    ref = orc.evaluate_add(ob, H, x, y0.copy())
+    assert np.linalg.norm(yd.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref)
+    # host-vector path
+    assert np.linalg.norm(op.evaluate_host(x) - orc.evaluate_add(ob, H, x, np.zeros(D))) \
+        <= HV_TOL * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("W,kind", [(32, "ring+field"), (48, "jpm")])
+def test_connectivity_bit_exact(W, kind, lattices, cuda):
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    H = _ham(jd, W, kind)
+    op = q.GpuOp(q.GpuBasis(b, W), H)
+    r, c, v, d = op.connectivity()
+    orr, occ, ovv = orc.sparse_triplets(orc.Basis(b), H)
+    off = orr != occ
+    # reference triplets are (J, j, H[J, j]); the GPU lists H[row, col]
+    o_key = np.lexsort((occ[off], orr[off]))
+    g_key = np.lexsort((c, r))
+    assert np.array_equal(r[g_key], orr[off][o_key])
+    assert np.array_equal(c[g_key], occ[off][o_key])
+    assert np.array_equal(v[g_key], ovv[off][o_key])
+    diag = np.zeros(D)
+    diag[orr[~off]] = ovv[~off]
+    assert np.allclose(d, diag, rtol=1e-14, atol=1e-17)
+    if kind == "ring+field" and W == 32:
+        assert r.shape[0] == 15360
+
+
+def _relabel(jd, f, W2):
+    """Copy of a lattice with spin i renamed f(i) (for >64-spin code paths)."""
+    import copy
+    j2 = copy.deepcopy(jd)
+    atoms = [None] * W2
+    for i, a in enumerate(jd["atoms"]):
+        atoms[f(i)] = a
+    filler = {"xyz": [0, 0, 0], "sl": "0"}
+    j2["atoms"] = [a if a is not None else filler for a in atoms]
+    for t in j2["tetrahedra"]:
+        t["member_spin_idx"] = [f(s) for s in t["member_spin_idx"]]
+    for r in j2["rings"]:
+        r["member_spin_idx"] = [f(s) for s in r["member_spin_idx"]]
+    for bd in j2["bonds"]:
+        bd["from_idx"], bd["to_idx"] = f(bd["from_idx"]), f(bd["to_idx"])
+    return j2
+
+
+def test_apply_parity_two_words(lattices, cuda):
+    """48 spins spread over bits 20..114: exercises the 128-bit state path."""
+    torch = cuda
+    W2 = 115
+    f = lambda i: 2 * i + 20  # noqa: E731
+    jd = _relabel(lattices[48], f, W2)
+    b48 = load_basis(48)
+    states = []
+    for lo in b48[:, 0]:
+        s = 0
+        lo = int(lo)
+        for i in range(48):
+            if (lo >> i) & 1:
+                s |= 1 << f(i)
+        states.append(s)
+    states.sort()
+    lohi = q.as_lohi(states)
+    H = q.build_hamiltonian_g(jd, [-0.9, -0.25, -0.6, -0.45])
+    for i in range(48):  # a field on the real spins only
+        H.add_term(0.013 * ((i % 4) - 1.5), q.SymbolicPMROperator.from_ops("z", [f(i)]))
+    H.add_term(0.021, q.SymbolicPMROperator.from_ops("zz", [f(3), f(40)]))
+    op = q.GpuOp(q.GpuBasis(lohi, W2), H)
+    x = _rand_x(len(states), 5)
+    yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
+    op.evaluate(torch.from_numpy(x).cuda(), yd)
+    ref = orc.evaluate_add(orc.Basis(lohi), H, x, np.zeros(len(states)))
+    assert np.linalg.norm(yd.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref)
+    basis = q.GpuBasis(lohi, W2)
+    assert np.array_equal(basis.search(lohi), np.arange(len(states)))
+
+
+@pytest.mark.parametrize("R", [1, 3, 16, 64])
+def test_block_apply_matches_columns(R, lattices, cuda):
+    torch = cuda
+    W = 32
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    H = _ham(jd, W, "ring+field")
+    op = q.GpuOp(q.GpuBasis(b, W), H)
+    X = torch.from_numpy(np.random.default_rng(R).standard_normal((D, R))).cuda()
+    Y = torch.zeros((D, R), dtype=torch.float64, device="cuda")
+    op.apply_block(X, Y)
+    for r in range(R):
+        y = torch.zeros(D, dtype=torch.float64, device="cuda")
+        op.evaluate(X[:, r].contiguous(), y)
+        assert torch.equal(Y[:, r], y)
+
+
+def test_block_apply_per_column_coefficients(lattices, cuda):
+    torch = cuda
+    W = 48
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    R = 5
+    rng = np.random.default_rng(3)
+    gs = rng.uniform(0.0, 1.0, size=(R, 4))
+    Bs = rng.normal(size=(R, 3)) * 0.1
+    # slot s < 4: ring sublattice s; slot 4 + s: field on sublattice s
+    Hs = q.lattice.add_ring_terms(q.SymbolicOpSum(), jd, [0.0] * 4, slots=True)
+    q.add_local_field(Hs, jd, [0.0, 0.0, 0.0], slots=True)
+    bas = q.GpuBasis(b, W)
+    op = q.GpuOp(bas, Hs, n_slots=8)
+    coeffs = np.zeros((8, R))
+    loc = q.get_loc_z()
+    for r in range(R):
+        coeffs[:4, r] = -gs[r]
+        coeffs[4:, r] = [-0.5 * float(Bs[r] @ loc[s]) for s in range(4)]
+    X = torch.from_numpy(rng.standard_normal((D, R))).cuda()
+    Y = torch.zeros((D, R), dtype=torch.float64, device="cuda")
+    op.apply_block(X, Y, coeffs=torch.from_numpy(coeffs).cuda())
+    ob = orc.Basis(b)
+    for r in range(R):
+        H = q.build_hamiltonian_g(jd, list(-gs[r]))
+        q.add_local_field(H, jd, Bs[r])
+        ref = orc.evaluate_add(ob, H, X[:, r].cpu().numpy(), np.zeros(D))
+        got = Y[:, r].cpu().numpy()
+        assert np.linalg.norm(got - ref) <= HV_TOL * np.linalg.norm(ref)

@@ -1,0 +1,44 @@
+"""The C-ABI library loads on a CPU-only host and exports every entry point
+declared in include/qsi_gpu.h; argument validation fails cleanly before any
+GPU work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+from qsi_subspace_ed_amd import _lib
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "qsi_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(qsi_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_all_declared_symbols_exported():
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    assert set(_declared()) == set(_lib.SIGNATURES)
+
+
+def test_version_and_validation():
+    lib = _lib.load()
+    assert b"gfx950" in lib.qsi_version()
+    h = C.c_void_p()
+    bad = np.array([[5, 0], [3, 0]], dtype=np.uint64)  # not increasing
+    rc = lib.qsi_basis_create(bad.ctypes.data_as(C.c_void_p), 2, 8, 0, C.byref(h))
+    assert rc == _lib.QSI_E_INVALID
+    assert b"strictly increasing" in lib.qsi_last_error()
+    big = np.array([[1 << 10, 0]], dtype=np.uint64)  # bit 10 with 8 spins
+    rc = lib.qsi_basis_create(big.ctypes.data_as(C.c_void_p), 1, 8, 0, C.byref(h))
+    assert rc == _lib.QSI_E_INVALID
+    rc = lib.qsi_basis_create(None, 1, 0, 0, C.byref(h))
+    assert rc == _lib.QSI_E_INVALID
