@@ -112,7 +112,10 @@ int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t* st,
   DevBuf bv, bu, bpart, bab;
   QSI_CHECK_HIP(bv.alloc(bytes));
   QSI_CHECK_HIP(bu.alloc(bytes));
-  QSI_CHECK_HIP(bpart.alloc(max_vec_partials() * sizeof(double)));
+  // partials of the fused apply+dot (one per apply workgroup) and of the
+  // streaming reductions
+  const int64_t n_part = std::max<int64_t>(max_vec_partials(), apply_num_blocks(op));
+  QSI_CHECK_HIP(bpart.alloc(static_cast<size_t>(n_part) * sizeof(double)));
   const int64_t chunk = 64;
   // per step: alpha, beta, 1/beta
   QSI_CHECK_HIP(bab.alloc(static_cast<size_t>(3 * std::max<int64_t>(max_it, 1)) * sizeof(double)));
