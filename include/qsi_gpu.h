@@ -82,6 +82,13 @@ int qsi_device_count(void);
  * (1..128).  bucket_bits = 0 picks K automatically.  Synchronises. */
 int qsi_basis_create(const uint64_t *h_lohi, int64_t dim, int n_spins,
                      int bucket_bits, qsi_basis_t *out);
+/* As qsi_basis_create; with_hash = 1 also builds a device hash index
+ * (state -> rank, open addressing, load <= 1/2, 16 bytes per slot for <= 64
+ * spins, 32 above) used by the QSI_APPLY_HASH* kernels.  qsi_basis_create
+ * does not build it (with_hash = 0).  No counterpart in the reference. */
+int qsi_basis_create_ex(const uint64_t *h_lohi, int64_t dim, int n_spins,
+                        int bucket_bits, int with_hash, qsi_basis_t *out);
+int qsi_basis_has_hash(qsi_basis_t b);
 int qsi_basis_destroy(qsi_basis_t b);
 int64_t qsi_basis_dim(qsi_basis_t b);
 int qsi_basis_n_spins(qsi_basis_t b);
@@ -103,6 +110,28 @@ int qsi_op_destroy(qsi_op_t op);
 int64_t qsi_op_dim(qsi_op_t op);
 int qsi_op_n_offdiag(qsi_op_t op);
 int qsi_op_n_diag(qsi_op_t op);
+
+/* Apply kernel choice (no counterpart in the reference; a tuning knob).
+ * QSI_APPLY_HASH{,4}: each lane owns a row and finds its connected states in
+ * the hash index, 1/4 lookups in flight per lane.  QSI_APPLY_LANE{,2,4,8}: the
+ * same through the top-bit bucket table and a bisection of the bucket.
+ * QSI_APPLY_WINDOW: per wavefront, one bucket lookup per term, then the
+ * connected rows search a 64-state window staged in LDS.  AUTO = LANE8 (the
+ * fastest measured on MI355X: the sorted-basis bisection keeps the lookups of
+ * neighbouring rows in the same cache lines, which the hash index cannot).
+ * All give bit-identical y. */
+enum {
+  QSI_APPLY_AUTO = 0,
+  QSI_APPLY_WINDOW = 1,
+  QSI_APPLY_LANE = 2,
+  QSI_APPLY_LANE2 = 3,
+  QSI_APPLY_LANE4 = 4,
+  QSI_APPLY_LANE8 = 5,
+  QSI_APPLY_HASH = 6,
+  QSI_APPLY_HASH4 = 7
+};
+int qsi_op_set_variant(qsi_op_t op, int variant);
+int qsi_op_get_variant(qsi_op_t op);
 
 /* y = H x (accumulate = 0) or y += H x (accumulate = 1).
  * accumulate = 0 is LazyOpSum::evaluate / LazyOpSumProd::perform_op

@@ -305,12 +305,16 @@ double tridiag_lowest(const std::vector<double>& d, const std::vector<double>& e
 }
 
 // Eigenvector of the symmetric tridiagonal (d, e) for the eigenvalue lam by
-// inverse iteration with a pivoted tridiagonal LU of (T - lam I).
+// inverse iteration with a pivoted tridiagonal LU of (T - lam I), started from
+// e_1: inside a cluster of ghost copies of lam this converges to the
+// projection of e_1 on the cluster, the copy carried by the first Lanczos
+// vectors.  First component made non-negative.
 std::vector<double> tridiag_inverse_iteration(const std::vector<double>& d,
                                               const std::vector<double>& e,
                                               double lam) {
   const int n = static_cast<int>(d.size());
-  std::vector<double> x(n, 1.0);
+  std::vector<double> x(n, 0.0);
+  x[0] = 1.0;
   if (n == 1) return x;
   double tnorm = 0.0;
   for (int i = 0; i < n; ++i)
@@ -319,7 +323,9 @@ std::vector<double> tridiag_inverse_iteration(const std::vector<double>& d,
   const double tiny = std::max(tnorm, 1e-300) * 1e-15;
   std::vector<double> dl(n - 1), dd(n), du(n - 1), du2(std::max(n - 2, 0), 0.0);
   std::vector<int> swp(n - 1, 0);
-  for (int i = 0; i < n; ++i) dd[i] = d[i] - lam;
+  // shift just below the cluster so every ghost copy is amplified alike
+  const double sigma = lam - std::max(1e-10 * tnorm, tiny);
+  for (int i = 0; i < n; ++i) dd[i] = d[i] - sigma;
   for (int i = 0; i + 1 < n; ++i) { dl[i] = e[i]; du[i] = e[i]; }
   for (int i = 0; i + 1 < n; ++i) {
     if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
@@ -342,20 +348,35 @@ std::vector<double> tridiag_inverse_iteration(const std::vector<double>& d,
     }
   }
   if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
-  for (int pass = 0; pass < 4; ++pass) {
-    std::vector<double> b = x;
-    for (int i = 0; i + 1 < n; ++i) {
-      if (swp[i]) std::swap(b[i], b[i + 1]);
-      b[i + 1] -= dl[i] * b[i];
+  auto solve_passes = [&](std::vector<double>& v) {
+    for (int pass = 0; pass < 4; ++pass) {
+      std::vector<double> b = v;
+      for (int i = 0; i + 1 < n; ++i) {
+        if (swp[i]) std::swap(b[i], b[i + 1]);
+        b[i + 1] -= dl[i] * b[i];
+      }
+      v[n - 1] = b[n - 1] / dd[n - 1];
+      v[n - 2] = (b[n - 2] - du[n - 2] * v[n - 1]) / dd[n - 2];
+      for (int i = n - 3; i >= 0; --i)
+        v[i] = (b[i] - du[i] * v[i + 1] - du2[i] * v[i + 2]) / dd[i];
+      double s = 0.0;
+      for (double t : v) s += t * t;
+      s = std::sqrt(s);
+      if (v[0] < 0) s = -s;
+      for (auto& t : v) t /= s;
     }
-    x[n - 1] = b[n - 1] / dd[n - 1];
-    x[n - 2] = (b[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
-    for (int i = n - 3; i >= 0; --i)
-      x[i] = (b[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
-    double s = 0.0;
-    for (double v : x) s += v * v;
-    s = std::sqrt(s);
-    for (auto& v : x) v /= s;
+  };
+  solve_passes(x);
+  double res = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double t = (d[i] - lam) * x[i];
+    if (i > 0) t += e[i - 1] * x[i - 1];
+    if (i + 1 < n) t += e[i] * x[i + 1];
+    res = std::max(res, std::fabs(t));
+  }
+  if (!(res <= 1e-8 * std::max(tnorm, tiny))) {  // e_1 missed a localised vector
+    x.assign(n, 1.0 / std::sqrt(static_cast<double>(n)));
+    solve_passes(x);
   }
   return x;
 }

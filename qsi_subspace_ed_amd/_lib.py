@@ -17,6 +17,18 @@ QSI_E_NOTFOUND = -4
 QSI_E_NOCONV = -5
 QSI_E_UNSUPPORTED = -6
 
+QSI_APPLY_AUTO = 0
+QSI_APPLY_WINDOW = 1
+QSI_APPLY_LANE = 2
+QSI_APPLY_LANE2 = 3
+QSI_APPLY_LANE4 = 4
+QSI_APPLY_LANE8 = 5
+QSI_APPLY_HASH = 6
+QSI_APPLY_HASH4 = 7
+VARIANTS = {"auto": QSI_APPLY_AUTO, "window": QSI_APPLY_WINDOW, "lane": QSI_APPLY_LANE,
+            "lane2": QSI_APPLY_LANE2, "lane4": QSI_APPLY_LANE4, "lane8": QSI_APPLY_LANE8,
+            "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4}
+
 
 class QsiError(RuntimeError):
     def __init__(self, code, msg):
@@ -72,6 +84,8 @@ SIGNATURES = {
     "qsi_version": (C.c_char_p, []),
     "qsi_device_count": (C.c_int, []),
     "qsi_basis_create": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.POINTER(_P)]),
+    "qsi_basis_create_ex": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    "qsi_basis_has_hash": (C.c_int, [_P]),
     "qsi_basis_destroy": (C.c_int, [_P]),
     "qsi_basis_dim": (_I64, [_P]),
     "qsi_basis_n_spins": (C.c_int, [_P]),
@@ -82,6 +96,8 @@ SIGNATURES = {
     "qsi_op_dim": (_I64, [_P]),
     "qsi_op_n_offdiag": (C.c_int, [_P]),
     "qsi_op_n_diag": (C.c_int, [_P]),
+    "qsi_op_set_variant": (C.c_int, [_P, C.c_int]),
+    "qsi_op_get_variant": (C.c_int, [_P]),
     "qsi_op_apply": (C.c_int, [_P, _P, _P, C.c_int, _P]),
     "qsi_op_apply_host": (C.c_int, [_P, _P, _P, C.c_int]),
     "qsi_op_apply_block": (C.c_int, [_P, _P, _I64, _P, _I64, C.c_int, _P, C.c_int, _P]),

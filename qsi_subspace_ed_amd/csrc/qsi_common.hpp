@@ -62,20 +62,6 @@ struct State {
   uint64_t w[NW];
 };
 
-template <int NW>
-__device__ __forceinline__ State<NW> load_state(const uint64_t* __restrict__ s,
-                                                int64_t i) {
-  State<NW> r;
-  if constexpr (NW == 1) {
-    r.w[0] = __builtin_nontemporal_load(s + i) ;
-  } else {
-    const ulonglong2 v = reinterpret_cast<const ulonglong2*>(s)[i];
-    r.w[0] = v.x;
-    r.w[1] = v.y;
-  }
-  return r;
-}
-
 // Plain (cached) load; used for lookups where reuse across lanes matters.
 template <int NW>
 __device__ __forceinline__ State<NW> load_state_cached(
@@ -128,7 +114,28 @@ struct BasisView {
   int64_t dim;
   int shift;               // n_spins - K
   int kbits;
+  int n_spins;
 };
+
+// true if the state has a bit at or above n_spins (it is then larger than
+// every basis state)
+template <int NW>
+__device__ __forceinline__ bool above_range(const State<NW>& s, int n_spins) {
+  if constexpr (NW == 1) {
+    return n_spins < 64 && (s.w[0] >> n_spins) != 0;
+  } else {
+    return n_spins < 128 && (s.w[1] >> (n_spins - 64)) != 0;
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ bool st_lt(const State<NW>& a, const State<NW>& b) {
+  if constexpr (NW == 1) {
+    return a.w[0] < b.w[0];
+  } else {
+    return (a.w[1] < b.w[1]) || (a.w[1] == b.w[1] && a.w[0] < b.w[0]);
+  }
+}
 
 // Rank of `key` in the basis, or -1.  The bucket [table[k], table[k+1]) is
 // searched with a branch-free lower-bound bisection.
@@ -148,6 +155,67 @@ __device__ __forceinline__ int64_t basis_lookup(const BasisView& b,
   }
   const State<NW> f = load_state_cached<NW>(b.states, lo);
   return st_eq<NW>(f, key) ? static_cast<int64_t>(lo) : -1;
+}
+
+// ---- hash index ------------------------------------------------------------
+// Open addressing with linear probing over 2^hbits slots, load <= 1/2.  A slot
+// is one 16-byte word pair: NW=1 {state, rank}; NW=2 uses two pairs
+// {lo, hi}, {rank, -}.  rank == kHashEmpty marks a free slot.
+constexpr uint64_t kHashEmpty = ~0ull;
+
+struct HashView {
+  const ulonglong2* slots;
+  uint64_t mask;  // 2^hbits - 1
+  int hbits;
+};
+
+template <int NW>
+__host__ __device__ __forceinline__ uint64_t hash_home(const State<NW>& s, int hbits) {
+  uint64_t k = s.w[0];
+  if constexpr (NW == 2) k ^= s.w[1] * 0xC2B2AE3D27D4EB4Full;
+  return (k * 0x9E3779B97F4A7C15ull) >> (64 - hbits);
+}
+
+// Rank of key or -1.
+template <int NW>
+__device__ __forceinline__ int64_t hash_lookup(const HashView& h,
+                                               const State<NW>& key) {
+  uint64_t p = hash_home<NW>(key, h.hbits);
+  for (;;) {
+    if constexpr (NW == 1) {
+      const ulonglong2 e = h.slots[p];
+      if (e.y == kHashEmpty) return -1;
+      if (e.x == key.w[0]) return static_cast<int64_t>(e.y);
+    } else {
+      const ulonglong2 e = h.slots[2 * p];
+      const ulonglong2 r = h.slots[2 * p + 1];
+      if (r.x == kHashEmpty) return -1;
+      if (e.x == key.w[0] && e.y == key.w[1]) return static_cast<int64_t>(r.x);
+    }
+    p = (p + 1) & h.mask;
+  }
+}
+
+// First rank whose state is >= key (dim if none): bucket from the table,
+// then a lower-bound bisection inside it.
+template <int NW>
+__device__ __forceinline__ int64_t basis_lower_bound(const BasisView& b,
+                                                     const State<NW>& key) {
+  if (above_range<NW>(key, b.n_spins)) return b.dim;
+  const uint32_t k = bucket_key<NW>(key, b.shift);
+  uint32_t lo = b.table[k];
+  uint32_t n = b.table[k + 1] - lo;
+  while (n > 0) {
+    const uint32_t half = n >> 1;
+    const State<NW> m = load_state_cached<NW>(b.states, lo + half);
+    if (st_lt<NW>(m, key)) {
+      lo += half + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return static_cast<int64_t>(lo);
 }
 
 }  // namespace qsi

@@ -60,6 +60,37 @@ __global__ void k_build_table(const uint64_t* __restrict__ states, int64_t dim,
   table[k] = static_cast<uint32_t>(lo);
 }
 
+// Insert every basis state into the hash index (slots pre-filled with ones).
+// A thread claims a free slot by a compare-and-swap on its rank word and then
+// writes the state; keys are unique, so no insertion compares keys.
+template <int NW>
+__global__ void k_build_hash(const uint64_t* __restrict__ states, int64_t dim,
+                             ulonglong2* __restrict__ slots, uint64_t mask,
+                             int hbits) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const State<NW> s = load_state_cached<NW>(states, i);
+  uint64_t p = hash_home<NW>(s, hbits);
+  for (;;) {
+    unsigned long long* rank_word =
+        (NW == 1) ? reinterpret_cast<unsigned long long*>(&slots[p].y)
+                  : reinterpret_cast<unsigned long long*>(&slots[2 * p + 1].x);
+    const unsigned long long old =
+        atomicCAS(rank_word, static_cast<unsigned long long>(kHashEmpty),
+                  static_cast<unsigned long long>(i));
+    if (old == kHashEmpty) {
+      if constexpr (NW == 1) {
+        slots[p].x = s.w[0];
+      } else {
+        slots[2 * p].x = s.w[0];
+        slots[2 * p].y = s.w[1];
+      }
+      return;
+    }
+    p = (p + 1) & mask;
+  }
+}
+
 template <int NW>
 __global__ void k_search(BasisView b, const uint64_t* __restrict__ q, int64_t n,
                          int64_t* __restrict__ rank) {
@@ -111,6 +142,15 @@ __device__ __forceinline__ State<NW> xor_state(const State<NW>& s,
   return r;
 }
 
+template <int NW>
+__device__ __forceinline__ bool diag_ok(const DiagTerm& d, const State<NW>& s) {
+  if constexpr (NW == 1) {
+    return (s.w[0] & d.m[0]) == d.v[0];
+  } else {
+    return ((s.w[0] & d.m[0]) == d.v[0]) & ((s.w[1] & d.m[1]) == d.v[1]);
+  }
+}
+
 // Diagonal value d(s) = sum_t c_t * (+-1 or field count); x-independent.
 template <int NW>
 __device__ __forceinline__ double diag_accumulate(
@@ -120,7 +160,7 @@ __device__ __forceinline__ double diag_accumulate(
     const DiagTerm& d = dg[t];
     double f;
     if (d.kind == 0) {
-      f = d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z));
+      f = diag_ok<NW>(d, s) ? d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z)) : 0.0;
     } else {
       int nz = __popcll(d.Z[0]);
       if constexpr (NW == 2) nz += __popcll(d.Z[1]);
@@ -134,8 +174,13 @@ __device__ __forceinline__ double diag_accumulate(
 
 // y[i] = yscale * y_in[i] + sum_t H_{i,t} x[...] + d(s_i) x[i]
 // If DOT, partials[block] = sum over the block's rows of x[i] * y[i].
-template <int NW, bool HAS_Z, bool DOT>
-__global__ __launch_bounds__(kApplyBlock) void k_apply(
+// Lane variant: every lane walks its own row's connected states and runs a
+// full table lookup for each.  G lookups are carried through the bucket
+// table, the bisection and the x gather together, so a lane keeps G
+// independent memory chains in flight; contributions are still added in
+// term order.
+template <int NW, bool HAS_Z, bool DOT, int G>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
@@ -150,21 +195,344 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply(
     double acc = y_in ? yscale * y_in[i] : 0.0;
     for (int t0 = 0; t0 < n_off; t0 += 64) {
       const int tn = min(64, n_off - t0);
-      // phase 1: which terms connect this row (uniform loop over terms)
+      // which terms connect this row (uniform loop over terms)
       uint64_t mask = 0;
       for (int t = 0; t < tn; ++t) {
         mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
       }
-      // phase 2: gather through the rank lookup, in term order
       while (mask) {
-        const int t = t0 + __builtin_ctzll(mask);
-        mask &= mask - 1;
-        const OffTerm& T = off[t];
-        const State<NW> sj = xor_state<NW>(s, T.X);
-        const int64_t J = basis_lookup<NW>(b, sj);
-        if (J >= 0) {
-          const double xj = x[J];
-          if (fabs(xj) > kApplyTol) {
+        int tt[G];
+        bool act[G];
+        State<NW> key[G];
+        uint32_t lo[G], n[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          act[g] = mask != 0;
+          tt[g] = act[g] ? t0 + __builtin_ctzll(mask) : t0;
+          mask &= mask - 1;
+          key[g] = xor_state<NW>(s, off[tt[g]].X);
+        }
+        // bucket bounds
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          lo[g] = 0;
+          n[g] = 0;
+          if (act[g]) {
+            const uint32_t k = bucket_key<NW>(key[g], b.shift);
+            if (k < (1u << b.kbits)) {
+              lo[g] = b.table[k];
+              n[g] = b.table[k + 1] - lo[g];
+            }
+          }
+        }
+        // interleaved bisection down to one candidate per key
+        bool more = true;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (n[g] > 1) {
+              const uint32_t half = n[g] >> 1;
+              const State<NW> m = load_state_cached<NW>(b.states, lo[g] + half);
+              lo[g] = st_le<NW>(m, key[g]) ? lo[g] + half : lo[g];
+              n[g] -= half;
+              more |= n[g] > 1;
+            }
+          }
+        }
+        double xj[G];
+        bool hit[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          hit[g] = false;
+          xj[g] = 0.0;
+          if (n[g] == 1) {
+            const State<NW> f = load_state_cached<NW>(b.states, lo[g]);
+            hit[g] = st_eq<NW>(f, key[g]);
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          if (hit[g]) xj[g] = x[lo[g]];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (hit[g] && fabs(xj[g]) > kApplyTol) {
+            const OffTerm& T = off[tt[g]];
+            double c = T.c * T.sgn;
+            if constexpr (HAS_Z) {
+              if (parity_notz<NW>(s, T.Z)) c = -c;
+            }
+            acc += c * xj[g];
+          }
+        }
+      }
+    }
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
+}
+
+// Hash variant (default when the basis has a hash index): as the lane
+// variant, but each connected state is found with one probe of the hash index
+// (one 16-byte slot, ~1.2 probes on average) instead of a bucket bisection.
+template <int NW, bool HAS_Z, bool DOT, int G>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_hash(
+    BasisView b, HashView hv, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  double prod = 0.0;
+  if (i < b.dim) {
+    const State<NW> s = load_state_cached<NW>(b.states, i);
+    const double xi = x[i];
+    double acc = y_in ? yscale * y_in[i] : 0.0;
+    for (int t0 = 0; t0 < n_off; t0 += 64) {
+      const int tn = min(64, n_off - t0);
+      uint64_t mask = 0;
+      for (int t = 0; t < tn; ++t) {
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
+      }
+      while (mask) {
+        int tt[G];
+        bool live[G];
+        State<NW> key[G];
+        uint64_t pos[G];
+        int64_t J[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          live[g] = mask != 0;
+          tt[g] = live[g] ? t0 + __builtin_ctzll(mask) : t0;
+          mask &= mask - 1;
+          key[g] = xor_state<NW>(s, off[tt[g]].X);
+          pos[g] = hash_home<NW>(key[g], hv.hbits);
+          J[g] = -1;
+        }
+        // interleaved probing
+        bool more = true;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (live[g]) {
+              uint64_t rk, k0, k1 = 0;
+              if constexpr (NW == 1) {
+                const ulonglong2 e = hv.slots[pos[g]];
+                k0 = e.x;
+                rk = e.y;
+              } else {
+                const ulonglong2 e = hv.slots[2 * pos[g]];
+                const ulonglong2 r = hv.slots[2 * pos[g] + 1];
+                k0 = e.x;
+                k1 = e.y;
+                rk = r.x;
+              }
+              bool eq = (k0 == key[g].w[0]);
+              if constexpr (NW == 2) eq = eq && (k1 == key[g].w[1]);
+              if (rk == kHashEmpty) {
+                live[g] = false;
+              } else if (eq) {
+                J[g] = static_cast<int64_t>(rk);
+                live[g] = false;
+              } else {
+                pos[g] = (pos[g] + 1) & hv.mask;
+                more = true;
+              }
+            }
+          }
+        }
+        double xj[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) xj[g] = (J[g] >= 0) ? x[J[g]] : 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (J[g] >= 0 && fabs(xj[g]) > kApplyTol) {
+            const OffTerm& T = off[tt[g]];
+            double c = T.c * T.sgn;
+            if constexpr (HAS_Z) {
+              if (parity_notz<NW>(s, T.Z)) c = -c;
+            }
+            acc += c * xj[g];
+          }
+        }
+      }
+    }
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Window variant (default).  A wavefront owns 64 consecutive rows.  For one
+// term, the rows it connects share the term's precondition bits, so their
+// targets s ^ X are increasing in lane order and fall in a short run of the
+// sorted basis (median ~40 ranks on the 64-spin ice basis).  Per chunk of 64
+// terms the wavefront
+//   1. tests every term on every row; a ballot gives each term's lane mask,
+//   2. finds each term's anchor rank (lower bound of its first target) with
+//      the 64 lookups spread over the 64 lanes,
+//   3. per term, stages basis[r .. r+63] and x[r .. r+63] in LDS with one
+//      coalesced load each, and every connected lane bisects the staged window
+//      (targets past the window fall back to a full lookup).
+// Contributions are added in term order, then the diagonal, exactly as in the
+// lane variant, so both variants give bit-identical y.
+// ----------------------------------------------------------------------------
+
+constexpr int kWavesPerBlock = kApplyBlock / 64;
+
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), lane);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+template <int NW, bool HAS_Z, bool DOT, int P>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_win(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+  __shared__ uint64_t rows_lds[kWavesPerBlock][64 * NW];
+  __shared__ uint64_t win_lds[kWavesPerBlock][P][64 * NW];
+  __shared__ double xwin_lds[kWavesPerBlock][P][64];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  const bool row_ok = i < b.dim;
+  const State<NW> s = load_state_cached<NW>(b.states, row_ok ? i : 0);
+#pragma unroll
+  for (int w = 0; w < NW; ++w) rows_lds[wid][lane * NW + w] = s.w[w];
+  double acc = 0.0;
+  double xi = 0.0;
+  if (row_ok) {
+    xi = x[i];
+    acc = y_in ? yscale * y_in[i] : 0.0;
+  }
+  wave_sync();
+  for (int t0 = 0; t0 < n_off; t0 += 64) {
+    const int tn = min(64, n_off - t0);
+    // 1. lane masks: lane k keeps the mask of term t0 + k
+    uint64_t my_mask = 0;
+    for (int k = 0; k < tn; ++k) {
+      const uint64_t m = __ballot(row_ok && term_ok<NW>(off[t0 + k], s));
+      my_mask = (lane == k) ? m : my_mask;
+    }
+    // 2. anchors: lower bound of the first connected row's target, one
+    //    lookup per lane, all in flight together
+    uint32_t anchor = 0;
+    if (lane < tn && my_mask != 0) {
+      const int a = __builtin_ctzll(my_mask);
+      State<NW> key;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) key.w[w] = rows_lds[wid][a * NW + w] ^ off[t0 + lane].X[w];
+      anchor = static_cast<uint32_t>(basis_lower_bound<NW>(b, key));
+    }
+    // 3. groups of P terms: stage P windows with back-to-back loads, then run
+    //    the P bisections interleaved
+    for (int kb = 0; kb < tn; kb += P) {
+      uint64_t gm[P];
+      int64_t gr[P];
+      uint64_t any = 0;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const int k = kb + j;
+        gm[j] = (k < tn) ? readlane64(my_mask, k) : 0;
+        gr[j] = gm[j] ? static_cast<int64_t>(__builtin_amdgcn_readlane(anchor, k)) : 0;
+        any |= gm[j];
+      }
+      if (any == 0) continue;
+      State<NW> wv[P];
+      double xv[P];
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const int64_t idx = gr[j] + lane;
+        if (gm[j] != 0 && idx < b.dim) {
+          wv[j] = load_state_cached<NW>(b.states, idx);
+          xv[j] = x[idx];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const int64_t idx = gr[j] + lane;
+        if (gm[j] != 0 && idx < b.dim) {
+#pragma unroll
+          for (int w = 0; w < NW; ++w) win_lds[wid][j][lane * NW + w] = wv[j].w[w];
+          xwin_lds[wid][j][lane] = xv[j];
+        }
+      }
+      wave_sync();
+      int lo[P], n[P], nv[P];
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        nv[j] = gm[j] ? static_cast<int>(min<int64_t>(64, b.dim - gr[j])) : 0;
+        lo[j] = 0;
+        n[j] = ((gm[j] >> lane) & 1) ? nv[j] : 0;
+      }
+      // lower-bound bisection of each key in its window (<= 7 rounds)
+#pragma unroll
+      for (int step = 0; step < 7; ++step) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          if (n[j] > 0) {
+            const State<NW> key = xor_state<NW>(s, off[t0 + kb + j].X);
+            const int half = n[j] >> 1;
+            State<NW> m;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) m.w[w] = win_lds[wid][j][(lo[j] + half) * NW + w];
+            if (st_lt<NW>(m, key)) {
+              lo[j] += half + 1;
+              n[j] -= half + 1;
+            } else {
+              n[j] = half;
+            }
+          }
+        }
+      }
+      // contributions in term order
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        if ((gm[j] >> lane) & 1) {
+          const OffTerm& T = off[t0 + kb + j];
+          const State<NW> key = xor_state<NW>(s, T.X);
+          bool found = false;
+          double xj = 0.0;
+          if (lo[j] < nv[j]) {
+            State<NW> m;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) m.w[w] = win_lds[wid][j][lo[j] * NW + w];
+            if (st_eq<NW>(m, key)) {
+              found = true;
+              xj = xwin_lds[wid][j][lo[j]];
+            }
+          } else {
+            const int64_t J = basis_lookup<NW>(b, key);
+            if (J >= 0) {
+              found = true;
+              xj = x[J];
+            }
+          }
+          if (found && fabs(xj) > kApplyTol) {
             double c = T.c * T.sgn;
             if constexpr (HAS_Z) {
               if (parity_notz<NW>(s, T.Z)) c = -c;
@@ -173,7 +541,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply(
           }
         }
       }
+      wave_sync();
     }
+  }
+  double prod = 0.0;
+  if (row_ok) {
     acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
     y[i] = acc;
     if constexpr (DOT) prod = xi * acc;
@@ -250,7 +622,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_block(
         const DiagTerm& d = dg[t];
         double f;
         if (d.kind == 0) {
-          f = d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z));
+          f = diag_ok<NW>(d, s) ? d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z)) : 0.0;
         } else {
           int nz = __popcll(d.Z[0]);
           if constexpr (NW == 2) nz += __popcll(d.Z[1]);
@@ -494,6 +866,7 @@ void basis_release(qsi_basis_s* b) {
   if (b->refs.fetch_sub(1) == 1) {
     if (b->d_states) (void)hipFree(b->d_states);
     if (b->d_table) (void)hipFree(b->d_table);
+    if (b->d_hash) (void)hipFree(b->d_hash);
     delete b;
   }
 }
@@ -508,10 +881,47 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
                            const double* yscale_dev, double* partials,
                            hipStream_t s) {
   const int64_t nb = apply_num_blocks(op);
-  hipLaunchKernelGGL((k_apply<NW, HAS_Z, DOT>), dim3(static_cast<uint32_t>(nb)),
-                     dim3(kApplyBlock), 0, s, op->basis->view(), op->d_off,
-                     op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
-                     yscale_dev, partials);
+  const dim3 grid(static_cast<uint32_t>(nb)), block(kApplyBlock);
+  const auto v = op->basis->view();
+  int variant = op->variant;
+  if (variant == QSI_APPLY_AUTO) variant = QSI_APPLY_LANE8;
+  switch (variant) {
+    case QSI_APPLY_LANE:
+      hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_LANE2:
+      hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 2>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_WINDOW:
+      hipLaunchKernelGGL((k_apply_win<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_LANE8:
+      hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 8>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_HASH:
+      hipLaunchKernelGGL((k_apply_hash<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,
+                         op->basis->hview(), op->d_off, op->n_off, op->d_diag,
+                         op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_HASH4:
+      hipLaunchKernelGGL((k_apply_hash<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
+                         op->basis->hview(), op->d_off, op->n_off, op->d_diag,
+                         op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+      break;
+    default:  // QSI_APPLY_LANE4
+      hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+  }
+
 }
 
 int launch_apply(const qsi_op_s* op, const double* x, double* y,
@@ -596,6 +1006,7 @@ int launch_scal_dev(const double* s_dev, double* x, int64_t n, hipStream_t s) {
 }
 
 int max_vec_partials() { return 8192; }
+int multidot_partials() { return 1 << 17; }
 
 int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
                     int64_t n, double* partials, double* out, double* acc_out,
@@ -605,7 +1016,9 @@ int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
     set_error("launch_multidot: too many columns");
     return QSI_E_INVALID;
   }
-  const int nb = std::min(vec_blocks(n), max_vec_partials() / kMaxCols);
+  // one partial per (workgroup, column): the caller's buffer holds
+  // multidot_partials() doubles
+  const int nb = std::max(1, std::min(vec_blocks(n), multidot_partials() / std::max(m, 1)));
   hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, w,
                      n, partials);
   QSI_CHECK_LAUNCH();
@@ -672,17 +1085,21 @@ int qsi_device_count(void) {
   return n;
 }
 
+// K = ceil(log2 D) + 4, at most 28 (a 1 GiB table).  Ice-rule bases fill the
+// top-bit buckets very unevenly (the top spins are strongly constrained), so
+// the table is kept wide: on the 64-spin Q=0 basis K=26 leaves a weighted
+// mean bucket of ~23 states (5 bisection steps).
 static int auto_bucket_bits(int64_t dim, int n_spins) {
   int lg = 0;
   while ((int64_t{1} << lg) < dim) ++lg;
-  int k = lg + 2;
+  int k = lg + 4;
   k = std::min(k, 28);
   k = std::min(k, n_spins);
   return std::max(k, 1);
 }
 
-int qsi_basis_create(const uint64_t* h_lohi, int64_t dim, int n_spins,
-                     int bucket_bits, qsi_basis_t* out) {
+int qsi_basis_create_ex(const uint64_t* h_lohi, int64_t dim, int n_spins,
+                        int bucket_bits, int with_hash, qsi_basis_t* out) {
   QSI_REQUIRE(out != nullptr, "qsi_basis_create: out is null");
   *out = nullptr;
   QSI_REQUIRE(h_lohi != nullptr || dim == 0, "qsi_basis_create: states are null");
@@ -762,9 +1179,39 @@ int qsi_basis_create(const uint64_t* h_lohi, int64_t dim, int n_spins,
   if (e != hipSuccess) return fail(e, "k_build_table launch");
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return fail(e, "k_build_table");
+  if (with_hash && dim > 0) {
+    int hb = 1;
+    while ((int64_t{1} << hb) < 2 * dim) ++hb;  // load factor <= 1/2
+    const size_t slots = size_t{1} << hb;
+    const size_t bytes = slots * sizeof(ulonglong2) * nw;
+    e = hipMalloc(&b->d_hash, bytes);
+    if (e != hipSuccess) return fail(e, "hipMalloc(hash)");
+    e = hipMemset(b->d_hash, 0xFF, bytes);
+    if (e != hipSuccess) return fail(e, "hipMemset(hash)");
+    b->hbits = hb;
+    const uint32_t g2 = static_cast<uint32_t>((dim + 255) / 256);
+    const uint64_t mask = slots - 1;
+    if (nw == 1)
+      hipLaunchKernelGGL(k_build_hash<1>, dim3(g2), dim3(256), 0, 0, b->d_states,
+                         dim, b->d_hash, mask, hb);
+    else
+      hipLaunchKernelGGL(k_build_hash<2>, dim3(g2), dim3(256), 0, 0, b->d_states,
+                         dim, b->d_hash, mask, hb);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(e, "k_build_hash launch");
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(e, "k_build_hash");
+  }
   *out = b;
   return QSI_OK;
 }
+
+int qsi_basis_create(const uint64_t* h_lohi, int64_t dim, int n_spins,
+                     int bucket_bits, qsi_basis_t* out) {
+  return qsi_basis_create_ex(h_lohi, dim, n_spins, bucket_bits, 0, out);
+}
+
+int qsi_basis_has_hash(qsi_basis_t b) { return b && b->d_hash ? 1 : 0; }
 
 int qsi_basis_destroy(qsi_basis_t b) {
   if (b) basis_release(b);
@@ -826,15 +1273,11 @@ int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
     // a term whose sign is 0 or whose up/down masks overlap is identically 0
     if (T.sign == 0 || (T.up[0] & T.down[0]) || (T.up[1] & T.down[1])) continue;
     if ((T.X[0] | T.X[1]) == 0) {
-      // diagonal.  A diagonal term with up/down projectors is a Z string
-      // times projectors; only pure Z strings occur on the hot path.
-      if ((T.up[0] | T.up[1] | T.down[0] | T.down[1]) != 0) {
-        delete op;
-        set_error("qsi_op_create: diagonal terms with projector masks are not supported");
-        return QSI_E_UNSUPPORTED;
-      }
+      // diagonal: a Z string, possibly times up/down projectors (a ring
+      // that visits a spin twice reduces to one)
+      const bool proj = (T.up[0] | T.up[1] | T.down[0] | T.down[1]) != 0;
       const int nz = __builtin_popcountll(T.Z[0]) + __builtin_popcountll(T.Z[1]);
-      if (nz == 1 && T.slot < 0) {
+      if (nz == 1 && T.slot < 0 && !proj) {
         // single-site Z: c * sign * Z_i; merge equal effective coefficients
         const double ce = T.coeff * T.sign;
         bool merged = false;
@@ -859,6 +1302,10 @@ int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
         continue;
       }
       DiagTerm d{};
+      for (int w = 0; w < 2; ++w) {
+        d.m[w] = T.up[w] | T.down[w];
+        d.v[w] = T.up[w];
+      }
       d.Z[0] = T.Z[0];
       d.Z[1] = T.Z[1];
       d.c = T.coeff;
@@ -917,6 +1364,18 @@ int qsi_op_destroy(qsi_op_t op) {
 }
 
 int64_t qsi_op_dim(qsi_op_t op) { return op ? op->basis->dim : -1; }
+
+int qsi_op_set_variant(qsi_op_t op, int variant) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_set_variant: null operator");
+  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_HASH4,
+              "qsi_op_set_variant: unknown variant");
+  QSI_REQUIRE(op->basis->d_hash || (variant != QSI_APPLY_HASH && variant != QSI_APPLY_HASH4),
+              "qsi_op_set_variant: the basis has no hash index");
+  op->variant = variant;
+  return QSI_OK;
+}
+
+int qsi_op_get_variant(qsi_op_t op) { return op ? op->variant : -1; }
 int qsi_op_n_offdiag(qsi_op_t op) { return op ? op->n_off : -1; }
 int qsi_op_n_diag(qsi_op_t op) { return op ? op->n_diag : -1; }
 

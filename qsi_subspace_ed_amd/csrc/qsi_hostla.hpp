@@ -139,11 +139,18 @@ inline double tridiag_lowest(const std::vector<double>& d,
 }
 
 // Unit eigenvector of the tridiagonal (d, e) (leading n) for eigenvalue lam
-// by inverse iteration with a row-pivoted LU of T - lam I.
+// by inverse iteration with a row-pivoted LU of T - lam I, started from e_1.
+// When lam belongs to a cluster of numerically equal eigenvalues (the ghost
+// copies a long Lanczos run produces), every cluster direction is amplified
+// alike, so the result is the normalised projection of e_1 onto the cluster:
+// the copy carried by the first Lanczos vectors, i.e. the Ritz vector of the
+// converged eigenvalue rather than one of its later ghosts.  Returned with a
+// non-negative first component.
 inline std::vector<double> tridiag_eigvec(const std::vector<double>& d,
                                           const std::vector<double>& e,
                                           size_t n, double lam) {
-  std::vector<double> x(n, 1.0);
+  std::vector<double> x(n, 0.0);
+  x[0] = 1.0;
   if (n == 1) return x;
   double tn = 0.0;
   for (size_t i = 0; i < n; ++i)
@@ -153,7 +160,10 @@ inline std::vector<double> tridiag_eigvec(const std::vector<double>& d,
   // rows: sub l[i] (i+1,i), diag a[i], super u[i] (i,i+1), second super w[i]
   std::vector<double> l(n - 1), a(n), u(n - 1), w(n, 0.0);
   std::vector<char> piv(n - 1, 0);
-  for (size_t i = 0; i < n; ++i) a[i] = d[i] - lam;
+  // shift just below the cluster: all copies of lam are amplified alike and
+  // the next distinct eigenvalue is suppressed by (shift / gap) per pass
+  const double sigma = lam - std::max(1e-10 * tn, tiny);
+  for (size_t i = 0; i < n; ++i) a[i] = d[i] - sigma;
   for (size_t i = 0; i + 1 < n; ++i) l[i] = u[i] = e[i];
   for (size_t i = 0; i + 1 < n; ++i) {
     if (std::fabs(a[i]) >= std::fabs(l[i])) {
@@ -176,20 +186,40 @@ inline std::vector<double> tridiag_eigvec(const std::vector<double>& d,
     }
   }
   if (a[n - 1] == 0.0) a[n - 1] = tiny;
-  for (int pass = 0; pass < 4; ++pass) {
-    std::vector<double> b = x;
-    for (size_t i = 0; i + 1 < n; ++i) {
-      if (piv[i]) std::swap(b[i], b[i + 1]);
-      b[i + 1] -= l[i] * b[i];
+  auto iterate = [&](std::vector<double>& xv) {
+    for (int pass = 0; pass < 4; ++pass) {
+      std::vector<double> b = xv;
+      for (size_t i = 0; i + 1 < n; ++i) {
+        if (piv[i]) std::swap(b[i], b[i + 1]);
+        b[i + 1] -= l[i] * b[i];
+      }
+      xv[n - 1] = b[n - 1] / a[n - 1];
+      xv[n - 2] = (b[n - 2] - u[n - 2] * xv[n - 1]) / a[n - 2];
+      for (size_t k = n - 2; k-- > 0;)
+        xv[k] = (b[k] - u[k] * xv[k + 1] - w[k] * xv[k + 2]) / a[k];
+      double s = 0.0;
+      for (double v : xv) s += v * v;
+      s = std::sqrt(s);
+      if (xv[0] < 0) s = -s;
+      for (auto& v : xv) v /= s;
     }
-    x[n - 1] = b[n - 1] / a[n - 1];
-    x[n - 2] = (b[n - 2] - u[n - 2] * x[n - 1]) / a[n - 2];
-    for (size_t k = n - 2; k-- > 0;)
-      x[k] = (b[k] - u[k] * x[k + 1] - w[k] * x[k + 2]) / a[k];
-    double s = 0.0;
-    for (double v : x) s += v * v;
-    s = std::sqrt(s);
-    for (auto& v : x) v /= s;
+  };
+  auto residual = [&](const std::vector<double>& xv) {
+    double r = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      double t = (d[i] - lam) * xv[i];
+      if (i > 0) t += e[i - 1] * xv[i - 1];
+      if (i + 1 < n) t += e[i] * xv[i + 1];
+      r = std::max(r, std::fabs(t));
+    }
+    return r;
+  };
+  iterate(x);
+  // e_1 can be nearly orthogonal to the wanted vector (a localised
+  // eigenvector); then restart from a flat vector
+  if (!(residual(x) <= 1e-8 * std::max(tn, tiny))) {
+    x.assign(n, 1.0 / std::sqrt(static_cast<double>(n)));
+    iterate(x);
   }
   return x;
 }

@@ -24,10 +24,13 @@ struct OffTerm {
   int32_t pad;
 };
 
-// Diagonal term.  kind 0: c * sgn * (-1)^popcount(~s & Z)  (a Z string);
+// Diagonal term.  kind 0: c * sgn * (-1)^popcount(~s & Z) when (s & m) == v
+// (a Z string times up/down projectors, m = up | down, v = up), else 0;
 // kind 1: c * (2 popcount(s & Z) - popcount(Z))  (a sum of single-site Z
 // terms that share one coefficient, merged on the host).
 struct DiagTerm {
+  uint64_t m[2];
+  uint64_t v[2];
   uint64_t Z[2];
   double c;
   double sgn;
@@ -46,7 +49,17 @@ struct qsi_basis_s {
   int device = 0;
   uint64_t* d_states = nullptr;  // dim * nw words
   uint32_t* d_table = nullptr;   // 2^kbits + 1
+  ulonglong2* d_hash = nullptr;  // hash index (null if not built)
+  int hbits = 0;
   std::atomic<int> refs{1};
+
+  qsi::HashView hview() const {
+    qsi::HashView h;
+    h.slots = d_hash;
+    h.mask = (hbits > 0) ? ((uint64_t{1} << hbits) - 1) : 0;
+    h.hbits = hbits;
+    return h;
+  }
 
   qsi::BasisView view() const {
     qsi::BasisView v;
@@ -55,6 +68,7 @@ struct qsi_basis_s {
     v.dim = dim;
     v.shift = shift;
     v.kbits = kbits;
+    v.n_spins = n_spins;
     return v;
   }
 };
@@ -65,6 +79,7 @@ struct qsi_op_s {
   int n_diag = 0;
   int n_slots = 0;
   bool has_z = false;  // any off-diagonal term with a Z string
+  int variant = 0;     // apply kernel: one of the QSI_APPLY_* values
   qsi::OffTerm* d_off = nullptr;
   qsi::DiagTerm* d_diag = nullptr;
   std::vector<qsi::OffTerm> h_off;
@@ -90,7 +105,9 @@ int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
 int64_t apply_num_blocks(const qsi_op_s* op);
 
 // streaming vector kernels; partial buffers need max_vec_partials() doubles
+// (multidot_partials() for launch_multidot)
 int max_vec_partials();
+int multidot_partials();
 int launch_dot(const double* a, const double* b, int64_t n, double* partials,
                double* out, hipStream_t s);
 int launch_axpy(double alpha, const double* x, double* y, int64_t n,

@@ -255,7 +255,7 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
   QSI_CHECK_HIP(bV.alloc(static_cast<size_t>(ld) * (ncv + 1) * sizeof(double)));
   QSI_CHECK_HIP(bW.alloc(static_cast<size_t>(ld) * (ncv + 1) * sizeof(double)));
   QSI_CHECK_HIP(bw.alloc(static_cast<size_t>(D) * sizeof(double)));
-  QSI_CHECK_HIP(bpart.alloc(max_vec_partials() * sizeof(double)));
+  QSI_CHECK_HIP(bpart.alloc(std::max(max_vec_partials(), multidot_partials()) * sizeof(double)));
   QSI_CHECK_HIP(bh.alloc(3 * 64 * sizeof(double)));
   QSI_CHECK_HIP(bscal.alloc(4 * sizeof(double)));
   double* V = bV.d();

@@ -63,13 +63,14 @@ class GpuBasis:
     """Sorted Z-basis resident in GPU memory with its top-bit bucket table
     (replaces ZBasisInterp, components/operator/include/operator.hpp:197-204)."""
 
-    def __init__(self, states, n_spins, bucket_bits=0):
+    def __init__(self, states, n_spins, bucket_bits=0, hash_index=False):
         lib = _lib.load()
         lohi = as_lohi(states)
         self._states_host = lohi
         h = C.c_void_p()
-        check(lib.qsi_basis_create(lohi.ctypes.data_as(C.c_void_p), lohi.shape[0],
-                                   int(n_spins), int(bucket_bits), C.byref(h)),
+        check(lib.qsi_basis_create_ex(lohi.ctypes.data_as(C.c_void_p), lohi.shape[0],
+                                      int(n_spins), int(bucket_bits), int(bool(hash_index)),
+                                      C.byref(h)),
               "qsi_basis_create")
         self._h = h
         self.n_spins = int(n_spins)
@@ -160,6 +161,14 @@ class GpuOp:
         self.basis = basis
         self.n_slots = ns
         self._n = basis.dim()
+
+    def set_variant(self, variant):
+        """Choose the apply kernel: 'auto' (= 'lane8'), 'hash', 'hash4'
+        (hash-index lookups; the basis needs hash_index=True),
+        'lane', 'lane2', 'lane4', 'lane8' (bucket-table lookups, 1-8 in flight
+        per lane) or 'window' (LDS-staged bucket windows per wavefront)."""
+        v = _lib.VARIANTS[variant]
+        check(_lib.load().qsi_op_set_variant(self._h, v), "qsi_op_set_variant")
 
     # -- Eigen / Spectra glue
     def rows(self):
