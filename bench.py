@@ -25,6 +25,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_64spin_pmc.json")
+CALIB_BYTES = 64 << 20  # the calibration copy reads and writes this many bytes
+
+
+def measured_traffic():
+    """HBM bytes per apply from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE of the apply kernel, each divided by what the
+    same counter reads for a copy of known size), or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            return float(json.load(f)["traffic_bytes_per_apply"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def build_workload(seed=1234, n_threads=16):
@@ -76,6 +89,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 64 MiB device copy (read+write) for counter calibration")
+    ap.add_argument("--no-solvers", action="store_true", help="skip the Lanczos/eigsh extras")
+    ap.add_argument("--variant", default="auto",
+                    help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")
+    ap.add_argument("--bucket-bits", type=int, default=0, help="top-bit table width K (0 = auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -92,8 +111,10 @@ def main():
     D = states.shape[0]
     print(f"[bench] 64-spin Q=0 basis D={D} enumerated in {t_enum:.2f} s", file=sys.stderr,
           flush=True)
-    basis = q.GpuBasis(states, 64)
+    basis = q.GpuBasis(states, 64, bucket_bits=args.bucket_bits,
+                       hash_index=args.variant.startswith("hash"))
     op = q.GpuOp(basis, H)
+    op.set_variant(args.variant)
     nnz = C.c_int64(0)
     _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
     nnz = nnz.value
@@ -124,18 +145,39 @@ def main():
     bytes_apply = D * (S + 8 + 8) + 8 * nnz
     achieved = bytes_apply / kern_s / 1e9
 
-    # Lanczos steps (apply + fused dot + axpy/norm + scale) on the device
-    st = q.Settings(krylov_dim=15, abs_tol=0.0, rel_tol=0.0, min_iterations=10**9,
-                    max_iterations=args.lanczos_steps, x0_seed=0, calc_eigenvector=False)
-    torch.cuda.synchronize()
-    tl0 = time.perf_counter()
-    op.lanczos_eigval0(st)
-    tl = time.perf_counter() - tl0
+    if args.calib_copy:
+        a = torch.full((CALIB_BYTES // 8,), 1.0, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=c)
+        torch.cuda.synchronize()
+        del a, c
 
-    # one mfsparse-style solve: 5 lowest, ncv=15, tol 1e-10
-    te0 = time.perf_counter()
-    evals, _, nconv, napp = op.eigsh(nev=5, ncv=15, tol=1e-10, seed=0, raise_on_noconv=False)
-    te = time.perf_counter() - te0
+    lanczos = None
+    eigsh = None
+    if not args.no_solvers and args.lanczos_steps > 0:
+        # Lanczos steps (apply + fused dot + axpy/norm + scale) on the device;
+        # two lengths so that the start-vector setup cancels out
+        def run_lanczos(n):
+            st = q.Settings(krylov_dim=15, abs_tol=0.0, rel_tol=0.0, min_iterations=10**9,
+                            max_iterations=n, x0_seed=0, calc_eigenvector=False)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            op.lanczos_eigval0(st)
+            return time.perf_counter() - t
+        n1, n2 = max(1, args.lanczos_steps // 4), args.lanczos_steps
+        run_lanczos(n1)
+        t1 = run_lanczos(n1)
+        t2 = run_lanczos(n2)
+        lanczos = {"steps": n2, "steps_per_s": (n2 - n1) / max(t2 - t1, 1e-9),
+                   "ms_per_step": (t2 - t1) / (n2 - n1) * 1e3}
+        # one mfsparse-style solve: 5 lowest, ncv=15, tol 1e-10
+        te0 = time.perf_counter()
+        evals, _, nconv, napp = op.eigsh(nev=5, ncv=15, tol=1e-10, seed=0,
+                                         raise_on_noconv=False)
+        te = time.perf_counter() - te0
+        eigsh = {"nev": 5, "ncv": 15, "tol": 1e-10, "seconds": te, "nconv": nconv,
+                 "applies": napp, "evals": [float(v) for v in evals]}
 
     out = {
         "metric": "H.v basis-states/s (64-spin Q=0 single-vector matrix-free apply)",
@@ -155,12 +197,14 @@ def main():
                    "g": [float(v) for v in g], "B": [float(v) for v in B]},
         "hv_per_s": args.steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic()},
         "apply_kernel_us": kern_s * 1e6,
+        "apply_variant": args.variant,
+        "bucket_bits": basis.bucket_bits(),
         "algorithmic_bytes_per_apply": int(bytes_apply),
-        "lanczos": {"steps": args.lanczos_steps, "steps_per_s": args.lanczos_steps / tl},
-        "eigsh": {"nev": 5, "ncv": 15, "tol": 1e-10, "seconds": te, "nconv": nconv,
-                  "applies": napp, "evals": [float(v) for v in evals]},
+        "lanczos": lanczos,
+        "eigsh": eigsh,
     }
     if not args.no_cpu:
         cb, y_cpu = cpu_baseline(states, H, xh)
