@@ -81,6 +81,58 @@ def cpu_baseline(states, H, x, budget_s=12.0, threads=None):
                       f"median {t * 1e3:.1f} ms per apply"}, y
 
 
+def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
+    """Batched FTLM on the 48-spin diag(2,2,3) lattice: sectors
+    n_spinon_pairs = 0 and 1 in one block launch, R random-sign start vectors
+    each, M Lanczos steps, FP64, g_s ~ U[0.2, 1], B = 0.  Compared with the
+    plain path: one single-vector Lanczos solve of M steps per sample, one
+    after another (timed on plain_samples samples per sector)."""
+    import torch
+    import qsi_subspace_ed_amd as q
+    from qsi_subspace_ed_amd import ftlm
+    jd = q.load_lattice(q.lattice_path("pyro_2,0,0_0,2,0_0,0,3.json"))
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(0.2, 1.0, size=4)
+    H = q.build_hamiltonian_g(jd, [-x for x in g])
+    bases = [q.enumerate_basis(jd, n, n_threads=16) for n in (0, 1)]
+    ops = [q.GpuOp(q.GpuBasis(b, 48), H) for b in bases]
+    V0s = [torch.from_numpy(ftlm.random_sign_block(b.shape[0], R, seed + k)).cuda()
+           for k, b in enumerate(bases)]
+    ftlm.lanczos_block(ops, V0s, 3)  # warm-up (graph capture, code load)
+    t0 = time.perf_counter()
+    al, be, dev_ms = ftlm.lanczos_block(ops, V0s, M, use_graph=True)
+    t_batched = time.perf_counter() - t0
+    _, _, dev_ms_nograph = ftlm.lanczos_block(ops, V0s, M, use_graph=False)
+    # thermodynamics of sector 0 from the batch (a sanity figure)
+    temps = np.array([0.05, 0.1, 0.2, 0.5, 1.0])
+    th = ftlm.thermodynamics(ftlm.quadrature(al[0], be[0]), bases[0].shape[0], temps)
+    # plain: single-vector Lanczos solves, one at a time
+    plain = []
+    for op in ops:
+        st = q.Settings(krylov_dim=M, abs_tol=0.0, rel_tol=0.0, min_iterations=10**9,
+                        max_iterations=M, x0_seed=0, calc_eigenvector=False)
+        op.lanczos_eigval0(st)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(plain_samples):
+            st.x0_seed = k + 1
+            op.lanczos_eigval0(st)
+        plain.append((time.perf_counter() - t0) / plain_samples)
+    n_solves = 2 * R
+    t_plain = R * sum(plain)  # 2R solves one after another
+    return {
+        "workload": "48-spin diag(2,2,3), sectors n_spinon_pairs=0,1 (D=%d,%d), R=%d, M=%d"
+                    % (bases[0].shape[0], bases[1].shape[0], R, M),
+        "batched_solves_per_s": n_solves / t_batched,
+        "batched_device_ms": dev_ms, "batched_device_ms_no_graph": dev_ms_nograph,
+        "batched_wall_s": t_batched,
+        "plain_solves_per_s": n_solves / t_plain,
+        "plain_ms_per_solve": [1e3 * v for v in plain],
+        "speedup": t_plain / t_batched,
+        "E_of_T_sector0": {"T": temps.tolist(), "E": th["E"].tolist(), "C": th["C"].tolist()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +147,7 @@ def main():
     ap.add_argument("--variant", default="auto",
                     help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")
     ap.add_argument("--bucket-bits", type=int, default=0, help="top-bit table width K (0 = auto)")
+    ap.add_argument("--no-ftlm", action="store_true", help="skip the batched FTLM comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -179,6 +232,13 @@ def main():
         eigsh = {"nev": 5, "ncv": 15, "tol": 1e-10, "seconds": te, "nconv": nconv,
                  "applies": napp, "evals": [float(v) for v in evals]}
 
+    ftlm_res = None
+    if not args.no_ftlm and not args.no_solvers:
+        ftlm_res = bench_ftlm()
+        print(f"[bench] FTLM batched {ftlm_res['batched_solves_per_s']:.0f} solves/s vs plain "
+              f"{ftlm_res['plain_solves_per_s']:.0f} ({ftlm_res['speedup']:.1f}x)",
+              file=sys.stderr, flush=True)
+
     out = {
         "metric": "H.v basis-states/s (64-spin Q=0 single-vector matrix-free apply)",
         "value": D * args.steps / wall,
@@ -205,6 +265,7 @@ def main():
         "algorithmic_bytes_per_apply": int(bytes_apply),
         "lanczos": lanczos,
         "eigsh": eigsh,
+        "ftlm_batched": ftlm_res,
     }
     if not args.no_cpu:
         cb, y_cpu = cpu_baseline(states, H, xh)

@@ -210,6 +210,30 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
               const double *d_v0, uint64_t seed, double *h_evals,
               double *d_evecs, int n_evecs, int *nconv, int *n_applies);
 
+/* Block Lanczos for FTLM: for each of n_sectors operators (sectors may have
+ * different basis sizes; all <= 64 spins or all > 64), R start vectors
+ * d_V0[s] (device, D_s x R row-major; columns normalised here) run M steps of
+ * the reference's 3-term recurrence (lanczos.cpp:111-146) side by side, with
+ * no reorthogonalisation and no convergence test.  All sectors share each
+ * launch (one persistent grid over the concatenated row tiles).  d_coeffs
+ * (optional, per sector n_slots x R) gives per-column slot coefficients, as in
+ * qsi_op_apply_block.  Output: h_alpha[(s*R + c)*M + j] = alpha_j and
+ * h_beta[...] = beta_j of column c of sector s; a beta that underflows ends
+ * that column (later coefficients are 0).  use_graph = 1 captures each step as
+ * a HIP graph.  *elapsed_ms (optional) = device time of the M steps.
+ * Synchronises.  No single counterpart in the reference (its FTLM driver,
+ * include/ftlm.hpp:56-91, does one sample at a time). */
+int qsi_lanczos_block(const qsi_op_t *ops, int n_sectors, int R,
+                      const double *const *d_V0, const double *const *d_coeffs,
+                      int M, double *h_alpha, double *h_beta, int use_graph,
+                      double *elapsed_ms);
+
+/* Eigenvalues (ascending) of the symmetric tridiagonal (d: n, e: n-1) and,
+ * if first_row != NULL, the first component of each unit eigenvector (the
+ * FTLM quadrature weights are their squares).  Host only. */
+int qsi_tridiag_eig(const double *d, const double *e, int n, double *evals,
+                    double *first_row);
+
 /* ---- vector kernels (streaming FP64, deterministic reductions) ----------- */
 int qsi_vec_dot(const double *d_a, const double *d_b, int64_t n, double *h_out,
                 qsi_stream_t stream); /* synchronises */

@@ -60,6 +60,7 @@ def load():
             "oracle_lanczos_eigval0": (C.c_int, [_P, _P, C.c_int, C.POINTER(LzSettings),
                                                  C.POINTER(C.c_double), _P, C.POINTER(LzResult),
                                                  _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+            "oracle_lanczos_coeffs": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, _P, _P]),
             "oracle_lanczos_dense": (C.c_int, [_P, C.c_int64, C.POINTER(LzSettings),
                                                C.POINTER(C.c_double), _P, C.POINTER(LzResult)]),
             "oracle_jacobi_eig": (None, [_P, C.c_int, _P, _P]),
@@ -232,6 +233,17 @@ def lanczos_eigval0(basis: Basis, H, st: LzSettings, cap=0):
         raise RuntimeError(f"oracle lanczos failed rc={rc}")
     k = min(nab.value, cap)
     return ev.value, (vec if st.calc_eigenvector else None), res, a[:k], b[:k]
+
+
+def lanczos_coeffs(basis: Basis, H, v0, M):
+    """(alphas, betas) of M steps of the 3-term recurrence from v0."""
+    arr, n = terms_from_opsum(H)
+    v0 = np.ascontiguousarray(v0, dtype=np.float64)
+    a = np.zeros(M)
+    b = np.zeros(M)
+    if load().oracle_lanczos_coeffs(basis._h, arr, n, _ptr(v0), M, _ptr(a), _ptr(b)) != 0:
+        raise RuntimeError("oracle lanczos_coeffs failed")
+    return a, b
 
 
 def lanczos_dense(M, st: LzSettings):

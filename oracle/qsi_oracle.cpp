@@ -676,6 +676,39 @@ int oracle_lanczos_eigval0(const void* b, const Term* terms, int n_terms,
   return bad ? -2 : 0;
 }
 
+// M steps of the 3-term recurrence of lanczos_iterate (lanczos.cpp:111-146,
+// no convergence test) from a given start vector (normalised here); the
+// FTLM sample coefficients.  A beta below 1e-300 ends the recurrence (the
+// remaining coefficients are left 0).
+int oracle_lanczos_coeffs(const void* b, const Term* terms, int n_terms,
+                          const double* v0, int M, double* alphas,
+                          double* betas) {
+  const Basis& B = *static_cast<const Basis*>(b);
+  OpSum H(terms, n_terms);
+  const size_t D = B.states.size();
+  std::vector<double> v(v0, v0 + D), u(D, 0.0);
+  const double n0 = std::sqrt(dot(v, v));
+  for (auto& x : v) x /= n0;
+  double beta = 0.0;
+  for (int j = 0; j < M; ++j) {
+    alphas[j] = betas[j] = 0.0;
+  }
+  for (int j = 0; j < M; ++j) {
+    if (j > 0)
+      for (auto& x : u) x *= -beta;
+    if (evaluate_add(B, H, v.data(), u.data()) != 0) return -1;
+    const double a = dot(v, u);
+    for (size_t i = 0; i < D; ++i) u[i] -= a * v[i];
+    beta = std::sqrt(dot(u, u));
+    alphas[j] = a;
+    betas[j] = beta;
+    if (!(beta > 1e-300)) break;
+    std::swap(v, u);
+    for (auto& x : v) x /= beta;
+  }
+  return 0;
+}
+
 // Same recurrence driven by an arbitrary dense symmetric matrix (the
 // MockHam scenario of test/test_lanczos.cpp:12-40,113-174).
 int oracle_lanczos_dense(const double* M, int64_t n, const LzSettings* st,

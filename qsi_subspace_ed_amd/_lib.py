@@ -106,6 +106,9 @@ SIGNATURES = {
                                       C.POINTER(LanczosResult)]),
     "qsi_eigsh": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, _P, C.c_uint64, _DP,
                             _P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "qsi_lanczos_block": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P),
+                                    C.POINTER(_P), C.c_int, _DP, _DP, C.c_int, _DP]),
+    "qsi_tridiag_eig": (C.c_int, [_DP, _DP, C.c_int, _DP, _DP]),
     "qsi_vec_dot": (C.c_int, [_P, _P, _I64, _DP, _P]),
     "qsi_vec_axpy": (C.c_int, [_D, _P, _P, _I64, _P]),
     "qsi_vec_scal": (C.c_int, [_D, _P, _I64, _P]),

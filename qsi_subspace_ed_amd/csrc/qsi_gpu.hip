@@ -26,13 +26,13 @@
 #include <new>
 #include <vector>
 
+#include "qsi_device.hpp"
 #include "qsi_internal.hpp"
 #include "qsi_reduce.hpp"
 
 namespace qsi {
 
 constexpr int kApplyBlock = 256;
-constexpr double kApplyTol = 1e-15;   // LazyOpSum::APPLY_TOL
 constexpr double kSparseTol = 1e-14;  // toSparseMatrix default tol
 
 // ============================================================================
@@ -108,69 +108,6 @@ __global__ void k_search(BasisView b, const uint64_t* __restrict__ q, int64_t n,
 // ============================================================================
 // Apply: one lane per output row
 // ============================================================================
-
-template <int NW>
-__device__ __forceinline__ bool term_ok(const OffTerm& t, const State<NW>& s) {
-  if constexpr (NW == 1) {
-    return (s.w[0] & t.m[0]) == t.v[0];
-  } else {
-    return ((s.w[0] & t.m[0]) == t.v[0]) & ((s.w[1] & t.m[1]) == t.v[1]);
-  }
-}
-
-template <int NW>
-__device__ __forceinline__ int parity_notz(const State<NW>& s,
-                                           const uint64_t* Z) {
-  int p = __popcll(~s.w[0] & Z[0]);
-  if constexpr (NW == 2) p += __popcll(~s.w[1] & Z[1]);
-  return p & 1;
-}
-
-template <int NW>
-__device__ __forceinline__ int popc_and(const State<NW>& s, const uint64_t* Z) {
-  int p = __popcll(s.w[0] & Z[0]);
-  if constexpr (NW == 2) p += __popcll(s.w[1] & Z[1]);
-  return p;
-}
-
-template <int NW>
-__device__ __forceinline__ State<NW> xor_state(const State<NW>& s,
-                                               const uint64_t* X) {
-  State<NW> r;
-  r.w[0] = s.w[0] ^ X[0];
-  if constexpr (NW == 2) r.w[1] = s.w[1] ^ X[1];
-  return r;
-}
-
-template <int NW>
-__device__ __forceinline__ bool diag_ok(const DiagTerm& d, const State<NW>& s) {
-  if constexpr (NW == 1) {
-    return (s.w[0] & d.m[0]) == d.v[0];
-  } else {
-    return ((s.w[0] & d.m[0]) == d.v[0]) & ((s.w[1] & d.m[1]) == d.v[1]);
-  }
-}
-
-// Diagonal value d(s) = sum_t c_t * (+-1 or field count); x-independent.
-template <int NW>
-__device__ __forceinline__ double diag_accumulate(
-    double acc, double xi, const State<NW>& s, const DiagTerm* __restrict__ dg,
-    int n_diag) {
-  for (int t = 0; t < n_diag; ++t) {
-    const DiagTerm& d = dg[t];
-    double f;
-    if (d.kind == 0) {
-      f = diag_ok<NW>(d, s) ? d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z)) : 0.0;
-    } else {
-      int nz = __popcll(d.Z[0]);
-      if constexpr (NW == 2) nz += __popcll(d.Z[1]);
-      f = static_cast<double>(2 * popc_and<NW>(s, d.Z) - nz);
-    }
-    // reference: dy = c * x[i] * applyState(psi)
-    acc += d.c * xi * f;
-  }
-  return acc;
-}
 
 // y[i] = yscale * y_in[i] + sum_t H_{i,t} x[...] + d(s_i) x[i]
 // If DOT, partials[block] = sum over the block's rows of x[i] * y[i].
@@ -559,8 +496,9 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
 // ============================================================================
 // Block apply: a wavefront owns one row at a time, lanes own columns.
 // Lanes first test the row's terms in parallel (one term per lane, results
-// combined with a wavefront ballot), then each connected row rank J is found
-// once and the R-wide row X[J, :] is read with one coalesced load per lane.
+// combined with a wavefront ballot), then look up the row's connected states
+// in parallel (one per lane), and each rank is broadcast so the R-wide row
+// X[J, :] is read with one coalesced load per lane.
 // ============================================================================
 
 template <int NW>
@@ -595,17 +533,27 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_block(
       // lane `col` tests term t0 + col for its group's row
       const int t = t0 + col;
       const bool ok = row_ok && t < n_off && term_ok<NW>(off[t], s);
-      uint64_t bal = __ballot(ok);
-      uint64_t mine = (G == 64) ? bal : ((bal >> (grp * G)) & ((1ull << G) - 1));
-      // each set bit: one lookup, done redundantly by the group's lanes
-      // (same address -> one memory transaction per group)
-      while (mine) {
-        const int tt = t0 + __builtin_ctzll(mine);
-        mine &= mine - 1;
-        const OffTerm& T = off[tt];
-        const State<NW> sj = xor_state<NW>(s, T.X);
-        const int64_t J = basis_lookup<NW>(b, sj);
-        if (J >= 0 && col_ok) {
+      const uint64_t bal = __ballot(ok);
+      const uint64_t mine = (G == 64) ? bal : ((bal >> (grp * G)) & ((1ull << G) - 1));
+      const int nflip = __popcll(mine);
+      int maxflip = nflip;
+      for (int o = G; o < 64; o <<= 1) maxflip = max(maxflip, __shfl_xor(maxflip, o, 64));
+      // lane `col` looks up the group's flip number `col`; all in flight together
+      int my_term = -1;
+      int64_t my_J = -1;
+      if (col < nflip) {
+        uint64_t m = mine;
+        for (int q = 0; q < col; ++q) m &= m - 1;
+        my_term = t0 + __builtin_ctzll(m);
+        my_J = basis_lookup<NW>(b, xor_state<NW>(s, off[my_term].X));
+      }
+      // in term order: broadcast each rank, one coalesced read of X[J, :]
+      for (int j = 0; j < maxflip; ++j) {
+        const int src = grp * G + j;
+        const int64_t J = __shfl(my_J, src, 64);
+        const int tt = __shfl(my_term, src, 64);
+        if (j < nflip && J >= 0 && col_ok) {
+          const OffTerm& T = off[tt];
           const double xj = X[J * ldx + col];
           if (fabs(xj) > kApplyTol) {
             double c = (coeffs && T.slot >= 0) ? coeffs[T.slot * R + col] : T.c;
