@@ -133,6 +133,101 @@ def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
     }
 
 
+def bench_serve(states64, n_req=40, max_cols=16, nev=1, ncv=20, tol=1e-10, seed=99):
+    """Served solves: n_req ground-state requests at random (g, h) points
+    (g_s ~ -U[0.2, 1], |B| <= 0.1) spread over five spaces (32-spin and
+    48-spin sectors n_spinon_pairs = 0, 1 and the 64-spin Q=0 sector), FP64,
+    nev lowest eigenvalues to relative residual tol.
+      plain   one qsi_eigsh per request, one after another (the operator of
+              each point built beforehand, not timed)
+      served  SolveServer: all spaces resident, continuous batching of up to
+              max_cols requests per space; closed loop (all requests queued at
+              t = 0) for throughput, then Poisson arrivals at 70% of that
+              rate for p50/p99 latency."""
+    import torch
+    import qsi_subspace_ed_amd as q
+    from qsi_subspace_ed_amd import serve
+    names = {32: "pyro_2,0,0_0,2,0_0,0,2.json", 48: "pyro_2,0,0_0,2,0_0,0,3.json",
+             64: "pyro_2,0,0_0,2,0_0,0,4.json"}
+    jds = {W: q.load_lattice(q.lattice_path(n)) for W, n in names.items()}
+    spaces = []
+    for W in (32, 48):
+        for n in (0, 1):
+            spaces.append((f"{W}/n{n}", W, n, q.enumerate_basis(jds[W], n, n_threads=16)))
+    spaces.append(("64/n0", 64, 0, states64))
+    rng = np.random.default_rng(seed)
+    loc = q.get_loc_z()
+    reqs = []
+    for k in range(n_req):
+        key, W, n, states = spaces[int(rng.integers(len(spaces)))]
+        g = list(-rng.uniform(0.2, 1.0, size=4))
+        B = rng.standard_normal(3)
+        B *= rng.uniform(0.0, 0.1) / np.linalg.norm(B)
+        h = [float(B @ loc[s]) for s in range(4)]
+        reqs.append((key, W, g, h, 1000 + k))
+    by_key = {s[0]: s for s in spaces}
+    # plain: one solve at a time
+    bases = {s[0]: q.GpuBasis(s[3], s[1]) for s in spaces}
+    # the plain path solves an unslotted operator: the point's coefficients
+    # are baked into its own term list
+    plain_ops = []
+    for key, W, g, h, sd in reqs:
+        Hs = serve.slotted_hamiltonian(jds[W])
+        c = serve.point_coeffs(g, h)
+        Hp = q.SymbolicOpSum()
+        for (_, o, s) in list(Hs.off_diag_terms) + list(Hs.diagonal_terms):
+            Hp.add_term(float(c[s]), o)
+        plain_ops.append(q.GpuOp(bases[key], Hp))
+    plain_ops[0].eigsh(nev=nev, ncv=ncv, tol=tol, seed=1)  # warm-up
+    plain_evals, plain_t = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for op, r in zip(plain_ops, reqs):
+        ts = time.perf_counter()
+        ev, _, _, _ = op.eigsh(nev=nev, ncv=ncv, tol=tol, seed=r[4], raise_on_noconv=False)
+        plain_t.append(time.perf_counter() - ts)
+        plain_evals.append(ev)
+    t_plain = time.perf_counter() - t0
+    del plain_ops
+    # served
+    free, _ = torch.cuda.mem_get_info()
+    srv = serve.SolveServer(int(0.8 * free), max_cols=max_cols, nev=nev, ncv=ncv, tol=tol)
+    for key, W, n, states in spaces:
+        srv.add_space(key, jds[W], states, W)
+
+    def requests():
+        return [serve.Request(key, serve.point_coeffs(g, h), seed=sd, tag=i)
+                for i, (key, W, g, h, sd) in enumerate(reqs)]
+    srv.run([serve.Request(key, serve.point_coeffs(g, h), seed=1)
+             for key, W, g, h, sd in reqs[:len(spaces)]])  # warm-up: uploads, code load
+    srv.stats = serve.ServeStats()
+    closed = srv.run(requests())
+    st_closed = srv.stats.summary()
+    rate = 0.7 * st_closed["solves_per_s"]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    srv.stats = serve.ServeStats()
+    opened = srv.run(requests(), arrivals)
+    st_open = srv.stats.summary()
+    srv.close()
+    errs = []
+    for r in closed + opened:
+        pe = plain_evals[r.tag]
+        errs.append(float(np.max(np.abs(r.result.evals - pe) / np.maximum(1.0, np.abs(pe)))))
+    dims = {s[0]: int(s[3].shape[0]) for s in spaces}
+    return {
+        "workload": "%d requests, nev=%d ncv=%d tol=%g, spaces %s" % (n_req, nev, ncv, tol, dims),
+        "max_cols": max_cols,
+        "plain_solves_per_s": n_req / t_plain,
+        "plain_p50_ms": float(np.percentile(plain_t, 50) * 1e3),
+        "plain_p99_ms": float(np.percentile(plain_t, 99) * 1e3),
+        "served_closed_loop": st_closed,
+        "served_poisson": dict(st_open, arrival_rate_per_s=rate),
+        "speedup": st_closed["solves_per_s"] * t_plain / n_req,
+        "max_rel_eig_diff_vs_plain": max(errs),
+        "requests_per_space": {k: sum(1 for r in reqs if r[0] == k) for k in by_key},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +243,7 @@ def main():
                     help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")
     ap.add_argument("--bucket-bits", type=int, default=0, help="top-bit table width K (0 = auto)")
     ap.add_argument("--no-ftlm", action="store_true", help="skip the batched FTLM comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-solves comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -239,6 +335,14 @@ def main():
               f"{ftlm_res['plain_solves_per_s']:.0f} ({ftlm_res['speedup']:.1f}x)",
               file=sys.stderr, flush=True)
 
+    serve_res = None
+    if not args.no_serve and not args.no_solvers:
+        serve_res = bench_serve(states)
+        print(f"[bench] served {serve_res['served_closed_loop']['solves_per_s']:.1f} solves/s vs "
+              f"plain {serve_res['plain_solves_per_s']:.1f} ({serve_res['speedup']:.1f}x), "
+              f"Poisson p50/p99 {serve_res['served_poisson']['p50_ms']:.0f}/"
+              f"{serve_res['served_poisson']['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+
     out = {
         "metric": "H.v basis-states/s (64-spin Q=0 single-vector matrix-free apply)",
         "value": D * args.steps / wall,
@@ -266,6 +370,7 @@ def main():
         "lanczos": lanczos,
         "eigsh": eigsh,
         "ftlm_batched": ftlm_res,
+        "served": serve_res,
     }
     if not args.no_cpu:
         cb, y_cpu = cpu_baseline(states, H, xh)

@@ -234,6 +234,36 @@ int qsi_lanczos_block(const qsi_op_t *ops, int n_sectors, int R,
 int qsi_tridiag_eig(const double *d, const double *e, int n, double *evals,
                     double *first_row);
 
+/* ---- solve engine: continuous batching of eigensolves on one basis -------
+ * Each request is one qsi_eigsh solve (lowest nev eigenvalues, thick-restart
+ * Lanczos with full reorthogonalisation; counterpart of the reference's
+ * mfsparse solve, include/matrix_diag_bits.hpp:51-95) with its own slot
+ * coefficients, i.e. its own (g, B) point on a shared device-resident basis
+ * and term list.  Up to max_cols requests run at once, one per column of a
+ * block apply; every qsi_engine_step advances each of them by one Lanczos
+ * step in shared launches, admits queued requests into free columns and
+ * retires converged ones.  A request's eigenvalues do not depend on the other
+ * requests in the engine.  The start vector is a Gaussian generated on the
+ * device from the request's seed.  `op` must outlive the engine.  No
+ * counterpart in the reference (it solves one Hamiltonian per process). */
+typedef struct qsi_engine_s *qsi_engine_t;
+int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
+                      int max_restarts, qsi_engine_t *out);
+int qsi_engine_destroy(qsi_engine_t e);
+/* device bytes qsi_engine_create allocates for these sizes */
+int64_t qsi_engine_footprint(int64_t dim, int max_cols, int ncv);
+/* queue one request; h_coeffs: n_slots values (NULL if the op has no slots) */
+int qsi_engine_submit(qsi_engine_t e, const double *h_coeffs, uint64_t seed,
+                      int64_t *id);
+int qsi_engine_active(qsi_engine_t e);  /* requests holding a column */
+int qsi_engine_queued(qsi_engine_t e);  /* requests waiting for one  */
+/* admit + one Lanczos step for every active request; synchronises */
+int qsi_engine_step(qsi_engine_t e, int *n_active);
+/* pop one finished request: returns 1 and fills id, h_evals (nev), nconv and
+ * steps (Lanczos steps taken); returns 0 when none has finished */
+int qsi_engine_poll(qsi_engine_t e, int64_t *id, double *h_evals, int *nconv,
+                    int *steps);
+
 /* ---- vector kernels (streaming FP64, deterministic reductions) ----------- */
 int qsi_vec_dot(const double *d_a, const double *d_b, int64_t n, double *h_out,
                 qsi_stream_t stream); /* synchronises */

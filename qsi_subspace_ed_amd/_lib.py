@@ -109,6 +109,15 @@ SIGNATURES = {
     "qsi_lanczos_block": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P),
                                     C.POINTER(_P), C.c_int, _DP, _DP, C.c_int, _DP]),
     "qsi_tridiag_eig": (C.c_int, [_DP, _DP, C.c_int, _DP, _DP]),
+    "qsi_engine_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, C.c_int, C.POINTER(_P)]),
+    "qsi_engine_destroy": (C.c_int, [_P]),
+    "qsi_engine_footprint": (_I64, [_I64, C.c_int, C.c_int]),
+    "qsi_engine_submit": (C.c_int, [_P, _DP, C.c_uint64, C.POINTER(_I64)]),
+    "qsi_engine_active": (C.c_int, [_P]),
+    "qsi_engine_queued": (C.c_int, [_P]),
+    "qsi_engine_step": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "qsi_engine_poll": (C.c_int, [_P, C.POINTER(_I64), _DP, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]),
     "qsi_vec_dot": (C.c_int, [_P, _P, _I64, _DP, _P]),
     "qsi_vec_axpy": (C.c_int, [_D, _P, _P, _I64, _P]),
     "qsi_vec_scal": (C.c_int, [_D, _P, _I64, _P]),

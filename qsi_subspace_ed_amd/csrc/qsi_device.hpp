@@ -65,7 +65,7 @@ __device__ __forceinline__ double diag_accumulate(
     } else {
       int nz = __popcll(d.Z[0]);
       if constexpr (NW == 2) nz += __popcll(d.Z[1]);
-      f = static_cast<double>(2 * popc_and<NW>(s, d.Z) - nz);
+      f = d.sgn * static_cast<double>(2 * popc_and<NW>(s, d.Z) - nz);
     }
     // reference: dy = c * x[i] * applyState(psi)
     acc += d.c * xi * f;

@@ -26,6 +26,7 @@
 #include <new>
 #include <vector>
 
+#include "qsi_blockk.hpp"
 #include "qsi_device.hpp"
 #include "qsi_internal.hpp"
 #include "qsi_reduce.hpp"
@@ -493,96 +494,6 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
   }
 }
 
-// ============================================================================
-// Block apply: a wavefront owns one row at a time, lanes own columns.
-// Lanes first test the row's terms in parallel (one term per lane, results
-// combined with a wavefront ballot), then look up the row's connected states
-// in parallel (one per lane), and each rank is broadcast so the R-wide row
-// X[J, :] is read with one coalesced load per lane.
-// ============================================================================
-
-template <int NW>
-__global__ __launch_bounds__(kApplyBlock) void k_apply_block(
-    BasisView b, const OffTerm* __restrict__ off, int n_off,
-    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ X,
-    int64_t ldx, double* __restrict__ Y, int64_t ldy, int R,
-    const double* __restrict__ coeffs, int accumulate) {
-  const int lane = threadIdx.x % kWave;
-  const int64_t wave_global =
-      (static_cast<int64_t>(xcd_remap(blockIdx.x, gridDim.x)) * kApplyBlock +
-       threadIdx.x) /
-      kWave;
-  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * kApplyBlock / kWave;
-  // rows per wave pass: lanes are split into groups of G >= R lanes
-  int G = 64;
-  while (G / 2 >= R && G > 1) G >>= 1;
-  const int groups = kWave / G;
-  const int grp = lane / G;
-  const int col = lane % G;
-  const bool col_ok = col < R;
-
-  for (int64_t r0 = wave_global * groups; r0 < b.dim; r0 += n_waves * groups) {
-    const int64_t i = r0 + grp;
-    const bool row_ok = i < b.dim;
-    State<NW> s;
-    if (row_ok) s = load_state_cached<NW>(b.states, i);
-    else s = load_state_cached<NW>(b.states, 0);
-    double acc = 0.0;
-    if (accumulate && row_ok && col_ok) acc = Y[i * ldy + col];
-    for (int t0 = 0; t0 < n_off; t0 += G) {
-      // lane `col` tests term t0 + col for its group's row
-      const int t = t0 + col;
-      const bool ok = row_ok && t < n_off && term_ok<NW>(off[t], s);
-      const uint64_t bal = __ballot(ok);
-      const uint64_t mine = (G == 64) ? bal : ((bal >> (grp * G)) & ((1ull << G) - 1));
-      const int nflip = __popcll(mine);
-      int maxflip = nflip;
-      for (int o = G; o < 64; o <<= 1) maxflip = max(maxflip, __shfl_xor(maxflip, o, 64));
-      // lane `col` looks up the group's flip number `col`; all in flight together
-      int my_term = -1;
-      int64_t my_J = -1;
-      if (col < nflip) {
-        uint64_t m = mine;
-        for (int q = 0; q < col; ++q) m &= m - 1;
-        my_term = t0 + __builtin_ctzll(m);
-        my_J = basis_lookup<NW>(b, xor_state<NW>(s, off[my_term].X));
-      }
-      // in term order: broadcast each rank, one coalesced read of X[J, :]
-      for (int j = 0; j < maxflip; ++j) {
-        const int src = grp * G + j;
-        const int64_t J = __shfl(my_J, src, 64);
-        const int tt = __shfl(my_term, src, 64);
-        if (j < nflip && J >= 0 && col_ok) {
-          const OffTerm& T = off[tt];
-          const double xj = X[J * ldx + col];
-          if (fabs(xj) > kApplyTol) {
-            double c = (coeffs && T.slot >= 0) ? coeffs[T.slot * R + col] : T.c;
-            c *= T.sgn;
-            if (parity_notz<NW>(s, T.Z)) c = -c;
-            acc += c * xj;
-          }
-        }
-      }
-    }
-    if (row_ok && col_ok) {
-      const double xi = X[i * ldx + col];
-      for (int t = 0; t < n_diag; ++t) {
-        const DiagTerm& d = dg[t];
-        double f;
-        if (d.kind == 0) {
-          f = diag_ok<NW>(d, s) ? d.sgn * (1 - 2 * parity_notz<NW>(s, d.Z)) : 0.0;
-        } else {
-          int nz = __popcll(d.Z[0]);
-          if constexpr (NW == 2) nz += __popcll(d.Z[1]);
-          f = static_cast<double>(2 * popc_and<NW>(s, d.Z) - nz);
-        }
-        const double c = (coeffs && d.slot >= 0) ? coeffs[d.slot * R + col] : d.c;
-        acc += c * xi * f;
-      }
-      Y[i * ldy + col] = acc;
-    }
-  }
-}
 
 // ============================================================================
 // Connectivity (row-major triplets of the off-diagonal part + diagonal)
@@ -1225,12 +1136,16 @@ int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
       // that visits a spin twice reduces to one)
       const bool proj = (T.up[0] | T.up[1] | T.down[0] | T.down[1]) != 0;
       const int nz = __builtin_popcountll(T.Z[0]) + __builtin_popcountll(T.Z[1]);
-      if (nz == 1 && T.slot < 0 && !proj) {
-        // single-site Z: c * sign * Z_i; merge equal effective coefficients
-        const double ce = T.coeff * T.sign;
+      if (nz == 1 && !proj) {
+        // single-site Z: c * sign * Z_i.  Unslotted terms merge on equal
+        // c * sign; slotted ones on equal (slot, c, sign), so that slot
+        // coefficients (which replace c) apply to the whole group.
+        const bool slotted = T.slot >= 0;
+        const double ce = slotted ? T.coeff : T.coeff * T.sign;
+        const double sg = slotted ? static_cast<double>(T.sign) : 1.0;
         bool merged = false;
         for (auto& f : fields) {
-          if (f.c == ce) {
+          if (f.slot == T.slot && f.c == ce && f.sgn == sg) {
             f.Z[0] |= T.Z[0];
             f.Z[1] |= T.Z[1];
             merged = true;
@@ -1242,9 +1157,9 @@ int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
           d.Z[0] = T.Z[0];
           d.Z[1] = T.Z[1];
           d.c = ce;
-          d.sgn = 1.0;
+          d.sgn = sg;
           d.kind = 1;
-          d.slot = -1;
+          d.slot = T.slot;
           fields.push_back(d);
         }
         continue;
@@ -1375,24 +1290,32 @@ int qsi_op_apply_block(qsi_op_t op, const double* d_X, int64_t ldx, double* d_Y,
   if (op->basis->dim == 0) return QSI_OK;
   QSI_REQUIRE(d_X && d_Y, "qsi_op_apply_block: null matrix");
   QSI_REQUIRE(d_X != d_Y, "qsi_op_apply_block: X and Y must not alias");
-  // waves needed: one wave handles 64/G rows per pass
+  BlkArgs a{};
+  a.n_sec = 1;
+  a.R = R;
   int G = 64;
   while (G / 2 >= R && G > 1) G >>= 1;
-  const int64_t rows_per_wave = 64 / G;
-  const int64_t waves = (op->basis->dim + rows_per_wave - 1) / rows_per_wave;
-  int64_t nb = (waves + (kApplyBlock / kWave) - 1) / (kApplyBlock / kWave);
-  nb = std::min<int64_t>(nb, 65536);
+  a.G = G;
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.accumulate = accumulate ? 1 : 0;
+  a.y_colmajor = 0;
+  a.sec[0].b = op->basis->view();
+  a.sec[0].off = op->d_off;
+  a.sec[0].dg = op->d_diag;
+  a.sec[0].coeffs = d_coeffs;
+  a.sec[0].n_off = op->n_off;
+  a.sec[0].n_diag = op->n_diag;
+  a.n_tiles = (op->basis->dim + kTileRows - 1) / kTileRows;
+  const int64_t waves = std::min<int64_t>(a.n_tiles, kBlkMaxWaves);
+  const uint32_t nb = static_cast<uint32_t>((waves + kBlkWaves - 1) / kBlkWaves);
   auto s = static_cast<hipStream_t>(stream);
   if (op->basis->nw == 1)
-    hipLaunchKernelGGL(k_apply_block<1>, dim3(static_cast<uint32_t>(nb)),
-                       dim3(kApplyBlock), 0, s, op->basis->view(), op->d_off,
-                       op->n_off, op->d_diag, op->n_diag, d_X, ldx, d_Y, ldy, R,
-                       d_coeffs, accumulate);
+    hipLaunchKernelGGL(k_blk_apply<1>, dim3(nb), dim3(kBlkThreads), 0, s, a, d_X, d_Y,
+                       static_cast<double*>(nullptr));
   else
-    hipLaunchKernelGGL(k_apply_block<2>, dim3(static_cast<uint32_t>(nb)),
-                       dim3(kApplyBlock), 0, s, op->basis->view(), op->d_off,
-                       op->n_off, op->d_diag, op->n_diag, d_X, ldx, d_Y, ldy, R,
-                       d_coeffs, accumulate);
+    hipLaunchKernelGGL(k_blk_apply<2>, dim3(nb), dim3(kBlkThreads), 0, s, a, d_X, d_Y,
+                       static_cast<double*>(nullptr));
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }

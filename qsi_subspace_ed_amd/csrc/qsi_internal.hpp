@@ -26,8 +26,8 @@ struct OffTerm {
 
 // Diagonal term.  kind 0: c * sgn * (-1)^popcount(~s & Z) when (s & m) == v
 // (a Z string times up/down projectors, m = up | down, v = up), else 0;
-// kind 1: c * (2 popcount(s & Z) - popcount(Z))  (a sum of single-site Z
-// terms that share one coefficient, merged on the host).
+// kind 1: c * sgn * (2 popcount(s & Z) - popcount(Z))  (a sum of single-site
+// Z terms that share one coefficient and slot, merged on the host).
 struct DiagTerm {
   uint64_t m[2];
   uint64_t v[2];

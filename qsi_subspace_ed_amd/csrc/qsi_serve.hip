@@ -1,0 +1,653 @@
+// Continuous-batching eigensolver engine: many independent lowest-eigenvalue
+// requests on one device-resident basis share every H·v launch.
+//
+// Each request is the thick-restart Lanczos solve of qsi_eigsh (the
+// counterpart of compute_spectrum_iterative -> Spectra::SymEigsSolver,
+// include/matrix_diag_bits.hpp:51-95) with its own coefficients (its own g,
+// B) held in one column of the engine.  One engine step advances every active
+// column by one Lanczos step:
+//   gather    X[:, c] = V_c[:, j_c]                (D x R row-major block)
+//   apply     W = H_c X  (one block launch, per-column coefficients)
+//   transpose W -> column-major w_c
+//   2 x (h_c = V_c[:, 0..j_c]^T w_c ; w_c -= V_c h_c)   (all columns per launch)
+//   beta_c = |w_c| ; V_c[:, j_c+1] = w_c / beta_c
+// then the host updates each column's projected matrix and, when a column's
+// basis is full, runs its Rayleigh-Ritz step, convergence test and restart.
+// Requests join free columns between steps and leave when converged.  Every
+// per-column computation uses the same row tiling and summation order
+// whatever the other columns hold, so a request's result does not depend on
+// the mix it ran in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "qsi_blockk.hpp"
+#include "qsi_hostla.hpp"
+
+namespace qsi {
+namespace {
+
+constexpr int kSrvThreads = 256;
+constexpr int kSrvRowsPerBlock = 4096;
+constexpr int kHMax = 64;
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Gaussian start vector from a counter-based generator (Box-Muller on two
+// splitmix64 uniforms per entry): v depends on (seed, i) only.
+__global__ void k_srv_randn(double* __restrict__ v, int64_t D, uint64_t seed) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  const uint64_t a = splitmix64(seed ^ splitmix64(2 * static_cast<uint64_t>(i)));
+  const uint64_t b = splitmix64(seed ^ splitmix64(2 * static_cast<uint64_t>(i) + 1));
+  const double u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = (static_cast<double>(b >> 11) + 0.5) * 0x1.0p-53;
+  v[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// Start of a step, one block per 64-row tile: for every active column c,
+// finish normalising V_c[j_c] (scale invb[c] when pend[c]) and copy it to
+// the row-major apply input X[row, c]; X[row, c] = 0 for idle columns.
+__global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
+    double* __restrict__ Vb, int64_t ld, int ncv1, const int* __restrict__ colj,
+    const int* __restrict__ pend, const double* __restrict__ invb, int R, int64_t D,
+    double* __restrict__ X) {
+  __shared__ double tile[64][65];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+  const int64_t r = r0 + lane;
+  for (int c = wid; c < R; c += kSrvThreads / 64) {
+    const int j = colj[c];
+    double v = 0.0;
+    if (j >= 0 && r < D) {
+      double* vp = Vb + (static_cast<int64_t>(c) * ncv1 + j) * ld + r;
+      v = *vp;
+      if (pend[c]) {
+        v *= invb[c];
+        *vp = v;
+      }
+    }
+    tile[lane][c] = v;
+  }
+  __syncthreads();
+  const int64_t rows = min<int64_t>(64, D - r0);
+  for (int t = threadIdx.x; t < rows * R; t += kSrvThreads)
+    X[r0 * R + t] = tile[t / R][t % R];
+}
+
+// Per-block partial dot products of column c's w with its basis vectors
+// V_c[0..m), m = j_c + 1 <= MMAX, reduced over the block's rows into
+// partials[(c*nblk + b)*kHMax + i].
+template <int MMAX>
+__device__ __forceinline__ void srv_block_dots(double (&acc)[MMAX], int m, int c, int nblk,
+                                               double* __restrict__ partials) {
+  __shared__ double red[kSrvThreads / 64][MMAX];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < MMAX; ++i) {
+    if (i < m) {
+      const double v = wave_sum(acc[i]);
+      if (lane == 0) red[wid][i] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < m) {
+    double v = 0.0;
+    for (int k = 0; k < kSrvThreads / 64; ++k) v += red[k][threadIdx.x];
+    partials[(static_cast<int64_t>(c) * nblk + blockIdx.x) * kHMax + threadIdx.x] = v;
+  }
+}
+
+// pass 1: partials of V_c^T w_c.  Grid (nblk, R).
+template <int MMAX>
+__global__ __launch_bounds__(kSrvThreads) void k_srv_mdot(
+    const double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ Wc,
+    const int* __restrict__ colj, int64_t D, double* __restrict__ partials) {
+  const int c = blockIdx.y;
+  const int j = colj[c];
+  if (j < 0) return;
+  const int m = j + 1;
+  const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
+  const double* w = Wc + static_cast<int64_t>(c) * ld;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
+  const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
+  double acc[MMAX];
+#pragma unroll
+  for (int i = 0; i < MMAX; ++i) acc[i] = 0.0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kSrvThreads) {
+    const double wr = w[r];
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i)
+      if (i < m) acc[i] += V[i * ld + r] * wr;
+  }
+  srv_block_dots<MMAX>(acc, m, c, gridDim.x, partials);
+}
+
+// pass 1 -> 2: w_c -= V_c h_c (h from pass 1), then partials of V_c^T w_c.
+// Grid (nblk, R).  The block walks its rows in 64-row tiles; wave k holds
+// basis vectors i = k, k + 4, ... in registers, the four waves' parts of
+// (V h)[row] meet in LDS, and each wave reuses its V entries for the dots.
+template <int MMAX>
+__global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
+    const double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
+    const int* __restrict__ colj, int64_t D, double* __restrict__ Wc,
+    double* __restrict__ partials) {
+  constexpr int NWV = kSrvThreads / 64;
+  constexpr int Q = (MMAX + NWV - 1) / NWV;
+  const int c = blockIdx.y;
+  const int j = colj[c];
+  if (j < 0) return;
+  const int m = j + 1;
+  __shared__ double hs[MMAX];
+  __shared__ double ared[NWV][64];
+  __shared__ double red[MMAX];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x];
+  __syncthreads();
+  const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
+  double* w = Wc + static_cast<int64_t>(c) * ld;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
+  const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
+  double acc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+  for (int64_t rt = r0; rt < r1; rt += 64) {
+    const int64_t r = rt + lane;
+    const bool ok = r < r1;
+    double v[Q];
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = wid + NWV * q;
+      v[q] = (ok && i < m) ? V[i * ld + r] : 0.0;
+      if (i < m) a += v[q] * hs[i];
+    }
+    ared[wid][lane] = a;
+    __syncthreads();
+    double wr = 0.0;
+    if (ok) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < NWV; ++k) s += ared[k][lane];
+      wr = w[r] - s;
+      if (wid == 0) w[r] = wr;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] += v[q] * wr;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int i = wid + NWV * q;
+    const double t = wave_sum(acc[q]);
+    if (lane == 0 && i < m) red[i] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < m)
+    partials[(static_cast<int64_t>(c) * gridDim.x + blockIdx.x) * kHMax + threadIdx.x] =
+        red[threadIdx.x];
+}
+
+// pass 2: u = w_c - V_c h_c written to V_c[j_c + 1] (normalised by the next
+// step's prep), partials of |u|^2 in partials[c*nblk + b].
+template <int MMAX>
+__global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
+    double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
+    const int* __restrict__ colj, int64_t D, const double* __restrict__ Wc,
+    double* __restrict__ partials) {
+  const int c = blockIdx.y;
+  const int j = colj[c];
+  if (j < 0) return;
+  const int m = j + 1;
+  __shared__ double hs[MMAX];
+  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x];
+  __syncthreads();
+  const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
+  double* out = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
+  const double* w = Wc + static_cast<int64_t>(c) * ld;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
+  const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
+  double nrm = 0.0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kSrvThreads) {
+    double a = 0.0;
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i)
+      if (i < m) a += V[i * ld + r] * hs[i];
+    const double u = w[r] - a;
+    out[r] = u;
+    nrm += u * u;
+  }
+  nrm = block_sum<kSrvThreads>(nrm);
+  if (threadIdx.x == 0) partials[static_cast<int64_t>(c) * gridDim.x + blockIdx.x] = nrm;
+}
+
+// h[c*kHMax + i] = sum_b partials (fixed order); acc (optional) += h.
+// Grid (R, kHMax), one wave each.
+__global__ void k_srv_fin(const double* __restrict__ partials, int nblk,
+                          const int* __restrict__ colj, double* __restrict__ h,
+                          double* __restrict__ acc) {
+  const int c = blockIdx.x, i = blockIdx.y;
+  const int j = colj[c];
+  if (j < 0 || i > j) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64)
+    v += partials[(static_cast<int64_t>(c) * nblk + b) * kHMax + i];
+  v = wave_sum(v);
+  if (threadIdx.x == 0) {
+    h[c * kHMax + i] = v;
+    if (acc) acc[c * kHMax + i] += v;
+  }
+}
+
+// beta[c] = sqrt(sum), invb[c] = 1/beta (0 if it underflows).  Grid R.
+__global__ void k_srv_beta(const double* __restrict__ partials, int nblk,
+                           const int* __restrict__ colj, double* __restrict__ beta,
+                           double* __restrict__ invb) {
+  const int c = blockIdx.x;
+  if (colj[c] < 0) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64) v += partials[static_cast<int64_t>(c) * nblk + b];
+  v = wave_sum(v);
+  if (threadIdx.x == 0) {
+    const double r = sqrt(v);
+    beta[c] = r;
+    invb[c] = r > 1e-300 ? 1.0 / r : 0.0;
+  }
+}
+
+// invb[c] = 1 / sqrt(*n2) (start vector normalisation).
+__global__ void k_srv_setinv(const double* __restrict__ n2, int c, double* __restrict__ invb) {
+  if (threadIdx.x == 0) invb[c] = 1.0 / sqrt(*n2);
+}
+
+}  // namespace
+}  // namespace qsi
+
+using namespace qsi;
+
+struct qsi_engine_s {
+  struct Request {
+    int64_t id;
+    uint64_t seed;
+    std::vector<double> coeffs;  // n_slots
+  };
+  struct Result {
+    int64_t id;
+    std::vector<double> evals;
+    int nconv;
+    int steps;
+  };
+  struct Col {
+    bool active = false;
+    int64_t req = -1;
+    int j = 0;        // index of the vector being expanded
+    int k = 0;        // kept Ritz vectors after the last restart
+    int restarts = 0;
+    int steps = 0;
+    double beta_last = 0.0;
+    std::vector<double> T;  // ncv x ncv column-major
+  };
+  qsi_op_s* op = nullptr;
+  int R = 0, nev = 0, ncv = 0, ncv1 = 0, max_restarts = 0, mmax = 0;
+  double tol = 0.0;
+  int64_t D = 0, ld = 0;
+  int nblk = 0;
+  hipStream_t st = nullptr;
+  // device: Krylov bases (column c: ncv + 1 vectors of stride ld), the
+  // row-major apply input X (D x R), the apply output w (column-major),
+  // slot coefficients, projections, norms and reduction partials
+  double *Vb = nullptr, *X = nullptr, *Wc = nullptr, *coeffs = nullptr;
+  double *h = nullptr, *hacc = nullptr, *beta = nullptr, *invb = nullptr, *part = nullptr;
+  double *tmp = nullptr, *Ydev = nullptr;
+  int *colj_d = nullptr, *pend_d = nullptr;
+  std::vector<void*> allocs;
+  // host
+  std::vector<Col> cols;
+  std::vector<int> colj, pend;
+  std::vector<double> coeffs_h;
+  std::deque<Request> queue;
+  std::deque<Result> done;
+  int64_t next_id = 0;
+  BlkArgs args{};
+  int n_blocks = 0;
+
+  ~qsi_engine_s() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  template <typename T>
+  hipError_t alloc(T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n * sizeof(T), 8));
+    if (e == hipSuccess) {
+      allocs.push_back(q);
+      *p = static_cast<T*>(q);
+    }
+    return e;
+  }
+};
+
+namespace {
+
+// Free columns take queued requests: coefficients, a seeded Gaussian start
+// vector, and its normalisation left pending for the next prep.
+int engine_admit(qsi_engine_s* E) {
+  bool any = false;
+  for (int c = 0; c < E->R && !E->queue.empty(); ++c) {
+    auto& col = E->cols[c];
+    if (col.active) continue;
+    qsi_engine_s::Request rq = std::move(E->queue.front());
+    E->queue.pop_front();
+    col = qsi_engine_s::Col();
+    col.active = true;
+    col.req = rq.id;
+    col.T.assign(static_cast<size_t>(E->ncv) * E->ncv, 0.0);
+    for (int s = 0; s < E->op->n_slots; ++s) E->coeffs_h[s * E->R + c] = rq.coeffs[s];
+    E->pend[c] = 1;
+    double* v0 = E->Vb + static_cast<int64_t>(c) * E->ncv1 * E->ld;
+    hipLaunchKernelGGL(k_srv_randn, dim3(static_cast<uint32_t>((E->D + 255) / 256)), dim3(256), 0,
+                       E->st, v0, E->D, rq.seed);
+    QSI_CHECK_LAUNCH();
+    int rc = launch_dot(v0, v0, E->D, E->part + 8, E->part, E->st);
+    if (rc != QSI_OK) return rc;
+    hipLaunchKernelGGL(k_srv_setinv, dim3(1), dim3(64), 0, E->st, E->part, c, E->invb);
+    QSI_CHECK_LAUNCH();
+    any = true;
+  }
+  if (any)
+    QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_h.data(),
+                                 E->coeffs_h.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 E->st));
+  return QSI_OK;
+}
+
+// Rayleigh-Ritz, convergence test and thick restart of column c whose basis
+// holds m vectors (V_c[m], the residual direction, still unnormalised).
+int engine_ritz(qsi_engine_s* E, int c, int m, bool exhausted) {
+  auto& col = E->cols[c];
+  const int ncv = E->ncv;
+  std::vector<double> Tm(static_cast<size_t>(m) * m), theta, Y;
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i)
+      Tm[static_cast<size_t>(j) * m + i] = col.T[static_cast<size_t>(j) * ncv + i];
+  hostla::sym_eig(Tm, m, theta, Y);
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+  const int want = std::min(E->nev, m);
+  int nconv = 0;
+  for (int i = 0; i < want; ++i) {
+    const double resid = col.beta_last * std::fabs(Y[static_cast<size_t>(i) * m + (m - 1)]);
+    if (resid < E->tol * std::max(eps23, std::fabs(theta[i]))) ++nconv;
+    else break;
+  }
+  if (nconv >= want || exhausted || col.restarts >= E->max_restarts) {
+    qsi_engine_s::Result r;
+    r.id = col.req;
+    r.evals.assign(theta.begin(), theta.begin() + want);
+    r.evals.resize(E->nev, std::numeric_limits<double>::quiet_NaN());
+    r.nconv = nconv;
+    r.steps = col.steps;
+    E->done.push_back(std::move(r));
+    col.active = false;
+    E->pend[c] = 0;
+    return QSI_OK;
+  }
+  // keep the kn lowest Ritz vectors, then the residual direction
+  int kn = E->nev + std::min(nconv, (m - E->nev) / 2);
+  kn = std::max(1, std::min(kn, m - 1));
+  double* V = E->Vb + static_cast<int64_t>(c) * E->ncv1 * E->ld;
+  std::vector<double> Yk(Y.begin(), Y.begin() + static_cast<size_t>(kn) * m);
+  QSI_CHECK_HIP(hipMemcpyAsync(E->Ydev, Yk.data(), Yk.size() * sizeof(double),
+                               hipMemcpyHostToDevice, E->st));
+  int rc = launch_combine(V, E->ld, m, E->Ydev, kn, E->tmp, E->ld, E->D, E->st);
+  if (rc != QSI_OK) return rc;
+  QSI_CHECK_HIP(hipMemcpyAsync(V, E->tmp, static_cast<size_t>(kn) * E->ld * sizeof(double),
+                               hipMemcpyDeviceToDevice, E->st));
+  QSI_CHECK_HIP(hipMemcpyAsync(V + static_cast<int64_t>(kn) * E->ld,
+                               V + static_cast<int64_t>(m) * E->ld,
+                               static_cast<size_t>(E->D) * sizeof(double),
+                               hipMemcpyDeviceToDevice, E->st));
+  // the Ritz coefficients are host data read by the copies above
+  QSI_CHECK_HIP(hipStreamSynchronize(E->st));
+  std::fill(col.T.begin(), col.T.end(), 0.0);
+  for (int i = 0; i < kn; ++i) col.T[static_cast<size_t>(i) * ncv + i] = theta[i];
+  col.k = kn;
+  col.j = kn;
+  col.restarts += 1;
+  return QSI_OK;
+}
+
+template <int MMAX>
+void launch_reorth(qsi_engine_s* E, dim3 gcol, hipStream_t s) {
+  const int R = E->R;
+  hipLaunchKernelGGL(k_srv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld, E->ncv1,
+                     E->Wc, E->colj_d, E->D, E->part);
+  hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
+                     E->h, E->hacc);
+  hipLaunchKernelGGL(k_srv_gemv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
+                     E->ncv1, E->h, E->colj_d, E->D, E->Wc, E->part);
+  hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
+                     E->h, E->hacc);
+  hipLaunchKernelGGL(k_srv_gemv_norm<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
+                     E->ncv1, E->h, E->colj_d, E->D, E->Wc, E->part);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
+                      int max_restarts, qsi_engine_t* out) {
+  QSI_REQUIRE(out != nullptr && op != nullptr, "qsi_engine_create: null argument");
+  *out = nullptr;
+  QSI_REQUIRE(max_cols >= 1 && max_cols <= 64, "qsi_engine_create: max_cols must be in [1, 64]");
+  QSI_REQUIRE(nev >= 1 && ncv >= 2 && ncv <= 63, "qsi_engine_create: need nev >= 1, 2 <= ncv <= 63");
+  QSI_REQUIRE(op->basis->dim > 0, "qsi_engine_create: empty basis");
+  auto* E = new (std::nothrow) qsi_engine_s();
+  if (!E) {
+    set_error("qsi_engine_create: host allocation failed");
+    return QSI_E_NOMEM;
+  }
+  E->op = op;
+  E->D = op->basis->dim;
+  E->ncv = static_cast<int>(std::min<int64_t>(ncv, E->D));
+  E->nev = std::min(nev, std::max(E->ncv - 1, 1));
+  E->ncv1 = E->ncv + 1;
+  E->mmax = E->ncv <= 16 ? 16 : (E->ncv <= 24 ? 24 : (E->ncv <= 32 ? 32 : 64));
+  E->R = max_cols;
+  E->tol = tol;
+  E->max_restarts = max_restarts;
+  E->ld = (E->D + 63) / 64 * 64;
+  E->nblk = static_cast<int>((E->D + kSrvRowsPerBlock - 1) / kSrvRowsPerBlock);
+  BlkArgs& a = E->args;
+  a.n_sec = 1;
+  a.R = E->R;
+  int G = 64;
+  while (G / 2 >= E->R && G > 1) G >>= 1;
+  a.G = G;
+  a.ldx = E->R;
+  a.ldy = E->ld;
+  a.accumulate = 0;
+  a.y_colmajor = 1;
+  a.sec[0].b = op->basis->view();
+  a.sec[0].off = op->d_off;
+  a.sec[0].dg = op->d_diag;
+  a.sec[0].n_off = op->n_off;
+  a.sec[0].n_diag = op->n_diag;
+  a.sec[0].tile_begin = 0;
+  a.sec[0].row_off = 0;
+  a.n_tiles = (E->D + kTileRows - 1) / kTileRows;
+  const int64_t want_waves = std::min<int64_t>(a.n_tiles, kBlkMaxWaves);
+  E->n_blocks = static_cast<int>((want_waves + kBlkWaves - 1) / kBlkWaves);
+  const size_t R = static_cast<size_t>(E->R);
+  hipError_t e = hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking);
+  const int ns = std::max(op->n_slots, 1);
+  if (e == hipSuccess) e = E->alloc(&E->Vb, R * E->ncv1 * E->ld);
+  if (e == hipSuccess) e = E->alloc(&E->X, static_cast<size_t>(E->D) * R);
+  if (e == hipSuccess) e = E->alloc(&E->Wc, R * E->ld);
+  if (e == hipSuccess) e = E->alloc(&E->tmp, static_cast<size_t>(E->ncv1) * E->ld);
+  if (e == hipSuccess) e = E->alloc(&E->coeffs, static_cast<size_t>(ns) * R);
+  if (e == hipSuccess) e = E->alloc(&E->h, R * kHMax);
+  if (e == hipSuccess) e = E->alloc(&E->hacc, R * kHMax);
+  if (e == hipSuccess) e = E->alloc(&E->beta, R);
+  if (e == hipSuccess) e = E->alloc(&E->invb, R);
+  if (e == hipSuccess)
+    e = E->alloc(&E->part, std::max<size_t>(R * E->nblk * kHMax, static_cast<size_t>(max_vec_partials()) + 8));
+  if (e == hipSuccess) e = E->alloc(&E->Ydev, static_cast<size_t>(kHMax) * kHMax);
+  if (e == hipSuccess) e = E->alloc(&E->colj_d, R);
+  if (e == hipSuccess) e = E->alloc(&E->pend_d, R);
+  if (e == hipSuccess) e = hipMemset(E->Vb, 0, R * E->ncv1 * E->ld * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(E->invb, 0, R * sizeof(double));
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_engine_create: ") + hipGetErrorString(e));
+    delete E;
+    return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
+  }
+  E->cols.assign(E->R, qsi_engine_s::Col());
+  E->colj.assign(E->R, -1);
+  E->pend.assign(E->R, 0);
+  E->coeffs_h.assign(static_cast<size_t>(ns) * R, 0.0);
+  E->args.sec[0].coeffs = op->n_slots > 0 ? E->coeffs : nullptr;
+  *out = E;
+  return QSI_OK;
+}
+
+int qsi_engine_destroy(qsi_engine_t E) {
+  if (E) {
+    (void)hipStreamSynchronize(E->st);
+    delete E;
+  }
+  return QSI_OK;
+}
+
+// Bytes of device memory an engine with these parameters allocates (the
+// Krylov bases, X, w and the restart scratch; the small arrays aside).
+int64_t qsi_engine_footprint(int64_t dim, int max_cols, int ncv) {
+  const int64_t ld = (dim + 63) / 64 * 64;
+  const int64_t R = max_cols;
+  return 8 * (R * (ncv + 1) * ld + dim * R + R * ld + (ncv + 1) * ld);
+}
+
+int qsi_engine_submit(qsi_engine_t E, const double* h_coeffs, uint64_t seed, int64_t* id) {
+  QSI_REQUIRE(E && id, "qsi_engine_submit: null argument");
+  const int ns = E->op->n_slots;
+  QSI_REQUIRE(ns == 0 || h_coeffs, "qsi_engine_submit: the operator has slots; coefficients required");
+  qsi_engine_s::Request r;
+  r.id = E->next_id++;
+  r.seed = seed;
+  if (ns > 0) r.coeffs.assign(h_coeffs, h_coeffs + ns);
+  E->queue.push_back(std::move(r));
+  *id = E->queue.back().id;
+  return QSI_OK;
+}
+
+int qsi_engine_active(qsi_engine_t E) {
+  if (!E) return -1;
+  int n = 0;
+  for (auto& c : E->cols) n += c.active ? 1 : 0;
+  return n;
+}
+
+int qsi_engine_queued(qsi_engine_t E) { return E ? static_cast<int>(E->queue.size()) : -1; }
+
+// One engine step: admit queued requests into free columns, then one Lanczos
+// step for every active column.  *n_active (optional) = columns active after
+// the step.  Synchronises.
+int qsi_engine_step(qsi_engine_t E, int* n_active) {
+  QSI_REQUIRE(E != nullptr, "qsi_engine_step: null engine");
+  int rc = engine_admit(E);
+  if (rc != QSI_OK) return rc;
+  int nact = 0;
+  for (int c = 0; c < E->R; ++c) {
+    E->colj[c] = E->cols[c].active ? E->cols[c].j : -1;
+    nact += E->cols[c].active ? 1 : 0;
+  }
+  if (nact == 0) {
+    if (n_active) *n_active = 0;
+    return QSI_OK;
+  }
+  hipStream_t s = E->st;
+  const int64_t D = E->D;
+  const int R = E->R;
+  QSI_CHECK_HIP(hipMemcpyAsync(E->colj_d, E->colj.data(), R * sizeof(int), hipMemcpyHostToDevice, s));
+  QSI_CHECK_HIP(hipMemcpyAsync(E->pend_d, E->pend.data(), R * sizeof(int), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>((D + 63) / 64)), dim3(kSrvThreads), 0,
+                     s, E->Vb, E->ld, E->ncv1, E->colj_d, E->pend_d, E->invb, R, D, E->X);
+  QSI_CHECK_LAUNCH();
+  if (E->op->basis->nw == 1)
+    hipLaunchKernelGGL(k_blk_apply<1>, dim3(E->n_blocks), dim3(kBlkThreads), 0, s, E->args, E->X,
+                       E->Wc, static_cast<double*>(nullptr));
+  else
+    hipLaunchKernelGGL(k_blk_apply<2>, dim3(E->n_blocks), dim3(kBlkThreads), 0, s, E->args, E->X,
+                       E->Wc, static_cast<double*>(nullptr));
+  QSI_CHECK_LAUNCH();
+  const dim3 gcol(E->nblk, R);
+  QSI_CHECK_HIP(hipMemsetAsync(E->hacc, 0, static_cast<size_t>(R) * kHMax * sizeof(double), s));
+  if (E->mmax == 16) launch_reorth<16>(E, gcol, s);
+  else if (E->mmax == 24) launch_reorth<24>(E, gcol, s);
+  else if (E->mmax == 32) launch_reorth<32>(E, gcol, s);
+  else launch_reorth<64>(E, gcol, s);
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_srv_beta, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d, E->beta,
+                     E->invb);
+  QSI_CHECK_LAUNCH();
+  std::vector<double> hh(static_cast<size_t>(R) * kHMax), bb(R);
+  QSI_CHECK_HIP(hipMemcpyAsync(hh.data(), E->hacc, hh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  QSI_CHECK_HIP(hipMemcpyAsync(bb.data(), E->beta, R * sizeof(double), hipMemcpyDeviceToHost, s));
+  QSI_CHECK_HIP(hipStreamSynchronize(s));
+  const int ncv = E->ncv;
+  for (int c = 0; c < R; ++c) {
+    auto& col = E->cols[c];
+    if (!col.active) continue;
+    const int j = col.j;
+    col.steps += 1;
+    // projections onto the kept Ritz vectors only couple to the first new
+    // vector (arrowhead); later ones are rounding noise
+    for (int i = 0; i <= j; ++i) {
+      const double v = hh[static_cast<size_t>(c) * kHMax + i];
+      if ((i < col.k && j == col.k) || i >= col.k) {
+        col.T[static_cast<size_t>(j) * ncv + i] = v;
+        col.T[static_cast<size_t>(i) * ncv + j] = v;
+      }
+    }
+    double tn = 0.0;
+    for (int i = 0; i <= j; ++i) tn = std::max(tn, std::fabs(col.T[static_cast<size_t>(j) * ncv + i]));
+    const double beta = bb[c];
+    const bool exhausted = beta <= 1e-13 * std::max(tn, 1e-300);
+    col.beta_last = exhausted ? 0.0 : beta;
+    col.j = j + 1;
+    E->pend[c] = 1;  // V_c[j + 1] waits for its 1/beta in the next prep
+    if (col.j == ncv || exhausted) {
+      rc = engine_ritz(E, c, col.j, exhausted);
+      if (rc != QSI_OK) return rc;
+    }
+  }
+  int act = 0;
+  for (auto& col : E->cols) act += col.active ? 1 : 0;
+  if (n_active) *n_active = act;
+  return QSI_OK;
+}
+
+// Pops one finished request: returns 1 and fills *id, h_evals (nev values),
+// *nconv and *steps (Lanczos steps it took), or 0 when none is finished.
+int qsi_engine_poll(qsi_engine_t E, int64_t* id, double* h_evals, int* nconv, int* steps) {
+  if (!E || E->done.empty()) return 0;
+  auto& r = E->done.front();
+  if (id) *id = r.id;
+  if (h_evals) std::copy(r.evals.begin(), r.evals.end(), h_evals);
+  if (nconv) *nconv = r.nconv;
+  if (steps) *steps = r.steps;
+  E->done.pop_front();
+  return 1;
+}
+
+}  // extern "C"

@@ -162,6 +162,10 @@ class GpuOp:
         self.n_slots = ns
         self._n = basis.dim()
 
+    @property
+    def handle(self):
+        return self._h
+
     def set_variant(self, variant):
         """Choose the apply kernel: 'auto' (= 'lane8'), 'hash', 'hash4'
         (hash-index lookups; the basis needs hash_index=True),

@@ -1,0 +1,64 @@
+"""Host side of the serving layer (no GPU): the slotted Hamiltonian has the
+same terms as the plain one, slot coefficients reproduce the plain
+coefficients, and the engine footprint agrees with its buffer list."""
+import ctypes as C
+
+import numpy as np
+
+import qsi_subspace_ed_amd as q
+from qsi_subspace_ed_amd import _lib, serve
+
+
+def _lat(W):
+    from conftest import LATTICE_OF
+    return q.load_lattice(q.lattice_path(LATTICE_OF[W]))
+
+
+def test_slotted_terms_match_plain():
+    jd = _lat(32)
+    g = [-0.9, -0.35, -0.6, -0.45]
+    B = [0.05, -0.02, 0.04]
+    h = [float(np.asarray(B) @ q.get_loc_z()[s]) for s in range(4)]
+    Hs = serve.slotted_hamiltonian(jd)
+    assert Hs.n_slots() == serve.N_SLOTS
+    coeffs = serve.point_coeffs(g, h)
+    plain = q.lattice.add_ring_terms(q.SymbolicOpSum(), jd, g)
+    q.add_local_field(plain, jd, B)
+    got = [(coeffs[s], op) for (_, op, s) in list(Hs.off_diag_terms) + list(Hs.diagonal_terms)]
+    want = [(c, op) for (c, op, _) in list(plain.off_diag_terms) + list(plain.diagonal_terms)]
+    assert len(got) == len(want)
+    for (c1, o1), (c2, o2) in zip(got, want):
+        assert (o1.X, o1.Z, o1.up, o1.down, o1.sign) == (o2.X, o2.Z, o2.up, o2.down, o2.sign)
+        assert c1 == c2
+
+
+def test_jpm_point_matches_g_vals():
+    g, h = serve.jpm_point(0.02, [0.1, 0.0, 0.0])
+    assert g == q.g_vals(0.02, [0.1, 0.0, 0.0])
+    assert np.allclose(h, [0.1 / np.sqrt(3)] * 2 + [-0.1 / np.sqrt(3)] * 2)
+
+
+def test_point_coeffs_validation():
+    try:
+        serve.point_coeffs([1, 2, 3], [0, 0, 0, 0])
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("expected ValueError")
+
+
+def test_engine_footprint_formula():
+    lib = _lib.load()
+    for D, R, ncv in ((2970, 8, 24), (2756394, 16, 20), (1, 1, 2)):
+        ld = (D + 63) // 64 * 64
+        # Krylov bases R x (ncv + 1) x ld, X (D x R), w (R x ld), restart scratch
+        want = 8 * (R * (ncv + 1) * ld + D * R + R * ld + (ncv + 1) * ld)
+        assert lib.qsi_engine_footprint(D, R, ncv) == want
+
+
+def test_engine_create_validation():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.qsi_engine_create(None, 4, 1, 10, 1e-10, 10, C.byref(h)) == _lib.QSI_E_INVALID
+    assert lib.qsi_engine_poll(None, None, None, None, None) == 0
+    assert lib.qsi_engine_step(None, None) == _lib.QSI_E_INVALID
