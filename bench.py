@@ -84,9 +84,10 @@ def cpu_baseline(states, H, x, budget_s=12.0, threads=None):
 def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
     """Batched FTLM on the 48-spin diag(2,2,3) lattice: sectors
     n_spinon_pairs = 0 and 1 in one block launch, R random-sign start vectors
-    each, M Lanczos steps, FP64, g_s ~ U[0.2, 1], B = 0.  Compared with the
-    plain path: one single-vector Lanczos solve of M steps per sample, one
-    after another (timed on plain_samples samples per sector)."""
+    each, M Lanczos steps, FP64, g_s ~ U[0.2, 1], B = 0, with and without the
+    operators' connectivity cached.  Compared with the plain path: one
+    single-vector Lanczos solve of M steps per sample, one after another
+    (timed on plain_samples samples per sector), matrix-free and cached."""
     import torch
     import qsi_subspace_ed_amd as q
     from qsi_subspace_ed_amd import ftlm
@@ -100,46 +101,72 @@ def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
            for k, b in enumerate(bases)]
     ftlm.lanczos_block(ops, V0s, 3)  # warm-up (graph capture, code load)
     t0 = time.perf_counter()
+    _, _, dev_ms_search = ftlm.lanczos_block(ops, V0s, M, use_graph=True)
+    t_batched_search = time.perf_counter() - t0
+    # connectivity cached (the FTLM run applies each operator R*M times)
+    tc0 = time.perf_counter()
+    for op in ops:
+        op.cache_connectivity(True)
+    t_cache = time.perf_counter() - tc0
+    ftlm.lanczos_block(ops, V0s, 3)
+    t0 = time.perf_counter()
     al, be, dev_ms = ftlm.lanczos_block(ops, V0s, M, use_graph=True)
     t_batched = time.perf_counter() - t0
     _, _, dev_ms_nograph = ftlm.lanczos_block(ops, V0s, M, use_graph=False)
     # thermodynamics of sector 0 from the batch (a sanity figure)
     temps = np.array([0.05, 0.1, 0.2, 0.5, 1.0])
     th = ftlm.thermodynamics(ftlm.quadrature(al[0], be[0]), bases[0].shape[0], temps)
-    # plain: single-vector Lanczos solves, one at a time
-    plain = []
+    # plain: single-vector Lanczos solves, one at a time, with the cached
+    # connectivity (the faster single-vector kernel), then matrix-free
+    def time_plain():
+        out = []
+        for op in ops:
+            st = q.Settings(krylov_dim=M, abs_tol=0.0, rel_tol=0.0, min_iterations=10**9,
+                            max_iterations=M, x0_seed=0, calc_eigenvector=False)
+            op.lanczos_eigval0(st)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(plain_samples):
+                st.x0_seed = k + 1
+                op.lanczos_eigval0(st)
+            out.append((time.perf_counter() - t0) / plain_samples)
+        return out
+    plain_c = time_plain()
     for op in ops:
-        st = q.Settings(krylov_dim=M, abs_tol=0.0, rel_tol=0.0, min_iterations=10**9,
-                        max_iterations=M, x0_seed=0, calc_eigenvector=False)
-        op.lanczos_eigval0(st)  # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(plain_samples):
-            st.x0_seed = k + 1
-            op.lanczos_eigval0(st)
-        plain.append((time.perf_counter() - t0) / plain_samples)
+        op.cache_connectivity(False)
+    plain = time_plain()
     n_solves = 2 * R
     t_plain = R * sum(plain)  # 2R solves one after another
+    t_plain_c = R * sum(plain_c)
     return {
         "workload": "48-spin diag(2,2,3), sectors n_spinon_pairs=0,1 (D=%d,%d), R=%d, M=%d"
                     % (bases[0].shape[0], bases[1].shape[0], R, M),
         "batched_solves_per_s": n_solves / t_batched,
         "batched_device_ms": dev_ms, "batched_device_ms_no_graph": dev_ms_nograph,
         "batched_wall_s": t_batched,
+        "connectivity_cache_build_s": t_cache,
+        "batched_search_solves_per_s": n_solves / t_batched_search,
+        "batched_search_device_ms": dev_ms_search,
         "plain_solves_per_s": n_solves / t_plain,
         "plain_ms_per_solve": [1e3 * v for v in plain],
+        "plain_cached_solves_per_s": n_solves / t_plain_c,
+        "plain_cached_ms_per_solve": [1e3 * v for v in plain_c],
         "speedup": t_plain / t_batched,
+        "speedup_vs_plain_cached": t_plain_c / t_batched,
         "E_of_T_sector0": {"T": temps.tolist(), "E": th["E"].tolist(), "C": th["C"].tolist()},
     }
 
 
-def bench_serve(states64, n_req=40, max_cols=16, nev=1, ncv=20, tol=1e-10, seed=99):
-    """Served solves: n_req ground-state requests at random (g, h) points
-    (g_s ~ -U[0.2, 1], |B| <= 0.1) spread over five spaces (32-spin and
+def bench_serve(states64, n_req=40, max_cols=16, nev=5, ncv=15, tol=1e-10, seed=99):
+    """Served solves: n_req requests at random (g, B) points (g_s ~ U[0, 1]
+    per hexagon sublattice with H = -sum g_s (O + O^dagger), |B| ~ U[0, 0.2]
+    in a uniformly random direction) spread over five spaces (32-spin and
     48-spin sectors n_spinon_pairs = 0, 1 and the 64-spin Q=0 sector), FP64,
-    nev lowest eigenvalues to relative residual tol.
+    the nev lowest eigenvalues to relative residual tol with ncv Krylov
+    vectors (the mfsparse defaults).
       plain   one qsi_eigsh per request, one after another (the operator of
-              each point built beforehand, not timed)
+              each point built beforehand, not timed), started from the same
+              device-generated Gaussian vector as the served request
       served  SolveServer: all spaces resident, continuous batching of up to
               max_cols requests per space; closed loop (all requests queued at
               t = 0) for throughput, then Poisson arrivals at 70% of that
@@ -160,9 +187,9 @@ def bench_serve(states64, n_req=40, max_cols=16, nev=1, ncv=20, tol=1e-10, seed=
     reqs = []
     for k in range(n_req):
         key, W, n, states = spaces[int(rng.integers(len(spaces)))]
-        g = list(-rng.uniform(0.2, 1.0, size=4))
+        g = list(-rng.uniform(0.0, 1.0, size=4))
         B = rng.standard_normal(3)
-        B *= rng.uniform(0.0, 0.1) / np.linalg.norm(B)
+        B *= rng.uniform(0.0, 0.2) / np.linalg.norm(B)
         h = [float(B @ loc[s]) for s in range(4)]
         reqs.append((key, W, g, h, 1000 + k))
     by_key = {s[0]: s for s in spaces}
@@ -178,16 +205,25 @@ def bench_serve(states64, n_req=40, max_cols=16, nev=1, ncv=20, tol=1e-10, seed=
         for (_, o, s) in list(Hs.off_diag_terms) + list(Hs.diagonal_terms):
             Hp.add_term(float(c[s]), o)
         plain_ops.append(q.GpuOp(bases[key], Hp))
-    plain_ops[0].eigsh(nev=nev, ncv=ncv, tol=tol, seed=1)  # warm-up
-    plain_evals, plain_t = [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for op, r in zip(plain_ops, reqs):
-        ts = time.perf_counter()
-        ev, _, _, _ = op.eigsh(nev=nev, ncv=ncv, tol=tol, seed=r[4], raise_on_noconv=False)
-        plain_t.append(time.perf_counter() - ts)
-        plain_evals.append(ev)
-    t_plain = time.perf_counter() - t0
+    from qsi_subspace_ed_amd.device import randn
+    def run_plain():
+        plain_ops[0].eigsh(nev=nev, ncv=ncv, tol=tol, seed=1)  # warm-up
+        evals, times = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for op, r in zip(plain_ops, reqs):
+            ts = time.perf_counter()
+            v0 = randn(op.rows(), r[4])
+            ev, _, _, _ = op.eigsh(nev=nev, ncv=ncv, tol=tol, v0=v0, raise_on_noconv=False)
+            times.append(time.perf_counter() - ts)
+            evals.append(ev)
+        return evals, times, time.perf_counter() - t0
+    # matrix-free applies (the mfsparse path), then with each operator's
+    # connectivity cached (built beforehand, not timed)
+    plain_evals, plain_t, t_plain = run_plain()
+    for op in plain_ops:
+        op.cache_connectivity(True)
+    plain_c_evals, plain_c_t, t_plain_c = run_plain()
     del plain_ops
     # served
     free, _ = torch.cuda.mem_get_info()
@@ -213,6 +249,8 @@ def bench_serve(states64, n_req=40, max_cols=16, nev=1, ncv=20, tol=1e-10, seed=
     for r in closed + opened:
         pe = plain_evals[r.tag]
         errs.append(float(np.max(np.abs(r.result.evals - pe) / np.maximum(1.0, np.abs(pe)))))
+        pc = plain_c_evals[r.tag]
+        errs.append(float(np.max(np.abs(r.result.evals - pc) / np.maximum(1.0, np.abs(pc)))))
     dims = {s[0]: int(s[3].shape[0]) for s in spaces}
     return {
         "workload": "%d requests, nev=%d ncv=%d tol=%g, spaces %s" % (n_req, nev, ncv, tol, dims),
@@ -220,6 +258,10 @@ def bench_serve(states64, n_req=40, max_cols=16, nev=1, ncv=20, tol=1e-10, seed=
         "plain_solves_per_s": n_req / t_plain,
         "plain_p50_ms": float(np.percentile(plain_t, 50) * 1e3),
         "plain_p99_ms": float(np.percentile(plain_t, 99) * 1e3),
+        "plain_cached_solves_per_s": n_req / t_plain_c,
+        "plain_cached_p50_ms": float(np.percentile(plain_c_t, 50) * 1e3),
+        "plain_cached_p99_ms": float(np.percentile(plain_c_t, 99) * 1e3),
+        "speedup_vs_plain_cached": st_closed["solves_per_s"] * t_plain_c / n_req,
         "served_closed_loop": st_closed,
         "served_poisson": dict(st_open, arrival_rate_per_s=rate),
         "speedup": st_closed["solves_per_s"] * t_plain / n_req,
@@ -244,6 +286,8 @@ def main():
     ap.add_argument("--bucket-bits", type=int, default=0, help="top-bit table width K (0 = auto)")
     ap.add_argument("--no-ftlm", action="store_true", help="skip the batched FTLM comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-solves comparison")
+    ap.add_argument("--no-cached", action="store_true",
+                    help="skip the cached-connectivity apply figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -328,6 +372,49 @@ def main():
         eigsh = {"nev": 5, "ncv": 15, "tol": 1e-10, "seconds": te, "nconv": nconv,
                  "applies": napp, "evals": [float(v) for v in evals]}
 
+    # the same workload with the connectivity cached on the device
+    cached = None
+    if not args.no_cached:
+        torch.cuda.synchronize()
+        tc0 = time.perf_counter()
+        cnnz = op.cache_connectivity(True)
+        torch.cuda.synchronize()
+        t_build = time.perf_counter() - tc0
+        op.set_variant("cached")
+        yc = torch.zeros_like(y)
+        for _ in range(args.warmup):
+            op.evaluate(x, yc)
+        torch.cuda.synchronize()
+        ec0 = torch.cuda.Event(enable_timing=True)
+        ec1 = torch.cuda.Event(enable_timing=True)
+        tw0 = time.perf_counter()
+        ec0.record()
+        for _ in range(args.steps):
+            op.evaluate(x, yc)
+        ec1.record()
+        torch.cuda.synchronize()
+        wall_c = time.perf_counter() - tw0
+        kc = ec0.elapsed_time(ec1) / 1e3 / args.steps
+        cached = {"states_per_s": D * args.steps / wall_c, "apply_kernel_us": kc * 1e6,
+                  "build_s": t_build, "nnz": int(cnnz),
+                  "device_bytes": int(cnnz) * 5 + 4 * (D + 1),
+                  "algorithmic_GBps": bytes_apply / kc / 1e9,
+                  "bit_identical_to_search": bool(torch.equal(y, yc))}
+        if not args.no_solvers and args.lanczos_steps > 0:
+            n1, n2 = max(1, args.lanczos_steps // 4), args.lanczos_steps
+            run_lanczos(n1)
+            t1c = run_lanczos(n1)
+            t2c = run_lanczos(n2)
+            cached["lanczos_ms_per_step"] = (t2c - t1c) / (n2 - n1) * 1e3
+            te0 = time.perf_counter()
+            evc, _, nconv_c, napp_c = op.eigsh(nev=5, ncv=15, tol=1e-10, seed=0,
+                                               raise_on_noconv=False)
+            cached["eigsh"] = {"seconds": time.perf_counter() - te0, "applies": napp_c,
+                               "nconv": nconv_c, "evals": [float(v) for v in evc]}
+        print(f"[bench] cached connectivity: {kc * 1e6:.1f} us/apply "
+              f"(built in {t_build * 1e3:.1f} ms, {cnnz} nonzeros)", file=sys.stderr, flush=True)
+        op.cache_connectivity(False)
+
     ftlm_res = None
     if not args.no_ftlm and not args.no_solvers:
         ftlm_res = bench_ftlm()
@@ -369,6 +456,7 @@ def main():
         "algorithmic_bytes_per_apply": int(bytes_apply),
         "lanczos": lanczos,
         "eigsh": eigsh,
+        "cached_connectivity": cached,
         "ftlm_batched": ftlm_res,
         "served": serve_res,
     }

@@ -116,8 +116,11 @@ int qsi_op_n_diag(qsi_op_t op);
  * the hash index, 1/4 lookups in flight per lane.  QSI_APPLY_LANE{,2,4,8}: the
  * same through the top-bit bucket table and a bisection of the bucket.
  * QSI_APPLY_WINDOW: per wavefront, one bucket lookup per term, then the
- * connected rows search a 64-state window staged in LDS.  AUTO = LANE8 (the
- * fastest measured on MI355X: the sorted-basis bisection keeps the lookups of
+ * connected rows search a 64-state window staged in LDS.  QSI_APPLY_CACHED:
+ * gathers through the cached connectivity (qsi_op_cache_connectivity; setting
+ * it requires the cache, dropping the cache resets it to AUTO).  AUTO =
+ * CACHED when the connectivity is cached, else LANE8 (the fastest search
+ * measured on MI355X: the sorted-basis bisection keeps the lookups of
  * neighbouring rows in the same cache lines, which the hash index cannot).
  * All give bit-identical y. */
 enum {
@@ -128,7 +131,8 @@ enum {
   QSI_APPLY_LANE4 = 4,
   QSI_APPLY_LANE8 = 5,
   QSI_APPLY_HASH = 6,
-  QSI_APPLY_HASH4 = 7
+  QSI_APPLY_HASH4 = 7,
+  QSI_APPLY_CACHED = 8
 };
 int qsi_op_set_variant(qsi_op_t op, int variant);
 int qsi_op_get_variant(qsi_op_t op);
@@ -156,6 +160,18 @@ int qsi_op_apply_host(qsi_op_t op, const double *h_x, double *h_y,
 int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
                        double *d_Y, int64_t ldy, int R, const double *d_coeffs,
                        int accumulate, qsi_stream_t stream);
+
+/* Keep (enable = 1) or drop (0) the operator's connectivity on the device:
+ * for every row, in term order, the rank of each connected state and the
+ * term's coefficient class and sign (5 bytes per nonzero plus 4 per row).
+ * Block applies (qsi_op_apply_block, qsi_lanczos_block, qsi_engine_*) then
+ * gather through it instead of testing and searching, with identical
+ * results; worth it when the same operator is applied many times.  *nnz
+ * (optional) receives the cached nonzero count (-1 when not cached).
+ * Requires at most 8 distinct (slot, coeff) pairs among the off-diagonal
+ * terms.  Synchronises.  The counterpart of the reference's `sparse`
+ * algorithm, which materialises H (operator_matrix.hpp:79-105) instead. */
+int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
 
 /* Connectivity materialiser (the matrix-free equivalent of
  * LazyOpSum::toSparseMatrix, operator_matrix.hpp:79-105): writes the nonzero
@@ -259,6 +275,14 @@ int qsi_engine_active(qsi_engine_t e);  /* requests holding a column */
 int qsi_engine_queued(qsi_engine_t e);  /* requests waiting for one  */
 /* admit + one Lanczos step for every active request; synchronises */
 int qsi_engine_step(qsi_engine_t e, int *n_active);
+/* The two halves of qsi_engine_step, so that several engines (each on its
+ * own stream) overlap: step_begin admits and enqueues (*launched = 0 when no
+ * request is active), step_end waits and does the host work. */
+int qsi_engine_step_begin(qsi_engine_t e, int *launched);
+int qsi_engine_step_end(qsi_engine_t e, int *n_active);
+/* 1 when step_end would not wait (no step enqueued, or it has completed),
+ * 0 while the enqueued step runs; does not block */
+int qsi_engine_step_ready(qsi_engine_t e);
 /* pop one finished request: returns 1 and fills id, h_evals (nev), nconv and
  * steps (Lanczos steps taken); returns 0 when none has finished */
 int qsi_engine_poll(qsi_engine_t e, int64_t *id, double *h_evals, int *nconv,
@@ -270,6 +294,11 @@ int qsi_vec_dot(const double *d_a, const double *d_b, int64_t n, double *h_out,
 int qsi_vec_axpy(double alpha, const double *d_x, double *d_y, int64_t n,
                  qsi_stream_t stream);
 int qsi_vec_scal(double alpha, double *d_x, int64_t n, qsi_stream_t stream);
+/* d_x[i] = standard Gaussian draw determined by (seed, i) alone (counter-based
+ * generator); the start vectors of qsi_engine requests.  The reference's
+ * start vectors come from std::mt19937 on the host (lanczos.cpp:55-62);
+ * qsi_eigsh with d_v0 = NULL keeps those. */
+int qsi_vec_randn(double *d_x, int64_t n, uint64_t seed, qsi_stream_t stream);
 
 /* ---- basis enumeration (host) --------------------------------------------
  * Ice-rule depth-first enumeration (components/build_basis/src/pyro_tree.cpp:

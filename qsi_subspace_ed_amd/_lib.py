@@ -25,9 +25,10 @@ QSI_APPLY_LANE4 = 4
 QSI_APPLY_LANE8 = 5
 QSI_APPLY_HASH = 6
 QSI_APPLY_HASH4 = 7
+QSI_APPLY_CACHED = 8
 VARIANTS = {"auto": QSI_APPLY_AUTO, "window": QSI_APPLY_WINDOW, "lane": QSI_APPLY_LANE,
             "lane2": QSI_APPLY_LANE2, "lane4": QSI_APPLY_LANE4, "lane8": QSI_APPLY_LANE8,
-            "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4}
+            "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4, "cached": QSI_APPLY_CACHED}
 
 
 class QsiError(RuntimeError):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "qsi_op_apply": (C.c_int, [_P, _P, _P, C.c_int, _P]),
     "qsi_op_apply_host": (C.c_int, [_P, _P, _P, C.c_int]),
     "qsi_op_apply_block": (C.c_int, [_P, _P, _I64, _P, _I64, C.c_int, _P, C.c_int, _P]),
+    "qsi_op_cache_connectivity": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "qsi_op_connectivity": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P, _P]),
     "qsi_lanczos_eigval0": (C.c_int, [_P, C.POINTER(LanczosSettings), _DP, _P,
                                       C.POINTER(LanczosResult)]),
@@ -116,11 +118,15 @@ SIGNATURES = {
     "qsi_engine_active": (C.c_int, [_P]),
     "qsi_engine_queued": (C.c_int, [_P]),
     "qsi_engine_step": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "qsi_engine_step_begin": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "qsi_engine_step_end": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "qsi_engine_step_ready": (C.c_int, [_P]),
     "qsi_engine_poll": (C.c_int, [_P, C.POINTER(_I64), _DP, C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),
     "qsi_vec_dot": (C.c_int, [_P, _P, _I64, _DP, _P]),
     "qsi_vec_axpy": (C.c_int, [_D, _P, _P, _I64, _P]),
     "qsi_vec_scal": (C.c_int, [_D, _P, _I64, _P]),
+    "qsi_vec_randn": (C.c_int, [_P, _I64, C.c_uint64, _P]),
     "qsi_enumerate_basis": (C.c_int, [C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
                                       C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(_I64)]),
     "qsi_free": (None, [_P]),

@@ -215,6 +215,65 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
   }
 }
 
+// Cached variant: the row's connected ranks come from the operator's cached
+// connectivity (qsi_op_cache_connectivity), so an apply is a gather with no
+// flip tests or searches.  Same terms, order and arithmetic as the lane
+// variant, so y is bit-identical.
+constexpr int kCachedPipe = 8;
+
+template <int NW, bool DOT>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
+    BasisView b, const CoefClass* __restrict__ cls, int n_cls,
+    const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ cj,
+    const uint8_t* __restrict__ cm, const DiagTerm* __restrict__ dg, int n_diag,
+    const double* __restrict__ x, double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  double cc[kMaxCoefClasses];
+#pragma unroll
+  for (int k = 0; k < kMaxCoefClasses; ++k) cc[k] = k < n_cls ? cls[k].c : 0.0;
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  double prod = 0.0;
+  if (i < b.dim) {
+    const double xi = x[i];
+    double acc = y_in ? yscale * y_in[i] : 0.0;
+    const uint32_t e0 = rowptr[i], e1 = rowptr[i + 1];
+    for (uint32_t e = e0; e < e1; e += kCachedPipe) {
+      uint32_t J[kCachedPipe];
+      uint8_t mt[kCachedPipe];
+      double xj[kCachedPipe];
+#pragma unroll
+      for (int u = 0; u < kCachedPipe; ++u) {
+        const bool in = e + u < e1;
+        J[u] = in ? cj[e + u] : 0xFFFFFFFFu;
+        mt[u] = in ? cm[e + u] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kCachedPipe; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kCachedPipe; ++u) {
+        if (J[u] != 0xFFFFFFFFu && fabs(xj[u]) > kApplyTol) {
+          const int k = mt[u] & 0x7F;
+          double c = cc[0];
+#pragma unroll
+          for (int q = 1; q < kMaxCoefClasses; ++q)
+            if (k == q) c = cc[q];
+          if (mt[u] & 0x80) c = -c;
+          acc += c * xj[u];
+        }
+      }
+    }
+    if (n_diag > 0) acc = diag_accumulate<NW>(acc, xi, load_state_cached<NW>(b.states, i), dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
+}
+
 // Hash variant (default when the basis has a hash index): as the lane
 // variant, but each connected state is found with one probe of the hash index
 // (one 16-byte slot, ~1.2 probes on average) instead of a bucket bisection.
@@ -743,8 +802,14 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   const dim3 grid(static_cast<uint32_t>(nb)), block(kApplyBlock);
   const auto v = op->basis->view();
   int variant = op->variant;
-  if (variant == QSI_APPLY_AUTO) variant = QSI_APPLY_LANE8;
+  if (variant == QSI_APPLY_AUTO)
+    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE8;
   switch (variant) {
+    case QSI_APPLY_CACHED:
+      hipLaunchKernelGGL((k_apply_cached<NW, DOT>), grid, block, 0, s, v, op->d_cls, op->n_cls,
+                         op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
     case QSI_APPLY_LANE:
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
@@ -813,6 +878,34 @@ int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
                        dot_partials, apply_num_blocks(op), dot_out, 0);
     QSI_CHECK_LAUNCH();
   }
+  return QSI_OK;
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Gaussian entries from a counter-based generator: Box-Muller on two
+// splitmix64 (Steele, Lea & Flood 2014) uniforms per entry, so v[i] depends
+// on (seed, i) only, whatever the launch shape.
+__global__ void k_randn(double* __restrict__ v, int64_t n, uint64_t seed) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = splitmix64(seed ^ splitmix64(2 * static_cast<uint64_t>(i)));
+  const uint64_t b = splitmix64(seed ^ splitmix64(2 * static_cast<uint64_t>(i) + 1));
+  const double u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = (static_cast<double>(b >> 11) + 0.5) * 0x1.0p-53;
+  v[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+int launch_randn(double* v, int64_t n, uint64_t seed, hipStream_t s) {
+  if (n <= 0) return QSI_OK;
+  hipLaunchKernelGGL(k_randn, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, s, v,
+                     n, seed);
+  QSI_CHECK_LAUNCH();
   return QSI_OK;
 }
 
@@ -1192,6 +1285,27 @@ int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
     }
   }
   for (auto& f : fields) op->h_diag.push_back(f);
+  // coefficient classes: (slot, c) pairs of the off-diagonal terms
+  bool cls_ok = true;
+  for (auto& o : op->h_off) {
+    int k = 0;
+    while (k < static_cast<int>(op->h_cls.size()) &&
+           !(op->h_cls[k].slot == o.slot && op->h_cls[k].c == o.c))
+      ++k;
+    if (k == static_cast<int>(op->h_cls.size())) {
+      if (k == kMaxCoefClasses) {
+        cls_ok = false;
+        break;
+      }
+      op->h_cls.push_back(CoefClass{o.c, o.slot, 0});
+    }
+    o.cls = k;
+  }
+  if (!cls_ok) {
+    op->h_cls.clear();
+    for (auto& o : op->h_off) o.cls = -1;
+  }
+  op->n_cls = static_cast<int>(op->h_cls.size());
   op->n_off = static_cast<int>(op->h_off.size());
   op->n_diag = static_cast<int>(op->h_diag.size());
   op->n_slots = n_slots;
@@ -1204,9 +1318,15 @@ int qsi_op_create(qsi_basis_t b, const qsi_term_t* terms, int n_terms,
   if (e == hipSuccess && op->n_diag)
     e = hipMemcpy(op->d_diag, op->h_diag.data(), op->h_diag.size() * sizeof(DiagTerm),
                   hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMalloc(&op->d_cls, kMaxCoefClasses * sizeof(CoefClass));
+  if (e == hipSuccess && op->n_cls)
+    e = hipMemcpy(op->d_cls, op->h_cls.data(), op->h_cls.size() * sizeof(CoefClass),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     if (op->d_off) (void)hipFree(op->d_off);
     if (op->d_diag) (void)hipFree(op->d_diag);
+    if (op->d_cls) (void)hipFree(op->d_cls);
     delete op;
     set_error(std::string("qsi_op_create: ") + hipGetErrorString(e));
     return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
@@ -1221,6 +1341,10 @@ int qsi_op_destroy(qsi_op_t op) {
   if (!op) return QSI_OK;
   if (op->d_off) (void)hipFree(op->d_off);
   if (op->d_diag) (void)hipFree(op->d_diag);
+  if (op->d_cls) (void)hipFree(op->d_cls);
+  if (op->d_rowptr) (void)hipFree(op->d_rowptr);
+  if (op->d_cj) (void)hipFree(op->d_cj);
+  if (op->d_cm) (void)hipFree(op->d_cm);
   if (op->basis) basis_release(op->basis);
   delete op;
   return QSI_OK;
@@ -1230,8 +1354,10 @@ int64_t qsi_op_dim(qsi_op_t op) { return op ? op->basis->dim : -1; }
 
 int qsi_op_set_variant(qsi_op_t op, int variant) {
   QSI_REQUIRE(op != nullptr, "qsi_op_set_variant: null operator");
-  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_HASH4,
+  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_CACHED,
               "qsi_op_set_variant: unknown variant");
+  QSI_REQUIRE(variant != QSI_APPLY_CACHED || op->conn_nnz >= 0,
+              "qsi_op_set_variant: no cached connectivity (qsi_op_cache_connectivity)");
   QSI_REQUIRE(op->basis->d_hash || (variant != QSI_APPLY_HASH && variant != QSI_APPLY_HASH4),
               "qsi_op_set_variant: the basis has no hash index");
   op->variant = variant;
@@ -1303,21 +1429,18 @@ int qsi_op_apply_block(qsi_op_t op, const double* d_X, int64_t ldx, double* d_Y,
   a.sec[0].b = op->basis->view();
   a.sec[0].off = op->d_off;
   a.sec[0].dg = op->d_diag;
+  a.sec[0].cls = op->d_cls;
+  a.sec[0].n_cls = op->n_cls;
   a.sec[0].coeffs = d_coeffs;
   a.sec[0].n_off = op->n_off;
   a.sec[0].n_diag = op->n_diag;
+  a.sec[0].rowptr = op->d_rowptr;
+  a.sec[0].cj = op->d_cj;
+  a.sec[0].cm = op->d_cm;
   a.n_tiles = (op->basis->dim + kTileRows - 1) / kTileRows;
-  const int64_t waves = std::min<int64_t>(a.n_tiles, kBlkMaxWaves);
-  const uint32_t nb = static_cast<uint32_t>((waves + kBlkWaves - 1) / kBlkWaves);
-  auto s = static_cast<hipStream_t>(stream);
-  if (op->basis->nw == 1)
-    hipLaunchKernelGGL(k_blk_apply<1>, dim3(nb), dim3(kBlkThreads), 0, s, a, d_X, d_Y,
-                       static_cast<double*>(nullptr));
-  else
-    hipLaunchKernelGGL(k_blk_apply<2>, dim3(nb), dim3(kBlkThreads), 0, s, a, d_X, d_Y,
-                       static_cast<double*>(nullptr));
-  QSI_CHECK_LAUNCH();
-  return QSI_OK;
+  const int nb = blk_apply_blocks(op->basis->nw, a.n_tiles, blk_cached(a));
+  return launch_blk_apply(a, op->basis->nw, nb, d_X, d_Y, nullptr,
+                          static_cast<hipStream_t>(stream));
 }
 
 int qsi_op_connectivity(qsi_op_t op, int64_t* nnz, int64_t* d_rows,
@@ -1431,6 +1554,12 @@ int qsi_vec_scal(double alpha, double* d_x, int64_t n, qsi_stream_t stream) {
   if (n <= 0) return QSI_OK;
   QSI_REQUIRE(d_x, "qsi_vec_scal: null vector");
   return launch_scal(alpha, d_x, n, static_cast<hipStream_t>(stream));
+}
+
+int qsi_vec_randn(double* d_x, int64_t n, uint64_t seed, qsi_stream_t stream) {
+  if (n <= 0) return QSI_OK;
+  QSI_REQUIRE(d_x, "qsi_vec_randn: null vector");
+  return launch_randn(d_x, n, seed, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

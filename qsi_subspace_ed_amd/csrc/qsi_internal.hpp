@@ -21,8 +21,19 @@ struct OffTerm {
   double c;     // coefficient (single-vector apply)
   double sgn;   // +-1 operator sign
   int32_t slot; // coefficient slot for block applies (-1: use c)
+  int32_t cls;  // coefficient class (block applies), -1 when classes are off
+};
+
+// Off-diagonal terms of one coefficient class share their per-column
+// coefficient in a block apply: slot `slot` of the column (or `c` when the
+// apply has no per-column coefficients or slot < 0).  Block applies keep the
+// classes' coefficients in registers instead of loading each term's.
+struct CoefClass {
+  double c;
+  int32_t slot;
   int32_t pad;
 };
+constexpr int kMaxCoefClasses = 8;
 
 // Diagonal term.  kind 0: c * sgn * (-1)^popcount(~s & Z) when (s & m) == v
 // (a Z string times up/down projectors, m = up | down, v = up), else 0;
@@ -84,6 +95,16 @@ struct qsi_op_s {
   qsi::DiagTerm* d_diag = nullptr;
   std::vector<qsi::OffTerm> h_off;
   std::vector<qsi::DiagTerm> h_diag;
+  int n_cls = 0;  // coefficient classes (0: more than kMaxCoefClasses)
+  qsi::CoefClass* d_cls = nullptr;
+  std::vector<qsi::CoefClass> h_cls;
+  // cached connectivity (qsi_op_cache_connectivity): CSR over the rows, the
+  // rank of each connected state (0xFFFFFFFF if absent) and its coefficient
+  // class | negative-sign bit << 7, off-diagonal terms in term order
+  uint32_t* d_rowptr = nullptr;
+  uint32_t* d_cj = nullptr;
+  uint8_t* d_cm = nullptr;
+  int64_t conn_nnz = -1;  // -1: not cached
 };
 
 namespace qsi {
@@ -128,6 +149,7 @@ int launch_gemv_sub(const double* V, int64_t ldv, int m, const double* h,
 // out[:, 0:k] = V[:, 0:m] Y  (Y m x k column-major, device)
 int launch_combine(const double* V, int64_t ldv, int m, const double* Y, int k,
                    double* out, int64_t ldo, int64_t n, hipStream_t s);
+int launch_randn(double* v, int64_t n, uint64_t seed, hipStream_t s);
 int launch_copy_scaled(const double* s_dev, const double* y, double* x,
                        int64_t n, hipStream_t s);
 // y += c * s_dev[0] * x

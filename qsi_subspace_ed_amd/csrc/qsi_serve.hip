@@ -36,25 +36,6 @@ constexpr int kSrvThreads = 256;
 constexpr int kSrvRowsPerBlock = 4096;
 constexpr int kHMax = 64;
 
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-
-// Gaussian start vector from a counter-based generator (Box-Muller on two
-// splitmix64 uniforms per entry): v depends on (seed, i) only.
-__global__ void k_srv_randn(double* __restrict__ v, int64_t D, uint64_t seed) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= D) return;
-  const uint64_t a = splitmix64(seed ^ splitmix64(2 * static_cast<uint64_t>(i)));
-  const uint64_t b = splitmix64(seed ^ splitmix64(2 * static_cast<uint64_t>(i) + 1));
-  const double u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
-  const double u2 = (static_cast<double>(b >> 11) + 0.5) * 0x1.0p-53;
-  v[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-}
-
 // Start of a step, one block per 64-row tile: for every active column c,
 // finish normalising V_c[j_c] (scale invb[c] when pend[c]) and copy it to
 // the row-major apply input X[row, c]; X[row, c] = 0 for idle columns.
@@ -164,6 +145,8 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
   for (int64_t rt = r0; rt < r1; rt += 64) {
     const int64_t r = rt + lane;
     const bool ok = r < r1;
+    // every wave reads w[r] before the barrier; wave 0 overwrites it after
+    const double w_old = ok ? w[r] : 0.0;
     double v[Q];
     double a = 0.0;
 #pragma unroll
@@ -179,7 +162,7 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < NWV; ++k) s += ared[k][lane];
-      wr = w[r] - s;
+      wr = w_old - s;
       if (wid == 0) w[r] = wr;
     }
 #pragma unroll
@@ -265,6 +248,15 @@ __global__ void k_srv_beta(const double* __restrict__ partials, int nblk,
   }
 }
 
+// The step's results for the host, written by a kernel straight into mapped
+// pinned memory (no copy engine): hh = hacc (R x kHMax), bb = beta (R).
+__global__ void k_srv_out(const double* __restrict__ hacc, const double* __restrict__ beta,
+                          int R, double* hh, double* bb) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R * kHMax) hh[t] = hacc[t];
+  if (t < R) bb[t] = beta[t];
+}
+
 // invb[c] = 1 / sqrt(*n2) (start vector normalisation).
 __global__ void k_srv_setinv(const double* __restrict__ n2, int c, double* __restrict__ invb) {
   if (threadIdx.x == 0) invb[c] = 1.0 / sqrt(*n2);
@@ -311,6 +303,23 @@ struct qsi_engine_s {
   double *tmp = nullptr, *Ydev = nullptr;
   int *colj_d = nullptr, *pend_d = nullptr;
   std::vector<void*> allocs;
+  // pinned staging of the restarts' Ritz coefficients, kHMax^2 per column
+  double* Ystage = nullptr;
+  // pinned per-step exchange: column state down, projections and norms up
+  int* colj_p = nullptr;
+  int* pend_p = nullptr;
+  double* coeffs_p = nullptr;
+  double* hh_p = nullptr;
+  double* bb_p = nullptr;
+  void* pin = nullptr;
+  double* hh_dev = nullptr;  // device addresses of hh_p / bb_p (mapped)
+  double* bb_dev = nullptr;
+  hipEvent_t done_ev = nullptr;  // recorded after a step's last command
+  bool inflight = false;  // a step is enqueued and not yet finished
+  struct Restart {
+    int c, m, kn;
+  };
+  std::vector<Restart> restarts;
   // host
   std::vector<Col> cols;
   std::vector<int> colj, pend;
@@ -322,7 +331,11 @@ struct qsi_engine_s {
   int n_blocks = 0;
 
   ~qsi_engine_s() {
+    if (st) (void)hipStreamSynchronize(st);
     for (void* p : allocs) (void)hipFree(p);
+    if (Ystage) (void)hipHostFree(Ystage);
+    if (pin) (void)hipHostFree(pin);
+    if (done_ev) (void)hipEventDestroy(done_ev);
     if (st) (void)hipStreamDestroy(st);
   }
   template <typename T>
@@ -355,19 +368,21 @@ int engine_admit(qsi_engine_s* E) {
     for (int s = 0; s < E->op->n_slots; ++s) E->coeffs_h[s * E->R + c] = rq.coeffs[s];
     E->pend[c] = 1;
     double* v0 = E->Vb + static_cast<int64_t>(c) * E->ncv1 * E->ld;
-    hipLaunchKernelGGL(k_srv_randn, dim3(static_cast<uint32_t>((E->D + 255) / 256)), dim3(256), 0,
-                       E->st, v0, E->D, rq.seed);
-    QSI_CHECK_LAUNCH();
-    int rc = launch_dot(v0, v0, E->D, E->part + 8, E->part, E->st);
+    int rc = launch_randn(v0, E->D, rq.seed, E->st);
+    if (rc != QSI_OK) return rc;
+    rc = launch_dot(v0, v0, E->D, E->part + 8, E->part, E->st);
     if (rc != QSI_OK) return rc;
     hipLaunchKernelGGL(k_srv_setinv, dim3(1), dim3(64), 0, E->st, E->part, c, E->invb);
     QSI_CHECK_LAUNCH();
     any = true;
   }
-  if (any)
-    QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_h.data(),
-                                 E->coeffs_h.size() * sizeof(double), hipMemcpyHostToDevice,
-                                 E->st));
+  if (any) {
+    // through pinned memory: an async copy from pageable memory is not
+    // ordered with the kernels of a non-blocking stream
+    std::copy(E->coeffs_h.begin(), E->coeffs_h.end(), E->coeffs_p);
+    QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_p, E->coeffs_h.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, E->st));
+  }
   return QSI_OK;
 }
 
@@ -401,28 +416,41 @@ int engine_ritz(qsi_engine_s* E, int c, int m, bool exhausted) {
     E->pend[c] = 0;
     return QSI_OK;
   }
-  // keep the kn lowest Ritz vectors, then the residual direction
+  // keep the kn lowest Ritz vectors, then the residual direction; the
+  // device side runs after every column's host work (engine_restarts)
   int kn = E->nev + std::min(nconv, (m - E->nev) / 2);
   kn = std::max(1, std::min(kn, m - 1));
-  double* V = E->Vb + static_cast<int64_t>(c) * E->ncv1 * E->ld;
-  std::vector<double> Yk(Y.begin(), Y.begin() + static_cast<size_t>(kn) * m);
-  QSI_CHECK_HIP(hipMemcpyAsync(E->Ydev, Yk.data(), Yk.size() * sizeof(double),
-                               hipMemcpyHostToDevice, E->st));
-  int rc = launch_combine(V, E->ld, m, E->Ydev, kn, E->tmp, E->ld, E->D, E->st);
-  if (rc != QSI_OK) return rc;
-  QSI_CHECK_HIP(hipMemcpyAsync(V, E->tmp, static_cast<size_t>(kn) * E->ld * sizeof(double),
-                               hipMemcpyDeviceToDevice, E->st));
-  QSI_CHECK_HIP(hipMemcpyAsync(V + static_cast<int64_t>(kn) * E->ld,
-                               V + static_cast<int64_t>(m) * E->ld,
-                               static_cast<size_t>(E->D) * sizeof(double),
-                               hipMemcpyDeviceToDevice, E->st));
-  // the Ritz coefficients are host data read by the copies above
-  QSI_CHECK_HIP(hipStreamSynchronize(E->st));
+  std::copy(Y.begin(), Y.begin() + static_cast<size_t>(kn) * m,
+            E->Ystage + static_cast<size_t>(c) * kHMax * kHMax);
+  E->restarts.push_back({c, m, kn});
   std::fill(col.T.begin(), col.T.end(), 0.0);
   for (int i = 0; i < kn; ++i) col.T[static_cast<size_t>(i) * ncv + i] = theta[i];
   col.k = kn;
   col.j = kn;
   col.restarts += 1;
+  return QSI_OK;
+}
+
+// V_c[0..kn) = V_c[0..m) Y_c and V_c[kn] = V_c[m] for the staged restarts.
+// The staging buffer is rewritten only after the next step synchronises.
+int engine_restarts(qsi_engine_s* E) {
+  const size_t per = static_cast<size_t>(kHMax) * kHMax;
+  for (const auto& r : E->restarts) {
+    double* V = E->Vb + static_cast<int64_t>(r.c) * E->ncv1 * E->ld;
+    double* Yd = E->Ydev + r.c * per;
+    QSI_CHECK_HIP(hipMemcpyAsync(Yd, E->Ystage + r.c * per,
+                                 static_cast<size_t>(r.kn) * r.m * sizeof(double),
+                                 hipMemcpyHostToDevice, E->st));
+    int rc = launch_combine(V, E->ld, r.m, Yd, r.kn, E->tmp, E->ld, E->D, E->st);
+    if (rc != QSI_OK) return rc;
+    QSI_CHECK_HIP(hipMemcpyAsync(V, E->tmp, static_cast<size_t>(r.kn) * E->ld * sizeof(double),
+                                 hipMemcpyDeviceToDevice, E->st));
+    QSI_CHECK_HIP(hipMemcpyAsync(V + static_cast<int64_t>(r.kn) * E->ld,
+                                 V + static_cast<int64_t>(r.m) * E->ld,
+                                 static_cast<size_t>(E->D) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, E->st));
+  }
+  E->restarts.clear();
   return QSI_OK;
 }
 
@@ -481,13 +509,17 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   a.sec[0].b = op->basis->view();
   a.sec[0].off = op->d_off;
   a.sec[0].dg = op->d_diag;
+  a.sec[0].cls = op->d_cls;
+  a.sec[0].n_cls = op->n_cls;
+  a.sec[0].rowptr = op->d_rowptr;
+  a.sec[0].cj = op->d_cj;
+  a.sec[0].cm = op->d_cm;
   a.sec[0].n_off = op->n_off;
   a.sec[0].n_diag = op->n_diag;
   a.sec[0].tile_begin = 0;
   a.sec[0].row_off = 0;
   a.n_tiles = (E->D + kTileRows - 1) / kTileRows;
-  const int64_t want_waves = std::min<int64_t>(a.n_tiles, kBlkMaxWaves);
-  E->n_blocks = static_cast<int>((want_waves + kBlkWaves - 1) / kBlkWaves);
+  E->n_blocks = blk_apply_blocks(op->basis->nw, a.n_tiles, blk_cached(a));
   const size_t R = static_cast<size_t>(E->R);
   hipError_t e = hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking);
   const int ns = std::max(op->n_slots, 1);
@@ -502,11 +534,32 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   if (e == hipSuccess) e = E->alloc(&E->invb, R);
   if (e == hipSuccess)
     e = E->alloc(&E->part, std::max<size_t>(R * E->nblk * kHMax, static_cast<size_t>(max_vec_partials()) + 8));
-  if (e == hipSuccess) e = E->alloc(&E->Ydev, static_cast<size_t>(kHMax) * kHMax);
+  if (e == hipSuccess) e = E->alloc(&E->Ydev, R * kHMax * kHMax);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&E->Ystage), R * kHMax * kHMax * sizeof(double),
+                      hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc(&E->pin, R * (kHMax + 2 + ns) * sizeof(double),
+                      hipHostMallocMapped | hipHostMallocCoherent);
+  if (e == hipSuccess) {
+    E->hh_p = static_cast<double*>(E->pin);
+    E->bb_p = E->hh_p + R * kHMax;
+    E->coeffs_p = E->bb_p + R;
+    E->colj_p = reinterpret_cast<int*>(E->coeffs_p + R * ns);
+    E->pend_p = E->colj_p + R;
+    void* dp = nullptr;
+    e = hipHostGetDevicePointer(&dp, E->pin, 0);
+    E->hh_dev = static_cast<double*>(dp);
+    E->bb_dev = E->hh_dev + R * kHMax;
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&E->done_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = E->alloc(&E->colj_d, R);
   if (e == hipSuccess) e = E->alloc(&E->pend_d, R);
-  if (e == hipSuccess) e = hipMemset(E->Vb, 0, R * E->ncv1 * E->ld * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(E->invb, 0, R * sizeof(double));
+  // on the engine's own stream: it does not wait for the null stream, so a
+  // null-stream memset could still be running under the first step
+  if (e == hipSuccess) e = hipMemsetAsync(E->Vb, 0, R * E->ncv1 * E->ld * sizeof(double), E->st);
+  if (e == hipSuccess) e = hipMemsetAsync(E->invb, 0, R * sizeof(double), E->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(E->st);
   if (e != hipSuccess) {
     set_error(std::string("qsi_engine_create: ") + hipGetErrorString(e));
     delete E;
@@ -559,11 +612,14 @@ int qsi_engine_active(qsi_engine_t E) {
 
 int qsi_engine_queued(qsi_engine_t E) { return E ? static_cast<int>(E->queue.size()) : -1; }
 
-// One engine step: admit queued requests into free columns, then one Lanczos
-// step for every active column.  *n_active (optional) = columns active after
-// the step.  Synchronises.
-int qsi_engine_step(qsi_engine_t E, int* n_active) {
-  QSI_REQUIRE(E != nullptr, "qsi_engine_step: null engine");
+// First half of an engine step: admit queued requests into free columns and
+// enqueue one Lanczos step for every active column on the engine's stream.
+// *launched (optional) = 0 when no column is active (nothing enqueued).
+// Engines on different streams overlap between step_begin and step_end.
+int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
+  QSI_REQUIRE(E != nullptr, "qsi_engine_step_begin: null engine");
+  QSI_REQUIRE(!E->inflight, "qsi_engine_step_begin: the previous step is not finished");
+  if (launched) *launched = 0;
   int rc = engine_admit(E);
   if (rc != QSI_OK) return rc;
   int nact = 0;
@@ -571,25 +627,19 @@ int qsi_engine_step(qsi_engine_t E, int* n_active) {
     E->colj[c] = E->cols[c].active ? E->cols[c].j : -1;
     nact += E->cols[c].active ? 1 : 0;
   }
-  if (nact == 0) {
-    if (n_active) *n_active = 0;
-    return QSI_OK;
-  }
+  if (nact == 0) return QSI_OK;
   hipStream_t s = E->st;
   const int64_t D = E->D;
   const int R = E->R;
-  QSI_CHECK_HIP(hipMemcpyAsync(E->colj_d, E->colj.data(), R * sizeof(int), hipMemcpyHostToDevice, s));
-  QSI_CHECK_HIP(hipMemcpyAsync(E->pend_d, E->pend.data(), R * sizeof(int), hipMemcpyHostToDevice, s));
+  std::copy(E->colj.begin(), E->colj.end(), E->colj_p);
+  std::copy(E->pend.begin(), E->pend.end(), E->pend_p);
+  QSI_CHECK_HIP(hipMemcpyAsync(E->colj_d, E->colj_p, R * sizeof(int), hipMemcpyHostToDevice, s));
+  QSI_CHECK_HIP(hipMemcpyAsync(E->pend_d, E->pend_p, R * sizeof(int), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>((D + 63) / 64)), dim3(kSrvThreads), 0,
                      s, E->Vb, E->ld, E->ncv1, E->colj_d, E->pend_d, E->invb, R, D, E->X);
   QSI_CHECK_LAUNCH();
-  if (E->op->basis->nw == 1)
-    hipLaunchKernelGGL(k_blk_apply<1>, dim3(E->n_blocks), dim3(kBlkThreads), 0, s, E->args, E->X,
-                       E->Wc, static_cast<double*>(nullptr));
-  else
-    hipLaunchKernelGGL(k_blk_apply<2>, dim3(E->n_blocks), dim3(kBlkThreads), 0, s, E->args, E->X,
-                       E->Wc, static_cast<double*>(nullptr));
-  QSI_CHECK_LAUNCH();
+  rc = launch_blk_apply(E->args, E->op->basis->nw, E->n_blocks, E->X, E->Wc, nullptr, s);
+  if (rc != QSI_OK) return rc;
   const dim3 gcol(E->nblk, R);
   QSI_CHECK_HIP(hipMemsetAsync(E->hacc, 0, static_cast<size_t>(R) * kHMax * sizeof(double), s));
   if (E->mmax == 16) launch_reorth<16>(E, gcol, s);
@@ -600,41 +650,78 @@ int qsi_engine_step(qsi_engine_t E, int* n_active) {
   hipLaunchKernelGGL(k_srv_beta, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d, E->beta,
                      E->invb);
   QSI_CHECK_LAUNCH();
-  std::vector<double> hh(static_cast<size_t>(R) * kHMax), bb(R);
-  QSI_CHECK_HIP(hipMemcpyAsync(hh.data(), E->hacc, hh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  QSI_CHECK_HIP(hipMemcpyAsync(bb.data(), E->beta, R * sizeof(double), hipMemcpyDeviceToHost, s));
-  QSI_CHECK_HIP(hipStreamSynchronize(s));
-  const int ncv = E->ncv;
-  for (int c = 0; c < R; ++c) {
-    auto& col = E->cols[c];
-    if (!col.active) continue;
-    const int j = col.j;
-    col.steps += 1;
-    // projections onto the kept Ritz vectors only couple to the first new
-    // vector (arrowhead); later ones are rounding noise
-    for (int i = 0; i <= j; ++i) {
-      const double v = hh[static_cast<size_t>(c) * kHMax + i];
-      if ((i < col.k && j == col.k) || i >= col.k) {
-        col.T[static_cast<size_t>(j) * ncv + i] = v;
-        col.T[static_cast<size_t>(i) * ncv + j] = v;
+  hipLaunchKernelGGL(k_srv_out, dim3((R * kHMax + 255) / 256), dim3(256), 0, s, E->hacc, E->beta,
+                     R, E->hh_dev, E->bb_dev);
+  QSI_CHECK_LAUNCH();
+  QSI_CHECK_HIP(hipEventRecord(E->done_ev, s));
+  E->inflight = true;
+  if (launched) *launched = 1;
+  return QSI_OK;
+}
+
+// Second half: wait for the enqueued step, update every column's projected
+// matrix, run Rayleigh-Ritz / convergence / restart where a column's basis is
+// full, and enqueue the restarts.  *n_active (optional) = columns active
+// afterwards.  Without an enqueued step it only reports *n_active.
+int qsi_engine_step_end(qsi_engine_t E, int* n_active) {
+  QSI_REQUIRE(E != nullptr, "qsi_engine_step_end: null engine");
+  if (E->inflight) {
+    E->inflight = false;
+    QSI_CHECK_HIP(hipEventSynchronize(E->done_ev));
+    const int ncv = E->ncv;
+    for (int c = 0; c < E->R; ++c) {
+      auto& col = E->cols[c];
+      if (!col.active) continue;
+      const int j = col.j;
+      col.steps += 1;
+      // projections onto the kept Ritz vectors only couple to the first new
+      // vector (arrowhead); later ones are rounding noise
+      for (int i = 0; i <= j; ++i) {
+        const double v = E->hh_p[static_cast<size_t>(c) * kHMax + i];
+        if ((i < col.k && j == col.k) || i >= col.k) {
+          col.T[static_cast<size_t>(j) * ncv + i] = v;
+          col.T[static_cast<size_t>(i) * ncv + j] = v;
+        }
+      }
+      double tn = 0.0;
+      for (int i = 0; i <= j; ++i)
+        tn = std::max(tn, std::fabs(col.T[static_cast<size_t>(j) * ncv + i]));
+      const double beta = E->bb_p[c];
+      const bool exhausted = beta <= 1e-13 * std::max(tn, 1e-300);
+      col.beta_last = exhausted ? 0.0 : beta;
+      col.j = j + 1;
+      E->pend[c] = 1;  // V_c[j + 1] waits for its 1/beta in the next prep
+      if (col.j == ncv || exhausted) {
+        const int rc = engine_ritz(E, c, col.j, exhausted);
+        if (rc != QSI_OK) return rc;
       }
     }
-    double tn = 0.0;
-    for (int i = 0; i <= j; ++i) tn = std::max(tn, std::fabs(col.T[static_cast<size_t>(j) * ncv + i]));
-    const double beta = bb[c];
-    const bool exhausted = beta <= 1e-13 * std::max(tn, 1e-300);
-    col.beta_last = exhausted ? 0.0 : beta;
-    col.j = j + 1;
-    E->pend[c] = 1;  // V_c[j + 1] waits for its 1/beta in the next prep
-    if (col.j == ncv || exhausted) {
-      rc = engine_ritz(E, c, col.j, exhausted);
-      if (rc != QSI_OK) return rc;
-    }
+    const int rc = engine_restarts(E);
+    if (rc != QSI_OK) return rc;
   }
   int act = 0;
   for (auto& col : E->cols) act += col.active ? 1 : 0;
   if (n_active) *n_active = act;
   return QSI_OK;
+}
+
+// 1 when no step is enqueued or the enqueued one has completed (step_end
+// will not wait), 0 while it runs, negative on error.
+int qsi_engine_step_ready(qsi_engine_t E) {
+  if (!E) return QSI_E_INVALID;
+  if (!E->inflight) return 1;
+  const hipError_t e = hipEventQuery(E->done_ev);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  set_error(std::string("qsi_engine_step_ready: ") + hipGetErrorString(e));
+  return QSI_E_HIP;
+}
+
+// One engine step: step_begin then step_end.  Synchronises.
+int qsi_engine_step(qsi_engine_t E, int* n_active) {
+  const int rc = qsi_engine_step_begin(E, nullptr);
+  if (rc != QSI_OK) return rc;
+  return qsi_engine_step_end(E, n_active);
 }
 
 // Pops one finished request: returns 1 and fills *id, h_evals (nev values),

@@ -258,6 +258,8 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
   QSI_CHECK_HIP(bpart.alloc(std::max(max_vec_partials(), multidot_partials()) * sizeof(double)));
   QSI_CHECK_HIP(bh.alloc(3 * 64 * sizeof(double)));
   QSI_CHECK_HIP(bscal.alloc(4 * sizeof(double)));
+  DevBuf bY;  // Ritz coefficients of a restart (at most ncv x ncv)
+  QSI_CHECK_HIP(bY.alloc(static_cast<size_t>(ncv) * ncv * sizeof(double)));
   double* V = bV.d();
   double* W = bW.d();
   double* w = bw.d();
@@ -347,8 +349,6 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
     int kn = nev + std::min(nconv, (m - nev) / 2);
     kn = std::max(1, std::min(kn, m - 1));
     std::vector<double> Yk(Y.begin(), Y.begin() + static_cast<size_t>(kn) * m);
-    DevBuf bY;
-    QSI_CHECK_HIP(bY.alloc(Yk.size() * sizeof(double)));
     QSI_CHECK_HIP(hipMemcpyAsync(bY.d(), Yk.data(), Yk.size() * sizeof(double),
                                  hipMemcpyHostToDevice, s));
     QSI_TRY(launch_combine(V, ld, m, bY.d(), kn, W, ld, D, s));
@@ -368,8 +368,6 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
   if (n_evecs > 0) {
     const int ne = std::min(n_evecs, m);
     std::vector<double> Yk(Y.begin(), Y.begin() + static_cast<size_t>(ne) * m);
-    DevBuf bY;
-    QSI_CHECK_HIP(bY.alloc(Yk.size() * sizeof(double)));
     QSI_CHECK_HIP(hipMemcpyAsync(bY.d(), Yk.data(), Yk.size() * sizeof(double),
                                  hipMemcpyHostToDevice, s));
     QSI_TRY(launch_combine(V, ld, m, bY.d(), ne, d_evecs, D, D, s));

@@ -230,6 +230,15 @@ class GpuOp:
                                              R, cp, int(accumulate), _stream()),
               "qsi_op_apply_block")
 
+    def cache_connectivity(self, enable=True):
+        """Keep the connected ranks on the device so block applies gather
+        instead of searching (qsi_op_cache_connectivity).  Returns the cached
+        nonzero count (-1 when dropped)."""
+        nnz = C.c_int64(-1)
+        check(_lib.load().qsi_op_cache_connectivity(self._h, int(bool(enable)), C.byref(nnz)),
+              "qsi_op_cache_connectivity")
+        return nnz.value
+
     def connectivity(self):
         """(rows, cols, vals, diag) numpy arrays: off-diagonal H[row, col] in
         row-major order and the diagonal."""
@@ -301,6 +310,15 @@ class GpuOp:
             self.close()
         except Exception:
             pass
+
+
+def randn(n, seed):
+    """CUDA float64 vector of n standard Gaussian draws fixed by (seed, i)
+    (qsi_vec_randn; the start vectors of engine requests)."""
+    torch = _torch()
+    v = torch.empty(int(n), dtype=torch.float64, device="cuda")
+    check(_lib.load().qsi_vec_randn(_ptr(v), int(n), int(seed), _stream()), "qsi_vec_randn")
+    return v
 
 
 def enumerate_basis(jdata, n_spinon_pairs=0, order="greedy", n_threads=None):

@@ -18,10 +18,11 @@ converged (continuous batching).  A request's result does not depend on what
 else runs in the engine.
 
 Server.  `SolveServer` keeps spaces resident on the device under an HBM
-budget (basis + terms + engine buffers, from qsi_engine_footprint), uploads a
-space when a request for it arrives and the budget allows, evicts idle spaces
-least-recently-used, and steps every engine with work round-robin.  Latency is
-time from a request's arrival to its result.
+budget (basis + cached connectivity + terms + engine buffers), uploads a space
+when a request for it arrives and the budget allows, evicts idle spaces
+least-recently-used, and keeps every engine with work stepping on its own
+stream at its own pace (a small space is not held to the step time of a large
+one).  Latency is time from a request's arrival to its result.
 """
 import ctypes as C
 import time
@@ -75,8 +76,11 @@ class Finished:
 class Engine:
     """One qsi_engine_t on one GpuOp (see include/qsi_gpu.h)."""
 
-    def __init__(self, op: GpuOp, max_cols=16, nev=1, ncv=20, tol=1e-10, max_restarts=500):
+    def __init__(self, op: GpuOp, max_cols=16, nev=1, ncv=20, tol=1e-10, max_restarts=500,
+                 cache_connectivity=True):
         lib = _lib.load()
+        if cache_connectivity:
+            op.cache_connectivity(True)
         h = C.c_void_p()
         check(lib.qsi_engine_create(op.handle, max_cols, nev, ncv, tol, max_restarts,
                                     C.byref(h)), "qsi_engine_create")
@@ -84,6 +88,7 @@ class Engine:
         self.op = op
         self.nev = nev
         self.max_cols = max_cols
+        self.inflight = False
 
     @staticmethod
     def footprint(dim, max_cols, ncv):
@@ -106,6 +111,26 @@ class Engine:
         check(_lib.load().qsi_engine_step(self._h, C.byref(n)), "qsi_engine_step")
         return n.value
 
+    def step_begin(self):
+        """Admit and enqueue one step; False when nothing is active."""
+        n = C.c_int()
+        check(_lib.load().qsi_engine_step_begin(self._h, C.byref(n)), "qsi_engine_step_begin")
+        self.inflight = bool(n.value)
+        return self.inflight
+
+    def ready(self):
+        """True when step_end would not wait."""
+        rc = _lib.load().qsi_engine_step_ready(self._h)
+        if rc < 0:
+            check(rc, "qsi_engine_step_ready")
+        return rc == 1
+
+    def step_end(self):
+        n = C.c_int()
+        self.inflight = False
+        check(_lib.load().qsi_engine_step_end(self._h, C.byref(n)), "qsi_engine_step_end")
+        return n.value
+
     def poll(self):
         out = []
         lib = _lib.load()
@@ -124,7 +149,7 @@ class Engine:
         return int(_lib.load().qsi_engine_queued(self._h))
 
     def busy(self):
-        return self.active + self.queued > 0
+        return self.inflight or self.active + self.queued > 0
 
     def run_all(self):
         """Step until every submitted request finished; returns them."""
@@ -164,11 +189,14 @@ class Space:
 
     def footprint(self, max_cols, ncv):
         """Device bytes when resident: basis words + bucket table (at most
-        2^28 + 1 entries, about 16 per state) + terms + engine buffers."""
+        2^28 + 1 entries, about 16 per state) + cached connectivity (5 bytes
+        per nonzero, at most n_spins / 4 nonzeros per row on pyrochlore ring
+        terms, + 4 per row) + terms + engine buffers."""
         words = 1 if self.n_spins <= 64 else 2
         basis = self.dim * 8 * words + 4 * min(16 * self.dim + 1, (1 << 28) + 1)
-        terms = 64 * 1024 * 1024 // 64  # term tables are a few KB; a MiB covers them
-        return basis + terms + Engine.footprint(self.dim, max_cols, ncv)
+        conn = self.dim * (4 + 5 * max(1, self.n_spins // 4))
+        terms = 1 << 20  # term tables are a few KB; a MiB covers them
+        return basis + conn + terms + Engine.footprint(self.dim, max_cols, ncv)
 
 
 @dataclass
@@ -287,17 +315,27 @@ class SolveServer:
             self.inflight[(r.space, r.engine_id)] = r
 
     def step(self, now=None):
-        """Admit pending requests, step every busy engine once; returns the
-        requests that finished."""
+        """Admit pending requests and advance every engine whose previous
+        step has completed (never waits); returns the requests that
+        finished."""
         self._admit()
         done = []
+        # each engine runs on its own stream at its own pace: finish a step
+        # whose work has completed, then enqueue the next one; engines whose
+        # step is still running are left alone (nothing here blocks)
         for sp in self.spaces.values():
-            if sp.engine is None or not sp.engine.busy():
+            eng = sp.engine
+            if eng is None:
                 continue
-            sp.engine.step()
-            sp.last_used = time.perf_counter()
-            self.stats.steps += 1
-            for f in sp.engine.poll():
+            if eng.inflight:
+                if not eng.ready():
+                    continue
+                eng.step_end()
+                sp.last_used = time.perf_counter()
+                self.stats.steps += 1
+            if eng.busy():
+                eng.step_begin()
+            for f in eng.poll():
                 r = self.inflight.pop((sp.key, f.id))
                 r.result = f
                 r.finish = time.perf_counter() if now is None else now

@@ -54,7 +54,7 @@ def test_search_ranks(W, kbits, cuda):
     assert np.all(basis.search(qs[:10]) == -1)
 
 
-@pytest.mark.parametrize("variant", ["auto", "window", "lane", "lane8", "hash"])
+@pytest.mark.parametrize("variant", ["auto", "window", "lane", "lane8", "hash", "cached"])
 @pytest.mark.parametrize("W,kind", [(8, "ring"), (32, "ring"), (32, "ring+field"),
                                     (48, "ring+field"), (48, "jpm")])
 def test_apply_parity(W, kind, variant, lattices, cuda):
@@ -64,6 +64,8 @@ def test_apply_parity(W, kind, variant, lattices, cuda):
     D = b.shape[0]
     H = _ham(jd, W, kind)
     op = q.GpuOp(q.GpuBasis(b, W, hash_index=True), H)
+    if variant == "cached":
+        op.cache_connectivity(True)
     op.set_variant(variant)
     ob = orc.Basis(b)
     x = _rand_x(D, W)
@@ -119,12 +121,24 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
     op = q.GpuOp(q.GpuBasis(b, W, bucket_bits=kbits, hash_index=True), H)
     x = torch.from_numpy(_rand_x(b.shape[0], 9)).cuda()
     ys = []
-    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4"):
+    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "cached"):
+        if v == "cached":
+            op.cache_connectivity(True)
         op.set_variant(v)
         y = torch.zeros_like(x)
         op.evaluate(x, y)
         ys.append(y)
-    assert torch.equal(ys[0], ys[1])
+    for y in ys[1:]:
+        assert torch.equal(ys[0], y)
+    # y += H x and y = 0.5 y + H x through the cached kernel match too
+    y0 = torch.from_numpy(_rand_x(b.shape[0], 10)).cuda()
+    y1 = y0.clone()
+    op.evaluate_add(x, y1)
+    op.set_variant("lane8")
+    y2 = y0.clone()
+    op.evaluate_add(x, y2)
+    assert torch.equal(y1, y2)
+    assert op.cache_connectivity(False) == -1
 
 
 def test_apply_partial_basis(lattices, cuda):
@@ -244,3 +258,31 @@ def test_block_apply_per_column_coefficients(lattices, cuda):
         ref = orc.evaluate_add(ob, H, X[:, r].cpu().numpy(), np.zeros(D))
         got = Y[:, r].cpu().numpy()
         assert np.linalg.norm(got - ref) <= HV_TOL * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("R", [1, 5, 16, 64])
+def test_cached_connectivity_block_apply_identical(R, lattices, cuda):
+    """Block applies through the cached connectivity give the same bits as
+    the searching kernel, with and without per-column coefficients."""
+    torch = cuda
+    W = 48
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    Hs = q.lattice.add_ring_terms(q.SymbolicOpSum(), jd, [0.0] * 4, slots=True)
+    q.add_local_field(Hs, jd, [0.0, 0.0, 0.0], slots=True)
+    op = q.GpuOp(q.GpuBasis(b, W), Hs, n_slots=8)
+    rng = np.random.default_rng(R)
+    X = torch.from_numpy(rng.standard_normal((D, R))).cuda()
+    co = torch.from_numpy(rng.standard_normal((8, R))).cuda()
+    Y0 = torch.zeros((D, R), dtype=torch.float64, device="cuda")
+    Y1 = torch.zeros_like(Y0)
+    op.apply_block(X, Y0, coeffs=co)
+    nnz = op.cache_connectivity(True)
+    assert nnz == 703680  # off-diagonal nonzeros of the 48-spin n=0 fixture
+    op.apply_block(X, Y1, coeffs=co)
+    assert torch.equal(Y0, Y1)
+    op.apply_block(X, Y0)
+    op.apply_block(X, Y1, accumulate=False)
+    assert torch.equal(Y0, Y1)
+    assert op.cache_connectivity(False) == -1

@@ -95,3 +95,23 @@ def test_exact_trace_with_complete_start_set(lattices, cuda):
     th = ftlm.thermodynamics(nodes, D, temps, e_ref=e_ref, normalise=False)
     assert np.allclose(th["Z_shifted"], Z_exact, rtol=1e-10)
     assert np.allclose(th["E"], E_exact, rtol=1e-10, atol=1e-12)
+
+
+def test_cached_connectivity_same_coefficients(lattices, cuda):
+    """Block Lanczos through cached connectivity: the same recurrence (the
+    per-wave reduction grid differs, so 1e-12 relative, not bits)."""
+    torch = cuda
+    jd = lattices[48]
+    H = _ham(jd, [0.7, 0.3, 0.9, 0.5], [0.02, 0.01, -0.03])
+    bases = [q.enumerate_basis(jd, n, n_threads=8) for n in (0, 1)]
+    ops = [q.GpuOp(q.GpuBasis(bb, 48), H) for bb in bases]
+    R, M = 8, 30
+    V0s = [torch.from_numpy(ftlm.random_sign_block(bb.shape[0], R, seed=3 + k)).cuda()
+           for k, bb in enumerate(bases)]
+    a0, b0, _ = ftlm.lanczos_block(ops, V0s, M)
+    for op in ops:
+        assert op.cache_connectivity(True) > 0
+    a1, b1, _ = ftlm.lanczos_block(ops, V0s, M)
+    sc = np.max(np.abs(a0))
+    assert np.max(np.abs(a0 - a1)) <= 1e-12 * sc
+    assert np.max(np.abs(b0 - b1)) <= 1e-12 * sc

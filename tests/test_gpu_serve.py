@@ -145,3 +145,109 @@ def test_server_budget_eviction(lattices, cuda):
         eng.close()
     for r in out:
         assert np.array_equal(r.result.evals, ref[r.tag]), r.tag
+
+
+def test_randn_deterministic(cuda):
+    """qsi_vec_randn: entries fixed by (seed, i); standard normal moments."""
+    from qsi_subspace_ed_amd.device import randn
+    a = randn(200001, 7).cpu().numpy()
+    b = randn(200001, 7).cpu().numpy()
+    c = randn(1000, 7).cpu().numpy()
+    assert np.array_equal(a, b) and np.array_equal(a[:1000], c)
+    assert not np.array_equal(a[:1000], randn(1000, 8).cpu().numpy())
+    assert abs(a.mean()) < 0.01 and abs(a.std() - 1.0) < 0.01
+
+
+def test_engine_matches_eigsh_same_start(lattices, cuda):
+    """An engine request and qsi_eigsh started from the same device Gaussian
+    vector converge to the same nev = 5 eigenvalues (1e-9 relative)."""
+    from qsi_subspace_ed_amd.device import randn
+    jd = lattices[48]
+    b = load_basis(48)
+    basis = q.GpuBasis(b, 48)
+    op = q.GpuOp(basis, serve.slotted_hamiltonian(jd))
+    eng = serve.Engine(op, max_cols=4, nev=5, ncv=15, tol=1e-10)
+    pts = POINTS[:3]
+    ids = {eng.submit(serve.point_coeffs(g, _h_of(B)), seed=500 + k): k
+           for k, (g, B) in enumerate(pts)}
+    for f in eng.run_all():
+        k = ids[f.id]
+        g, B = pts[k]
+        ev, _, nconv, _ = q.GpuOp(basis, _plain_h(jd, g, B)).eigsh(
+            nev=5, ncv=15, tol=1e-10, v0=randn(b.shape[0], 500 + k))
+        assert f.nconv == 5 and nconv == 5
+        for i in range(5):
+            assert _close(f.evals[i], ev[i]), (k, i, f.evals[i], ev[i])
+
+
+def test_engine_cached_connectivity_bit_identical(lattices, cuda):
+    """The engine gives the same bits with and without cached connectivity."""
+    jd = lattices[32]
+    b = load_basis(32)
+    basis = q.GpuBasis(b, 32)
+    out = []
+    for cache in (False, True):
+        op = q.GpuOp(basis, serve.slotted_hamiltonian(jd))
+        eng = serve.Engine(op, max_cols=4, nev=3, ncv=16, tol=1e-10, cache_connectivity=cache)
+        for k, (g, B) in enumerate(POINTS[:4]):
+            eng.submit(serve.point_coeffs(g, _h_of(B)), seed=70 + k)
+        out.append({f.id: f.evals for f in eng.run_all()})
+        eng.close()
+    assert out[0].keys() == out[1].keys()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_server_concurrent_engines_same_bits(lattices, cuda):
+    """Several resident spaces step side by side on their own streams; every
+    request gets the same bits as on a lone engine."""
+    jd32, jd48 = lattices[32], lattices[48]
+    spaces = {"32/0": (jd32, load_basis(32), 32),
+              "32/1": (jd32, q.enumerate_basis(jd32, 1, n_threads=8), 32),
+              "48/0": (jd48, load_basis(48), 48)}
+    kw = dict(max_cols=4, nev=3, ncv=12, tol=1e-10)
+    srv = serve.SolveServer(1 << 40, **kw)
+    for key, (jd, b, W) in spaces.items():
+        srv.add_space(key, jd, b, W)
+    keys = list(spaces)
+    reqs = [serve.Request(keys[k % 3], serve.point_coeffs(POINTS[k % 5][0], _h_of(POINTS[k % 5][1])),
+                          seed=900 + k, tag=k) for k in range(10)]
+    out = srv.run(reqs)
+    srv.close()
+    assert len(out) == len(reqs)
+    for key, (jd, b, W) in spaces.items():
+        op = q.GpuOp(q.GpuBasis(b, W), serve.slotted_hamiltonian(jd))
+        eng = serve.Engine(op, **kw)
+        for r in reqs:
+            if r.space == key:
+                eng.submit(r.coeffs, r.seed)
+                ref = eng.run_all()[0]
+                got = r.result
+                assert got.steps == ref.steps, (r.tag, got.steps, ref.steps)
+                assert np.array_equal(got.evals, ref.evals), (r.tag, got.evals, ref.evals)
+        eng.close()
+
+
+def test_engine_first_step_after_create(lattices, cuda):
+    """Requests admitted in the very first step of a freshly created engine
+    on a large basis (its creation-time zeroing must be finished by then)
+    converge like the same requests admitted later."""
+    jd = lattices[48]
+    b = q.enumerate_basis(jd, 1, n_threads=8)
+    basis = q.GpuBasis(b, 48)
+    pts = [serve.point_coeffs(g, _h_of(B)) for g, B in POINTS[:4]]
+    res = []
+    for warm in (False, True):
+        op = q.GpuOp(basis, serve.slotted_hamiltonian(jd))
+        eng = serve.Engine(op, max_cols=16, nev=3, ncv=15, tol=1e-10)
+        if warm:
+            eng.submit(pts[0], seed=1)
+            eng.run_all()
+        ids = [eng.submit(c, seed=40 + k) for k, c in enumerate(pts)]
+        got = {f.id: f for f in eng.run_all()}
+        res.append([got[i] for i in ids])
+        eng.close()
+    for a, b2 in zip(*res):
+        assert a.nconv == 3 and b2.nconv == 3
+        assert a.steps > 1
+        np.testing.assert_allclose(a.evals, b2.evals, rtol=1e-9, atol=0)
