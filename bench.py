@@ -26,15 +26,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_64spin_pmc.json")
+PMC_SUMMARY_96 = os.path.join(ROOT, "profiles", "apply_96spin_pmc.json")
 CALIB_BYTES = 64 << 20  # the calibration copy reads and writes this many bytes
 
 
-def measured_traffic():
+def measured_traffic(path=PMC_SUMMARY):
     """HBM bytes per apply from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE of the apply kernel, each divided by what the
     same counter reads for a copy of known size), or None."""
     try:
-        with open(PMC_SUMMARY) as f:
+        with open(path) as f:
             return float(json.load(f)["traffic_bytes_per_apply"])
     except (OSError, KeyError, ValueError):
         return None
@@ -108,6 +109,9 @@ def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
     for op in ops:
         op.cache_connectivity(True)
     t_cache = time.perf_counter() - tc0
+    # block configuration chosen by the autotuner (one per operator; the
+    # multi-sector launch uses the first sector's)
+    tuned = [op.autotune(R, reps=2) for op in ops]
     ftlm.lanczos_block(ops, V0s, 3)
     t0 = time.perf_counter()
     al, be, dev_ms = ftlm.lanczos_block(ops, V0s, M, use_graph=True)
@@ -145,6 +149,7 @@ def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
         "batched_device_ms": dev_ms, "batched_device_ms_no_graph": dev_ms_nograph,
         "batched_wall_s": t_batched,
         "connectivity_cache_build_s": t_cache,
+        "autotuned_block_cfg": [{"cfg": c, "ms_per_apply": m} for c, m in tuned],
         "batched_search_solves_per_s": n_solves / t_batched_search,
         "batched_search_device_ms": dev_ms_search,
         "plain_solves_per_s": n_solves / t_plain,
@@ -270,6 +275,143 @@ def bench_serve(states64, n_req=40, max_cols=16, nev=5, ncv=15, tol=1e-10, seed=
     }
 
 
+def cpu_sample_rate(states, H, x, budget_s=15.0, threads=None):
+    """Oracle push apply (the reference's OpenMP algorithm) over a bounded
+    sample of rows of a basis too large to apply whole on the CPU in the
+    bench's time: rows [0, n) with n grown until one run takes about a
+    tenth of the budget, then the median of a few runs.  Returns the rate in
+    rows/s and a description of the sample."""
+    from oracle import pyoracle as orc
+    lib = orc.load()
+    if threads:
+        lib.oracle_set_threads(int(threads))
+    nthr = lib.oracle_num_threads()
+    t0 = time.perf_counter()
+    ob = orc.Basis(states)
+    t_build = time.perf_counter() - t0
+    D = states.shape[0]
+    y = np.zeros(D)
+    n = 1 << 20
+    while True:
+        t0 = time.perf_counter()
+        orc.evaluate_add_rows(ob, H, x, y, 0, n)
+        t = time.perf_counter() - t0
+        if t > budget_s / 10 or n >= D:
+            break
+        n = min(D, n * 4)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.evaluate_add_rows(ob, H, x, y, 0, n)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return n / t, nthr, (f"rows [0, {n}) of the {D}-state basis (its first {100 * n / D:.2f}%), "
+                         f"median of 3 runs, {t:.2f} s each; oracle basis built in {t_build:.0f} s")
+
+
+def bench_96(args):
+    """The HBM-scale single-vector apply: 96-spin diag(2,4,3), Q=0, the basis
+    enumerated and sorted on the device, g_s ~ U[0.2, 1] and |B| = 0.05 in a
+    random direction (seeded), FP64; the kernel picked by the autotuner; 200
+    Lanczos steps for the lowest Ritz values."""
+    import torch
+    import qsi_subspace_ed_amd as q
+    from qsi_subspace_ed_amd import _lib
+    import ctypes as C
+    torch.cuda.set_device(0)
+    jd = q.load_lattice(q.lattice_path("pyro_2,0,0_0,4,0_0,0,3.json"))
+    t0 = time.perf_counter()
+    basis = q.enumerate_basis_device(jd, 0)
+    t_enum = time.perf_counter() - t0
+    D = basis.dim()
+    print(f"[bench] 96-spin Q=0 basis D={D} enumerated on the device in {t_enum:.1f} s",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(args.seed)
+    g = rng.uniform(0.2, 1.0, size=4)
+    b = rng.normal(size=3)
+    B = 0.05 * b / np.linalg.norm(b)
+    H = q.build_hamiltonian_g(jd, [-v for v in g])
+    q.add_local_field(H, jd, B)
+    op = q.GpuOp(basis, H)
+    nnz = C.c_int64(0)
+    _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
+    nnz = nnz.value
+    x = torch.randn(D, dtype=torch.float64, device="cuda", generator=torch.Generator(
+        device="cuda").manual_seed(args.seed))
+    x /= torch.linalg.norm(x)
+    y = torch.zeros_like(x)
+    tuned = op.autotune(0, reps=1, x=x, y=y) if args.variant == "auto" else None
+    if tuned is None:
+        op.set_variant(args.variant)
+    print(f"[bench] autotuned variant {tuned}", file=sys.stderr, flush=True)
+    steps = max(1, min(args.steps, 10))
+    for _ in range(max(1, min(args.warmup, 2))):
+        op.evaluate(x, y)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        op.evaluate(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_s = e0.elapsed_time(e1) / 1e3 / steps
+    S = 16
+    bytes_apply = D * (S + 8 + 8) + 8 * nnz
+    achieved = bytes_apply / kern_s / 1e9
+    lz = None
+    if args.lanczos_steps > 0:
+        M = args.lanczos_steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        al, be = op.lanczos_coeffs(M, seed=args.seed)
+        t_lz = time.perf_counter() - t0
+        T = np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+        ritz = np.linalg.eigvalsh(T)
+        lz = {"steps": int(len(al)), "seconds": t_lz, "ms_per_step": t_lz / len(al) * 1e3,
+              "lowest_ritz_values": [float(v) for v in ritz[:5]]}
+        print(f"[bench] {M} Lanczos steps in {t_lz:.1f} s, E0 ~ {ritz[0]:.12f}",
+              file=sys.stderr, flush=True)
+    out = {
+        "metric": "H.v basis-states/s (96-spin Q=0 single-vector matrix-free apply)",
+        "value": D * steps / wall,
+        "unit": "basis-states/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded g_s ~ U[0.2,1], |B|=0.05 random direction; basis enumerated "
+                "on the device)",
+        "config": {"workload": "pyro diag(2,4,3) 96 spins, Q=0, single-vector H.v",
+                   "n_spins": 96, "basis_dim": int(D), "nnz_offdiag": int(nnz),
+                   "g": [float(v) for v in g], "B": [float(v) for v in B]},
+        "hv_per_s": steps / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(PMC_SUMMARY_96)},
+        "apply_kernel_us": kern_s * 1e6,
+        "apply_variant": tuned[0] if tuned else args.variant,
+        "autotuned": tuned is not None,
+        "enumeration_s": t_enum,
+        "algorithmic_bytes_per_apply": int(bytes_apply),
+        "lanczos": lz,
+    }
+    if not args.no_cpu:
+        states = basis.states()
+        xh = x.cpu().numpy()
+        rate, nthr, sample = cpu_sample_rate(states, H, xh)
+        out["cpu_baseline"] = {"value": rate, "unit": "basis-states/s", "cores": int(nthr),
+                               "kind": "port", "sample": sample}
+        out["speedup_vs_cpu"] = out["value"] / rate
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,11 +430,17 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-solves comparison")
     ap.add_argument("--no-cached", action="store_true",
                     help="skip the cached-connectivity apply figures")
+    ap.add_argument("--workload", default="64spin", choices=["64spin", "96spin"],
+                    help="64spin: the default line; 96spin: the HBM-scale apply")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
+
+    if args.workload == "96spin":
+        bench_96(args)
+        return
 
     import torch
     import qsi_subspace_ed_amd as q
@@ -307,7 +455,11 @@ def main():
     basis = q.GpuBasis(states, 64, bucket_bits=args.bucket_bits,
                        hash_index=args.variant.startswith("hash"))
     op = q.GpuOp(basis, H)
-    op.set_variant(args.variant)
+    tuned = None
+    if args.variant == "auto":
+        tuned = op.autotune(0, reps=3)  # fastest searching kernel on this workload
+    else:
+        op.set_variant(args.variant)
     nnz = C.c_int64(0)
     _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
     nnz = nnz.value
@@ -451,7 +603,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic()},
         "apply_kernel_us": kern_s * 1e6,
-        "apply_variant": args.variant,
+        "apply_variant": tuned[0] if tuned else args.variant,
+        "autotuned": tuned is not None,
         "bucket_bits": basis.bucket_bits(),
         "algorithmic_bytes_per_apply": int(bytes_apply),
         "lanczos": lanczos,

@@ -173,6 +173,19 @@ int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
  * algorithm, which materialises H (operator_matrix.hpp:79-105) instead. */
 int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
 
+/* Autotuner: times the apply kernels on this operator (reps calls each, after
+ * a warm-up, with HIP events) and keeps the fastest.  R = 0 tunes the
+ * single-vector kernel (the QSI_APPLY_* variants that apply: hash variants
+ * when the basis has a hash index, CACHED when the connectivity is cached)
+ * and sets the operator's variant; R in [1, 64] tunes the block apply for R
+ * columns (kernel depth and grid size) used by qsi_op_apply_block,
+ * qsi_lanczos_block and qsi_engine_create.  d_x, d_y: scratch of D (R = 0)
+ * or D x R doubles, or NULL to allocate.  *best (optional) receives the
+ * chosen variant / configuration, *ms its time per apply.  No counterpart in
+ * the reference.  Synchronises. */
+int qsi_op_autotune(qsi_op_t op, int R, int reps, double *d_x, double *d_y,
+                    int *best, double *ms);
+
 /* Connectivity materialiser (the matrix-free equivalent of
  * LazyOpSum::toSparseMatrix, operator_matrix.hpp:79-105): writes the nonzero
  * off-diagonal entries H[row, col] = val in row-major order plus the diagonal
@@ -213,6 +226,15 @@ typedef struct {
 int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t *settings,
                         double *eigval, double *d_eigvec,
                         qsi_lanczos_result_t *result);
+
+/* M steps of the reference's 3-term recurrence (lanczos.cpp:111-146, no
+ * reorthogonalisation) from d_v0 (D doubles, normalised here) or, when d_v0
+ * is NULL, from qsi_vec_randn(seed): h_alpha[j], h_beta[j] for j < M (zero
+ * after a beta that underflows, which ends the recurrence; *n_steps, if
+ * given, receives the steps taken).  The Ritz values are the eigenvalues of
+ * the tridiagonal (qsi_tridiag_eig).  Synchronises. */
+int qsi_lanczos_coeffs(qsi_op_t op, const double *d_v0, uint64_t seed, int M,
+                       double *h_alpha, double *h_beta, int *n_steps);
 
 /* Lowest-nev eigenpairs with a thick-restart Lanczos (full
  * reorthogonalisation, ncv-dimensional search space), the GPU counterpart of
@@ -312,6 +334,21 @@ int qsi_enumerate_basis(int n_spins, int n_tetra, const int32_t *tetra_members,
                         const int32_t *order, int n_spinon_pairs,
                         int n_threads, uint64_t **out_lohi, int64_t *out_dim);
 void qsi_free(void *p);
+
+/* The same enumeration on the device, for bases too large for the host
+ * (96 spins: ~2e9 states): a breadth-first frontier of the search tree is
+ * expanded on the host, every subtree is walked on the device (a counting
+ * pass, then a writing pass), and the states are radix-sorted there.  The
+ * result is a basis handle as from qsi_basis_create (bucket_bits = 0: auto);
+ * *out_dim (optional) receives D.  The states can be read back with
+ * qsi_basis_download.  Synchronises. */
+int qsi_enumerate_basis_device(int n_spins, int n_tetra,
+                               const int32_t *tetra_members,
+                               const int32_t *order, int n_spinon_pairs,
+                               int bucket_bits, qsi_basis_t *out,
+                               int64_t *out_dim);
+/* Copies the basis states to h_lohi (D rows of [lo, hi]); synchronises. */
+int qsi_basis_download(qsi_basis_t b, uint64_t *h_lohi);
 
 #ifdef __cplusplus
 }

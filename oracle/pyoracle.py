@@ -53,6 +53,8 @@ def load():
             "oracle_basis_free": (None, [_P]),
             "oracle_basis_search": (None, [_P, _P, C.c_int64, _P, C.c_int]),
             "oracle_evaluate_add": (C.c_int, [_P, _P, C.c_int, _P, _P]),
+            "oracle_evaluate_add_rows": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int64,
+                                                   C.c_int64]),
             "oracle_sparse_triplets": (C.c_int64, [_P, _P, C.c_int, C.c_double, C.c_int64,
                                                    _P, _P, _P]),
             "oracle_random_unit": (None, [C.c_uint32, _P, C.c_int64]),
@@ -180,6 +182,18 @@ def evaluate_add(basis: Basis, H, x, y):
     x = np.ascontiguousarray(x, dtype=np.float64)
     assert y.dtype == np.float64 and y.flags.c_contiguous
     rc = load().oracle_evaluate_add(basis._h, arr, n, _ptr(x), _ptr(y))
+    if rc != 0:
+        raise RuntimeError("oracle apply: a connected state's high word has no bucket")
+    return y
+
+
+def evaluate_add_rows(basis: Basis, H, x, y, r0, r1):
+    """The pushes of rows [r0, r1) of y += H x (in place on y)."""
+    arr, n = terms_from_opsum(H)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    assert y.dtype == np.float64 and y.flags.c_contiguous
+    rc = load().oracle_evaluate_add_rows(basis._h, arr, n, _ptr(x), _ptr(y),
+                                         C.c_int64(int(r0)), C.c_int64(int(r1)))
     if rc != 0:
         raise RuntimeError("oracle apply: a connected state's high word has no bucket")
     return y

@@ -190,12 +190,14 @@ constexpr double kApplyTol = 1e-15;  // LazyOpSum::APPLY_TOL, operator_matrix.hp
 // LazyOpSum::evaluate_add = evaluate_add_off_diag_omp + evaluate_add_diagonal
 // (components/operator/src/operator_matrix.cpp:28-82,108-112): push form,
 // term-outer loop, static row schedule, atomic scatter.
-int evaluate_add(const Basis& B, const OpSum& H, const double* x, double* y) {
-  const int64_t D = static_cast<int64_t>(B.states.size());
+// The pushes of rows [r0, r1) only (the full apply is r0 = 0, r1 = D): used
+// to time the algorithm on a bounded sample of a very large basis.
+int evaluate_add_rows(const Basis& B, const OpSum& H, const double* x, double* y,
+                      int64_t r0, int64_t r1) {
   int missing_bucket = 0;
   for (const auto& [c, op] : H.off) {
 #pragma omp parallel for schedule(static) reduction(| : missing_bucket)
-    for (int64_t i = 0; i < D; ++i) {
+    for (int64_t i = r0; i < r1; ++i) {
       u128 s = B.states[i];
       int sgn = apply_state(op, s);
       if (sgn != 0 && std::fabs(x[i]) > kApplyTol) {
@@ -211,12 +213,16 @@ int evaluate_add(const Basis& B, const OpSum& H, const double* x, double* y) {
   }
   for (const auto& [c, op] : H.diag) {
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < D; ++i) {
+    for (int64_t i = r0; i < r1; ++i) {
       u128 s = B.states[i];
       y[i] += c * x[i] * static_cast<double>(apply_state(op, s));
     }
   }
   return missing_bucket ? -1 : 0;
+}
+
+int evaluate_add(const Basis& B, const OpSum& H, const double* x, double* y) {
+  return evaluate_add_rows(B, H, x, y, 0, static_cast<int64_t>(B.states.size()));
 }
 
 // ---- small dense symmetric eigensolver (cyclic Jacobi) ----------------------
@@ -586,6 +592,15 @@ int oracle_evaluate_add(const void* b, const Term* terms, int n_terms,
   const Basis& B = *static_cast<const Basis*>(b);
   OpSum H(terms, n_terms);
   return evaluate_add(B, H, x, y);
+}
+
+// evaluate_add restricted to the pushes of rows [r0, r1).
+int oracle_evaluate_add_rows(const void* b, const Term* terms, int n_terms,
+                             const double* x, double* y, int64_t r0, int64_t r1) {
+  const Basis& B = *static_cast<const Basis*>(b);
+  OpSum H(terms, n_terms);
+  const int64_t D = static_cast<int64_t>(B.states.size());
+  return evaluate_add_rows(B, H, x, y, std::max<int64_t>(0, r0), std::min(r1, D));
 }
 
 // toSparseMatrix triplets (operator_matrix.hpp:79-105), in the reference's

@@ -103,9 +103,12 @@ SIGNATURES = {
     "qsi_op_apply_host": (C.c_int, [_P, _P, _P, C.c_int]),
     "qsi_op_apply_block": (C.c_int, [_P, _P, _I64, _P, _I64, C.c_int, _P, C.c_int, _P]),
     "qsi_op_cache_connectivity": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
+    "qsi_op_autotune": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.POINTER(C.c_int), _DP]),
     "qsi_op_connectivity": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P, _P]),
     "qsi_lanczos_eigval0": (C.c_int, [_P, C.POINTER(LanczosSettings), _DP, _P,
                                       C.POINTER(LanczosResult)]),
+    "qsi_lanczos_coeffs": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _DP, _DP,
+                                     C.POINTER(C.c_int)]),
     "qsi_eigsh": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, _P, C.c_uint64, _DP,
                             _P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsi_lanczos_block": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P),
@@ -130,6 +133,9 @@ SIGNATURES = {
     "qsi_enumerate_basis": (C.c_int, [C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
                                       C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(_I64)]),
     "qsi_free": (None, [_P]),
+    "qsi_enumerate_basis_device": (C.c_int, [C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
+                                             C.POINTER(_P), C.POINTER(_I64)]),
+    "qsi_basis_download": (C.c_int, [_P, _P]),
 }
 
 _lib = None

@@ -57,14 +57,14 @@ __device__ __forceinline__ void blk_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Ranks of up to kBlkBatch keys with all bisections interleaved; kNoRank for
+// Ranks of up to B keys with all bisections interleaved; kNoRank for
 // keys absent from the basis (or inactive slots).
-template <int NW>
+template <int NW, int B>
 __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>* key,
                                              const bool* act, uint32_t* rank) {
-  uint32_t lo[kBlkBatch], n[kBlkBatch];
+  uint32_t lo[B], n[B];
 #pragma unroll
-  for (int g = 0; g < kBlkBatch; ++g) {
+  for (int g = 0; g < B; ++g) {
     lo[g] = 0;
     n[g] = 0;
     if (act[g]) {
@@ -79,7 +79,7 @@ __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>
   while (more) {
     more = false;
 #pragma unroll
-    for (int g = 0; g < kBlkBatch; ++g) {
+    for (int g = 0; g < B; ++g) {
       if (n[g] > 1) {
         const uint32_t half = n[g] >> 1;
         const State<NW> m = load_state_cached<NW>(b.states, lo[g] + half);
@@ -90,7 +90,7 @@ __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>
     }
   }
 #pragma unroll
-  for (int g = 0; g < kBlkBatch; ++g) {
+  for (int g = 0; g < B; ++g) {
     rank[g] = kNoRank;
     if (n[g] == 1) {
       const State<NW> f = load_state_cached<NW>(b.states, lo[g]);
@@ -104,7 +104,7 @@ __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>
 //
 // Each wavefront takes a 64-row tile in two phases per round:
 //   A  one lane per row finds the row's connected terms, looks up the
-//      pre-image ranks kBlkBatch at a time and writes (rank, term, sign)
+//      pre-image ranks BATCH at a time and writes (rank, term, sign)
 //      entries to LDS in term order;
 //   B  G lanes per row (one per column) walk the row's entries and read
 //      X[J, 0:R] as one coalesced segment per connected state.
@@ -112,7 +112,7 @@ __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>
 // entries; partial sums of rows cut by a round boundary go through Y.
 // Contributions are added in term order, then the diagonal terms in order, so
 // column c's result depends only on X[:, c] and its coefficients.
-template <int NW>
+template <int NW, int BATCH>
 __global__ __launch_bounds__(kBlkThreads) void k_blk_apply(
     BlkArgs a, const double* __restrict__ X, double* __restrict__ Y,
     double* __restrict__ partials) {
@@ -194,20 +194,20 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_apply(
           for (int q = 0; q < 4; ++q) {
             uint64_t m = conn[q];
             while (m) {
-              State<NW> key[kBlkBatch];
-              bool act[kBlkBatch];
-              int tl[kBlkBatch];
-              uint32_t rk[kBlkBatch];
+              State<NW> key[BATCH];
+              bool act[BATCH];
+              int tl[BATCH];
+              uint32_t rk[BATCH];
 #pragma unroll
-              for (int g = 0; g < kBlkBatch; ++g) {
+              for (int g = 0; g < BATCH; ++g) {
                 act[g] = m != 0;
                 tl[g] = q * 64 + (act[g] ? __builtin_ctzll(m) : 0);
                 m &= m - 1;
                 key[g] = xor_state<NW>(st, S.off[tb + tl[g]].X);
               }
-              lookup_batch<NW>(S.b, key, act, rk);
+              lookup_batch<NW, BATCH>(S.b, key, act, rk);
 #pragma unroll
-              for (int g = 0; g < kBlkBatch; ++g) {
+              for (int g = 0; g < BATCH; ++g) {
                 if (act[g]) {
                   const OffTerm& T = S.off[tb + tl[g]];
                   const bool ng = (T.sgn < 0.0) != static_cast<bool>(parity_notz<NW>(st, T.Z));
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void k_cc_fill(BasisView b, const OffTerm* __r
         m &= m - 1;
         key[g] = xor_state<NW>(st, off[tt[g]].X);
       }
-      lookup_batch<NW>(b, key, act, rk);
+      lookup_batch<NW, kBlkBatch>(b, key, act, rk);
 #pragma unroll
       for (int g = 0; g < kBlkBatch; ++g) {
         if (act[g]) {
@@ -357,12 +357,10 @@ __global__ __launch_bounds__(256) void k_cc_fill(BasisView b, const OffTerm* __r
   }
 }
 
-constexpr int kCcPipe = 8;  // gathers in flight per lane
-
 // The block apply over cached connectivity: same arithmetic, order and
-// results as k_blk_apply.  G lanes per row (one per column), kCcPipe
-// entries' gathers in flight.
-template <int NW>
+// results as k_blk_apply.  G lanes per row (one per column), PIPE entries'
+// gathers in flight.
+template <int NW, int PIPE>
 __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
     BlkArgs a, const double* __restrict__ X, double* __restrict__ Y,
     double* __restrict__ partials) {
@@ -412,21 +410,21 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
       const int64_t gi = a.y_colmajor ? col * a.ldy + row : row * a.ldy + col;
       double acc = a.accumulate ? Y[gi] : 0.0;
       const uint32_t e0 = S.rowptr[i], e1 = S.rowptr[i + 1];
-      for (uint32_t e = e0; e < e1; e += kCcPipe) {
-        uint32_t J[kCcPipe];
-        uint8_t mt[kCcPipe];
-        double xj[kCcPipe];
+      for (uint32_t e = e0; e < e1; e += PIPE) {
+        uint32_t J[PIPE];
+        uint8_t mt[PIPE];
+        double xj[PIPE];
 #pragma unroll
-        for (int u = 0; u < kCcPipe; ++u) {
+        for (int u = 0; u < PIPE; ++u) {
           const bool in = e + u < e1;
           J[u] = in ? S.cj[e + u] : kNoRank;
           mt[u] = in ? S.cm[e + u] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < kCcPipe; ++u)
+        for (int u = 0; u < PIPE; ++u)
           xj[u] = J[u] != kNoRank ? X[(S.row_off + static_cast<int64_t>(J[u])) * a.ldx + col] : 0.0;
 #pragma unroll
-        for (int u = 0; u < kCcPipe; ++u) {
+        for (int u = 0; u < PIPE; ++u) {
           if (J[u] != kNoRank && fabs(xj[u]) > kApplyTol) {
             const int k = mt[u] & 0x7F;
             double c = cc[0];
@@ -570,42 +568,49 @@ struct DevMem {
 
 }  // namespace
 
-int blk_apply_blocks(int nw, int64_t n_tiles, bool cached) {
-  // resident blocks per CU of each kernel: [cached][nw - 1]
-  static int per_cu[2][2] = {{0, 0}, {0, 0}};
+namespace {
+
+using BlkKernel = void (*)(BlkArgs, const double*, double*, double*);
+
+// Block-apply kernel of a configuration: cached kernels by gathers in flight
+// (4, 8, 16), searching kernels by lookups in flight (4, 2).
+BlkKernel blk_kernel(int nw, bool cached, int cfg) {
+  const int k = blk_cfg_kernel(cfg);
+  if (cached) {
+    if (nw == 1) return k == 0 ? k_blk_capply<1, 4> : (k == 2 ? k_blk_capply<1, 16> : k_blk_capply<1, 8>);
+    return k == 0 ? k_blk_capply<2, 4> : (k == 2 ? k_blk_capply<2, 16> : k_blk_capply<2, 8>);
+  }
+  if (nw == 1) return k == 2 ? k_blk_apply<1, 2> : k_blk_apply<1, 4>;
+  return k == 2 ? k_blk_apply<2, 2> : k_blk_apply<2, 4>;
+}
+
+}  // namespace
+
+int blk_apply_blocks(int nw, int64_t n_tiles, bool cached, int cfg) {
+  // resident blocks per CU of each kernel: [cached][nw - 1][kernel]
+  static int per_cu[2][2][3] = {};
   static int n_cu = 0;
-  int& occ = per_cu[cached ? 1 : 0][nw - 1];
+  int& occ = per_cu[cached ? 1 : 0][nw - 1][blk_cfg_kernel(cfg)];
   if (occ == 0) {
     int dev = 0, o = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         n_cu <= 0)
       n_cu = 256;
-    hipError_t e;
-    if (cached)
-      e = nw == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_blk_capply<1>, kBlkThreads, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_blk_capply<2>, kBlkThreads, 0);
-    else
-      e = nw == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_blk_apply<1>, kBlkThreads, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_blk_apply<2>, kBlkThreads, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &o, blk_kernel(nw, cached, cfg), kBlkThreads, 0);
     occ = (e == hipSuccess && o > 0) ? o : 4;
   }
+  int64_t resident = static_cast<int64_t>(occ) * n_cu;
+  if (blk_cfg_half(cfg)) resident = std::max<int64_t>(1, resident / 2);
   const int64_t need = (n_tiles + kBlkWaves - 1) / kBlkWaves;
-  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(need, static_cast<int64_t>(occ) * n_cu)));
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(need, resident)));
 }
 
 int launch_blk_apply(const BlkArgs& a, int nw, int n_blocks, const double* X, double* Y,
                      double* partials, hipStream_t s) {
-  if (blk_cached(a)) {
-    if (nw == 1)
-      hipLaunchKernelGGL(k_blk_capply<1>, dim3(n_blocks), dim3(kBlkThreads), 0, s, a, X, Y, partials);
-    else
-      hipLaunchKernelGGL(k_blk_capply<2>, dim3(n_blocks), dim3(kBlkThreads), 0, s, a, X, Y, partials);
-  } else if (nw == 1) {
-    hipLaunchKernelGGL(k_blk_apply<1>, dim3(n_blocks), dim3(kBlkThreads), 0, s, a, X, Y, partials);
-  } else {
-    hipLaunchKernelGGL(k_blk_apply<2>, dim3(n_blocks), dim3(kBlkThreads), 0, s, a, X, Y, partials);
-  }
+  hipLaunchKernelGGL(blk_kernel(nw, blk_cached(a), a.cfg), dim3(n_blocks), dim3(kBlkThreads), 0,
+                     s, a, X, Y, partials);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }
@@ -725,7 +730,8 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
   }
   args.n_tiles = tiles;
   // persistent grid: enough wavefronts to fill the chip, never more than tiles
-  const int n_blocks = blk_apply_blocks(nw, tiles, blk_cached(args));
+  args.cfg = ops[0]->blk_cfg;
+  const int n_blocks = blk_apply_blocks(nw, tiles, blk_cached(args), args.cfg);
   const int64_t n_waves = static_cast<int64_t>(n_blocks) * kBlkWaves;
   const size_t blk = static_cast<size_t>(rows) * R;
   DevMem mem;
@@ -853,6 +859,126 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
 
 // Eigenvalues and first eigenvector components of one Lanczos tridiagonal
 // (Gauss quadrature nodes and weights for FTLM).
+namespace {
+
+// Average milliseconds of `reps` calls of f() (after one warm-up), timed with
+// events on stream s.
+template <typename F>
+int time_ms(F&& f, int reps, hipStream_t s, double* ms) {
+  hipEvent_t e0, e1;
+  QSI_CHECK_HIP(hipEventCreate(&e0));
+  QSI_CHECK_HIP(hipEventCreate(&e1));
+  int rc = f();
+  if (rc == QSI_OK) {
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == QSI_OK; ++r) rc = f();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double* d_y,
+                               int* best, double* best_ms) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_autotune: null operator");
+  QSI_REQUIRE(R >= 0 && R <= 64, "qsi_op_autotune: R must be in [0, 64]");
+  reps = std::max(1, reps);
+  const int64_t D = op->basis->dim;
+  if (D == 0) return QSI_OK;
+  const size_t n = static_cast<size_t>(D) * std::max(R, 1);
+  DevMem mem;
+  if (!d_x) {
+    QSI_CHECK_HIP(mem.alloc(&d_x, n));
+    // Gaussian input: every term's |x_J| is above the apply tolerance
+    const int rc = launch_randn(d_x, static_cast<int64_t>(n), 12345, nullptr);
+    if (rc != QSI_OK) return rc;
+  }
+  if (!d_y) QSI_CHECK_HIP(mem.alloc(&d_y, n));
+  double* coeffs = nullptr;
+  if (R > 0 && op->n_slots > 0) {
+    QSI_CHECK_HIP(mem.alloc(&coeffs, static_cast<size_t>(op->n_slots) * R));
+    const int rc = launch_randn(coeffs, static_cast<int64_t>(op->n_slots) * R, 7, nullptr);
+    if (rc != QSI_OK) return rc;
+  }
+  QSI_CHECK_HIP(hipDeviceSynchronize());
+  int pick = -1;
+  double pick_ms = 0.0;
+  if (R == 0) {
+    std::vector<int> cand = {QSI_APPLY_LANE, QSI_APPLY_LANE2, QSI_APPLY_LANE4, QSI_APPLY_LANE8,
+                             QSI_APPLY_WINDOW};
+    if (op->basis->d_hash) {
+      cand.push_back(QSI_APPLY_HASH);
+      cand.push_back(QSI_APPLY_HASH4);
+    }
+    if (op->conn_nnz >= 0) cand.push_back(QSI_APPLY_CACHED);
+    const int saved = op->variant;
+    for (int v : cand) {
+      op->variant = v;
+      double ms = 0.0;
+      const int rc = time_ms(
+          [&]() { return launch_apply(op, d_x, d_y, nullptr, 0.0, nullptr, nullptr, nullptr); },
+          reps, nullptr, &ms);
+      if (rc != QSI_OK) {
+        op->variant = saved;
+        return rc;
+      }
+      if (pick < 0 || ms < pick_ms) {
+        pick = v;
+        pick_ms = ms;
+      }
+    }
+    op->variant = pick;
+  } else {
+    BlkArgs a{};
+    a.n_sec = 1;
+    a.R = R;
+    int G = 64;
+    while (G / 2 >= R && G > 1) G >>= 1;
+    a.G = G;
+    a.ldx = a.ldy = R;
+    a.sec[0].b = op->basis->view();
+    a.sec[0].off = op->d_off;
+    a.sec[0].dg = op->d_diag;
+    a.sec[0].cls = op->d_cls;
+    a.sec[0].n_cls = op->n_cls;
+    a.sec[0].coeffs = coeffs;
+    a.sec[0].n_off = op->n_off;
+    a.sec[0].n_diag = op->n_diag;
+    a.sec[0].rowptr = op->d_rowptr;
+    a.sec[0].cj = op->d_cj;
+    a.sec[0].cm = op->d_cm;
+    a.n_tiles = (D + kTileRows - 1) / kTileRows;
+    const bool cached = blk_cached(a);
+    const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
+    for (int k : kernels) {
+      for (int half = 0; half < 2; ++half) {
+        a.cfg = k | (half << 2);
+        const int nb = blk_apply_blocks(op->basis->nw, a.n_tiles, cached, a.cfg);
+        double ms = 0.0;
+        const int rc = time_ms(
+            [&]() { return launch_blk_apply(a, op->basis->nw, nb, d_x, d_y, nullptr, nullptr); },
+            reps, nullptr, &ms);
+        if (rc != QSI_OK) return rc;
+        if (pick < 0 || ms < pick_ms) {
+          pick = a.cfg;
+          pick_ms = ms;
+        }
+      }
+    }
+    op->blk_cfg = pick;
+  }
+  if (best) *best = pick;
+  if (best_ms) *best_ms = pick_ms;
+  return QSI_OK;
+}
+
 extern "C" int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t* nnz) {
   QSI_REQUIRE(op != nullptr, "qsi_op_cache_connectivity: null operator");
   const int rc = op_cache_connectivity(op, enable != 0);

@@ -43,7 +43,15 @@ struct BlkArgs {
   int64_t ldx, ldy; // row strides of the X and Y blocks (>= R)
   int accumulate;   // Y += H X instead of Y = H X (apply kernel)
   int y_colmajor;   // Y stored column-major: Y[col * ldy + row] (apply kernel)
+  int cfg;          // kernel configuration (blk_cfg_*), chosen by the autotuner
 };
+
+// Block-apply configuration: bits 0-1 pick the kernel (cached connectivity:
+// 4 / 8 / 16 gathers in flight per lane; searching: 4 / 4 / 2 lookups in
+// flight), bit 2 halves the persistent grid.
+constexpr int kBlkCfgDefault = 1;
+inline int blk_cfg_kernel(int cfg) { return cfg & 3; }
+inline bool blk_cfg_half(int cfg) { return (cfg >> 2) & 1; }
 
 // True when every sector of `a` has its connectivity cached: the block
 // apply then gathers through the cached ranks instead of searching.
@@ -56,7 +64,7 @@ inline bool blk_cached(const BlkArgs& a) {
 // Blocks of a persistent block-apply grid over n_tiles 64-row tiles: all
 // resident at once (occupancy x compute units), never more than the tiles
 // need.  `cached` picks the kernel (see blk_cached).
-int blk_apply_blocks(int nw, int64_t n_tiles, bool cached);
+int blk_apply_blocks(int nw, int64_t n_tiles, bool cached, int cfg);
 
 // Y (+)= H X over the sectors of `a` (the block-apply kernels; see
 // qsi_block.hip).  partials: n_sec x (n_blocks * kBlkWaves) x R, or null.

@@ -1165,6 +1165,113 @@ int qsi_basis_create(const uint64_t* h_lohi, int64_t dim, int n_spins,
 
 int qsi_basis_has_hash(qsi_basis_t b) { return b && b->d_hash ? 1 : 0; }
 
+}  // extern "C"
+
+namespace qsi {
+namespace {
+
+// flag[0] = 1 when some state has bits at or above n_spins, 2 when the
+// states are not strictly increasing (any offender's code; plain stores).
+template <int NW>
+__global__ void k_check_sorted(const uint64_t* __restrict__ st, int64_t dim, int n_spins,
+                               int* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const State<NW> s = load_state_cached<NW>(st, i);
+  bool bad_range;
+  if constexpr (NW == 1) bad_range = n_spins < 64 && (s.w[0] >> n_spins) != 0;
+  else bad_range = n_spins < 128 && (s.w[1] >> (n_spins - 64)) != 0;
+  bool bad_order = false;
+  if (i > 0) bad_order = !st_lt<NW>(load_state_cached<NW>(st, i - 1), s);
+  if (bad_range || bad_order) flag[0] = bad_range ? 1 : 2;
+}
+
+}  // namespace
+
+// A basis over device states the caller hands over (dim x nw words, freed by
+// the basis): validates them on the device and builds the bucket table.
+int basis_adopt_device(uint64_t* d_states, int64_t dim, int n_spins, int bucket_bits,
+                       qsi_basis_s** out) {
+  *out = nullptr;
+  auto* b = new (std::nothrow) qsi_basis_s();
+  if (!b) {
+    (void)hipFree(d_states);
+    set_error("qsi_basis: host allocation failed");
+    return QSI_E_NOMEM;
+  }
+  b->dim = dim;
+  b->n_spins = n_spins;
+  b->nw = n_spins <= 64 ? 1 : 2;
+  b->kbits = bucket_bits ? bucket_bits : auto_bucket_bits(dim, n_spins);
+  b->shift = n_spins - b->kbits;
+  (void)hipGetDevice(&b->device);
+  b->d_states = d_states;
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(std::string("qsi_basis: ") + what + ": " + hipGetErrorString(e));
+    basis_release(b);
+    return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
+  };
+  int* flag = nullptr;
+  hipError_t e = hipMalloc(&flag, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(flag, 0, sizeof(int));
+  if (e != hipSuccess) return fail(e, "validation buffer");
+  if (dim > 0) {
+    const uint32_t g = static_cast<uint32_t>((dim + 255) / 256);
+    if (b->nw == 1)
+      hipLaunchKernelGGL(k_check_sorted<1>, dim3(g), dim3(256), 0, 0, d_states, dim, n_spins, flag);
+    else
+      hipLaunchKernelGGL(k_check_sorted<2>, dim3(g), dim3(256), 0, 0, d_states, dim, n_spins, flag);
+  }
+  int hflag = 0;
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(flag);
+  if (e != hipSuccess) return fail(e, "validation");
+  if (hflag) {
+    set_error(hflag & 1 ? "qsi_basis: a state has bits at or above n_spins"
+                        : "qsi_basis: states are not strictly increasing");
+    basis_release(b);
+    return QSI_E_INVALID;
+  }
+  const size_t nkeys = size_t{1} << b->kbits;
+  e = hipMalloc(&b->d_table, (nkeys + 1) * sizeof(uint32_t));
+  if (e != hipSuccess) return fail(e, "hipMalloc(table)");
+  const uint32_t grid = static_cast<uint32_t>((nkeys + 1 + 255) / 256);
+  const uint32_t nk = static_cast<uint32_t>(nkeys);
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_build_table<1>, dim3(grid), dim3(256), 0, 0, b->d_states, dim, b->shift,
+                       nk, b->d_table);
+  else
+    hipLaunchKernelGGL(k_build_table<2>, dim3(grid), dim3(256), 0, 0, b->d_states, dim, b->shift,
+                       nk, b->d_table);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(e, "k_build_table");
+  *out = b;
+  return QSI_OK;
+}
+
+}  // namespace qsi
+
+extern "C" {
+
+int qsi_basis_download(qsi_basis_t b, uint64_t* h_lohi) {
+  QSI_REQUIRE(b != nullptr && (h_lohi || b->dim == 0), "qsi_basis_download: null argument");
+  if (b->dim == 0) return QSI_OK;
+  if (b->nw == 2) {
+    QSI_CHECK_HIP(hipMemcpy(h_lohi, b->d_states, static_cast<size_t>(b->dim) * 16,
+                            hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint64_t> lo(static_cast<size_t>(b->dim));
+    QSI_CHECK_HIP(hipMemcpy(lo.data(), b->d_states, lo.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < b->dim; ++i) {
+      h_lohi[2 * i] = lo[i];
+      h_lohi[2 * i + 1] = 0;
+    }
+  }
+  return QSI_OK;
+}
+
 int qsi_basis_destroy(qsi_basis_t b) {
   if (b) basis_release(b);
   return QSI_OK;
@@ -1438,7 +1545,8 @@ int qsi_op_apply_block(qsi_op_t op, const double* d_X, int64_t ldx, double* d_Y,
   a.sec[0].cj = op->d_cj;
   a.sec[0].cm = op->d_cm;
   a.n_tiles = (op->basis->dim + kTileRows - 1) / kTileRows;
-  const int nb = blk_apply_blocks(op->basis->nw, a.n_tiles, blk_cached(a));
+  a.cfg = op->blk_cfg;
+  const int nb = blk_apply_blocks(op->basis->nw, a.n_tiles, blk_cached(a), a.cfg);
   return launch_blk_apply(a, op->basis->nw, nb, d_X, d_Y, nullptr,
                           static_cast<hipStream_t>(stream));
 }

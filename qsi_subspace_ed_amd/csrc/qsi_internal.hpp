@@ -105,12 +105,16 @@ struct qsi_op_s {
   uint32_t* d_cj = nullptr;
   uint8_t* d_cm = nullptr;
   int64_t conn_nnz = -1;  // -1: not cached
+  int blk_cfg = 1;        // block-apply configuration (qsi_op_autotune)
 };
 
 namespace qsi {
 
 void basis_retain(qsi_basis_s* b);
 void basis_release(qsi_basis_s* b);
+// takes ownership of d_states (dim x nw words, device)
+int basis_adopt_device(uint64_t* d_states, int64_t dim, int n_spins, int bucket_bits,
+                       qsi_basis_s** out);
 
 // Launch helpers shared by the solvers (all stream ordered).
 // y = yscale * y_in + H x  (y_in may be null or alias y); if dot_out is

@@ -519,7 +519,8 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   a.sec[0].tile_begin = 0;
   a.sec[0].row_off = 0;
   a.n_tiles = (E->D + kTileRows - 1) / kTileRows;
-  E->n_blocks = blk_apply_blocks(op->basis->nw, a.n_tiles, blk_cached(a));
+  a.cfg = op->blk_cfg;
+  E->n_blocks = blk_apply_blocks(op->basis->nw, a.n_tiles, blk_cached(a), a.cfg);
   const size_t R = static_cast<size_t>(E->R);
   hipError_t e = hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking);
   const int ns = std::max(op->n_slots, 1);

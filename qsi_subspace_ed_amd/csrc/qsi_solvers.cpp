@@ -19,6 +19,7 @@
 #include <thread>
 #include <vector>
 
+#include "qsi_enum.hpp"
 #include "qsi_hostla.hpp"
 #include "qsi_internal.hpp"
 
@@ -236,6 +237,65 @@ int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t* st,
   return QSI_OK;
 }
 
+int qsi_lanczos_coeffs(qsi_op_t op, const double* d_v0, uint64_t seed, int M,
+                       double* h_alpha, double* h_beta, int* n_steps) {
+  QSI_REQUIRE(op && h_alpha && h_beta, "qsi_lanczos_coeffs: null argument");
+  QSI_REQUIRE(M >= 1, "qsi_lanczos_coeffs: M must be >= 1");
+  const int64_t D = op->basis->dim;
+  QSI_REQUIRE(D > 0, "qsi_lanczos_coeffs: empty basis");
+  const size_t bytes = static_cast<size_t>(D) * sizeof(double);
+  DevBuf bv, bu, bpart, bab, bscal;
+  QSI_CHECK_HIP(bv.alloc(bytes));
+  QSI_CHECK_HIP(bu.alloc(bytes));
+  const int64_t n_part = std::max<int64_t>(max_vec_partials(), apply_num_blocks(op));
+  QSI_CHECK_HIP(bpart.alloc(static_cast<size_t>(n_part) * sizeof(double)));
+  QSI_CHECK_HIP(bab.alloc(static_cast<size_t>(3 * M) * sizeof(double)));
+  QSI_CHECK_HIP(bscal.alloc(2 * sizeof(double)));
+  hipStream_t s = nullptr;
+  double* v = bv.d();
+  double* u = bu.d();
+  if (d_v0) {
+    QSI_CHECK_HIP(hipMemcpyAsync(v, d_v0, bytes, hipMemcpyDeviceToDevice, s));
+  } else {
+    QSI_TRY(launch_randn(v, D, seed, s));
+  }
+  double n2 = 0.0;
+  QSI_TRY(launch_dot(v, v, D, bpart.d(), bscal.d(), s));
+  QSI_CHECK_HIP(hipMemcpy(&n2, bscal.d(), sizeof(double), hipMemcpyDeviceToHost));
+  QSI_REQUIRE(n2 > 0.0, "qsi_lanczos_coeffs: zero start vector");
+  QSI_TRY(launch_scal(1.0 / std::sqrt(n2), v, D, s));
+  std::fill(h_alpha, h_alpha + M, 0.0);
+  std::fill(h_beta, h_beta + M, 0.0);
+  const int chunk = 64;
+  int done = 0;
+  std::vector<double> hab;
+  for (int it0 = 0; it0 < M; it0 += chunk) {
+    const int it1 = std::min(M, it0 + chunk);
+    for (int it = it0; it < it1; ++it) {
+      QSI_TRY(lanczos_step(op, v, u, it == 0, bab.d() + 3 * (it - 1) + 1, bab.d() + 3 * it,
+                           bpart.d(), s));
+      std::swap(v, u);
+    }
+    hab.resize(static_cast<size_t>(3 * (it1 - it0)));
+    QSI_CHECK_HIP(hipMemcpy(hab.data(), bab.d() + 3 * it0, hab.size() * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    bool stop = false;
+    for (int it = it0; it < it1; ++it) {
+      h_alpha[it] = hab[3 * (it - it0)];
+      h_beta[it] = hab[3 * (it - it0) + 1];
+      done = it + 1;
+      if (!(h_beta[it] > 1e-300)) {  // invariant subspace: the recurrence ends
+        h_beta[it] = 0.0;
+        stop = true;
+        break;
+      }
+    }
+    if (stop) break;
+  }
+  if (n_steps) *n_steps = done;
+  return QSI_OK;
+}
+
 int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
               const double* d_v0, uint64_t seed, double* h_evals,
               double* d_evecs, int n_evecs, int* nconv_out, int* n_applies) {
@@ -392,59 +452,7 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
 
 namespace {
 
-using u128 = unsigned __int128;
-
-struct EnumLattice {
-  int n = 0;
-  std::vector<u128> tmask;           // tetra bitmask in visiting order
-  std::vector<int> tmin, tmax, tsize;
-  std::vector<int> nb0, nb1;         // first two tetras of each (visited) spin
-  unsigned pairs = 0;
-};
-
-struct Node {
-  u128 state;
-  unsigned spin;
-  unsigned pairs;
-};
-
-// possible_spin_states (pyro_tree.cpp:20-81): bit 0 = down allowed, bit 1 =
-// up allowed, judged on the spin's two tetrahedra with the spins visited so
-// far.
-inline int possible(const EnumLattice& L, const Node& c) {
-  int res = 3;
-  const u128 known = (static_cast<u128>(1) << c.spin) - 1;
-  for (int ud = 0; ud < 2; ++ud) {
-    const u128 st = ud ? (c.state | (static_cast<u128>(1) << c.spin)) : c.state;
-    for (int k = 0; k < 2; ++k) {
-      const int t = k == 0 ? L.nb0[c.spin] : L.nb1[c.spin];
-      const u128 tm = L.tmask[t];
-      const int q = __builtin_popcountll(static_cast<uint64_t>(st & tm)) +
-                    __builtin_popcountll(static_cast<uint64_t>((st & tm) >> 64));
-      const u128 km = tm & known;
-      const int nk = __builtin_popcountll(static_cast<uint64_t>(km)) +
-                     __builtin_popcountll(static_cast<uint64_t>(km >> 64)) + 1;
-      const int nu = L.tsize[t] - nk;
-      if (q + nu < L.tmin[t] || q > L.tmax[t]) {
-        res &= ~(1 << ud);
-        break;
-      }
-    }
-  }
-  return res;
-}
-
-// fork_state_impl (pyro_tree.cpp:168-189)
-template <typename Push>
-inline void fork(const EnumLattice& L, const Node& c, Push&& push) {
-  const int poss = possible(L, c);
-  const bool may_pair = c.pairs < L.pairs;
-  const u128 up = static_cast<u128>(1) << c.spin;
-  if (poss & 1) push(Node{c.state, c.spin + 1, c.pairs});
-  else if (may_pair) push(Node{c.state, c.spin + 1, c.pairs + 1});
-  if (poss & 2) push(Node{c.state | up, c.spin + 1, c.pairs});
-  else if (may_pair) push(Node{c.state | up, c.spin + 1, c.pairs + 1});
-}
+using namespace qsi::enumr;
 
 void dfs(const EnumLattice& L, Node root, std::vector<u128>& out) {
   std::vector<Node> stack;
@@ -473,59 +481,14 @@ int qsi_enumerate_basis(int n_spins, int n_tetra, const int32_t* tetra_members,
   QSI_REQUIRE(n_spins >= 1 && n_spins <= 128, "qsi_enumerate_basis: n_spins must be in [1,128]");
   QSI_REQUIRE(n_tetra >= 1 && tetra_members && order, "qsi_enumerate_basis: null lattice");
   QSI_REQUIRE(n_spinon_pairs >= 0, "qsi_enumerate_basis: n_spinon_pairs < 0");
-  // iperm: original spin id -> visiting position (lattice::apply_permutation)
-  std::vector<int> iperm(n_spins, -1);
-  for (int k = 0; k < n_spins; ++k) {
-    const int sid = order[k];
-    QSI_REQUIRE(sid >= 0 && sid < n_spins && iperm[sid] < 0,
-                "qsi_enumerate_basis: order is not a permutation");
-    iperm[sid] = k;
-  }
   EnumLattice L;
-  L.n = n_spins;
-  L.pairs = static_cast<unsigned>(n_spinon_pairs);
-  L.tmask.resize(n_tetra);
-  L.tmin.resize(n_tetra);
-  L.tmax.resize(n_tetra);
-  L.tsize.resize(n_tetra);
-  std::vector<std::vector<int>> nbs(n_spins);
-  for (int t = 0; t < n_tetra; ++t) {
-    u128 m = 0;
-    for (int q = 0; q < 4; ++q) {
-      const int sid = tetra_members[4 * t + q];
-      QSI_REQUIRE(sid >= 0 && sid < n_spins, "qsi_enumerate_basis: tetra member out of range");
-      m |= static_cast<u128>(1) << iperm[sid];
-      nbs[iperm[sid]].push_back(t);
-    }
-    L.tmask[t] = m;
-    L.tsize[t] = 4;
-    L.tmin[t] = 4 / 2;        // tetra default rules (tetra_graph_io.hpp:45-46)
-    L.tmax[t] = (4 + 1) / 2;
-  }
-  L.nb0.resize(n_spins);
-  L.nb1.resize(n_spins);
-  for (int i = 0; i < n_spins; ++i) {
-    QSI_REQUIRE(nbs[i].size() >= 2, "qsi_enumerate_basis: every spin needs two tetrahedra");
-    L.nb0[i] = nbs[i][0];
-    L.nb1[i] = nbs[i][1];
+  {
+    const int rc = build_lattice(n_spins, n_tetra, tetra_members, order, n_spinon_pairs, L);
+    if (rc != QSI_OK) return rc;
   }
   // breadth-first frontier, then independent depth-first subtrees
-  std::vector<Node> front{Node{0, 0, 0}};
-  const size_t target = static_cast<size_t>(std::max(1, n_threads)) * 64;
-  while (front.size() < target) {
-    std::vector<Node> next;
-    bool any = false;
-    for (const Node& c : front) {
-      if (static_cast<int>(c.spin) == L.n) {
-        next.push_back(c);
-        continue;
-      }
-      any = true;
-      fork(L, c, [&](const Node& x) { next.push_back(x); });
-    }
-    front.swap(next);
-    if (!any) break;
-  }
+  std::vector<Node> front =
+      frontier(L, static_cast<size_t>(std::max(1, n_threads)) * 64);
   const int nt = std::max(1, n_threads);
   std::vector<std::vector<u128>> parts(nt);
   std::atomic<size_t> next_node{0};

@@ -63,6 +63,15 @@ class GpuBasis:
     """Sorted Z-basis resident in GPU memory with its top-bit bucket table
     (replaces ZBasisInterp, components/operator/include/operator.hpp:197-204)."""
 
+    @classmethod
+    def _adopt(cls, handle, n_spins):
+        """Wrap a basis handle built on the device (states fetched lazily)."""
+        self = cls.__new__(cls)
+        self._h = handle
+        self.n_spins = int(n_spins)
+        self._states_host = None
+        return self
+
     def __init__(self, states, n_spins, bucket_bits=0, hash_index=False):
         lib = _lib.load()
         lohi = as_lohi(states)
@@ -86,6 +95,12 @@ class GpuBasis:
         return int(_lib.load().qsi_basis_bucket_bits(self._h))
 
     def states(self):
+        """(D, 2) uint64 [lo, hi] host copy of the states."""
+        if self._states_host is None:
+            out = np.empty((self.dim(), 2), dtype=np.uint64)
+            check(_lib.load().qsi_basis_download(self._h, out.ctypes.data_as(C.c_void_p)),
+                  "qsi_basis_download")
+            self._states_host = out
         return self._states_host
 
     def search(self, queries_lohi):
@@ -230,6 +245,23 @@ class GpuOp:
                                              R, cp, int(accumulate), _stream()),
               "qsi_op_apply_block")
 
+    def autotune(self, R=0, reps=3, x=None, y=None):
+        """Time the apply kernels and keep the fastest (qsi_op_autotune):
+        R = 0 the single-vector variant, R >= 1 the block apply for R
+        columns.  x, y: optional CUDA float64 scratch (D or D x R).  Returns
+        (choice, ms per apply); the choice is a variant name for R = 0 and a
+        block configuration number otherwise."""
+        best = C.c_int(-1)
+        ms = C.c_double(0.0)
+        check(_lib.load().qsi_op_autotune(self._h, int(R), int(reps),
+                                          _ptr(x) if x is not None else None,
+                                          _ptr(y) if y is not None else None,
+                                          C.byref(best), C.byref(ms)), "qsi_op_autotune")
+        if R == 0:
+            names = {v: k for k, v in _lib.VARIANTS.items() if k != "auto"}
+            return names.get(best.value, best.value), ms.value
+        return best.value, ms.value
+
     def cache_connectivity(self, enable=True):
         """Keep the connected ranks on the device so block applies gather
         instead of searching (qsi_op_cache_connectivity).  Returns the cached
@@ -276,6 +308,23 @@ class GpuOp:
                                      int(res.n_iterations), float(res.eigval_error),
                                      float(res.eigvec_error))
 
+    def lanczos_coeffs(self, M, v0=None, seed=0):
+        """(alphas, betas) of M steps of the 3-term recurrence from v0 (CUDA
+        float64, normalised here) or from randn(D, seed)."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        if v0 is not None:
+            _require_cuda_f64(v0, self._n, "v0")
+        a = np.zeros(int(M))
+        b = np.zeros(int(M))
+        n = C.c_int(0)
+        check(_lib.load().qsi_lanczos_coeffs(self._h, _ptr(v0) if v0 is not None else None,
+                                             int(seed), int(M),
+                                             a.ctypes.data_as(C.POINTER(C.c_double)),
+                                             b.ctypes.data_as(C.POINTER(C.c_double)),
+                                             C.byref(n)), "qsi_lanczos_coeffs")
+        return a[: n.value], b[: n.value]
+
     def eigsh(self, nev=5, ncv=15, tol=1e-10, max_restarts=1000, v0=None, seed=0,
               n_evecs=0, raise_on_noconv=True):
         """Lowest nev eigenpairs (SmallestAlge) by thick-restart Lanczos.
@@ -321,9 +370,7 @@ def randn(n, seed):
     return v
 
 
-def enumerate_basis(jdata, n_spinon_pairs=0, order="greedy", n_threads=None):
-    """Sorted ice-rule basis of a lattice as a (D, 2) uint64 [lo, hi] array
-    (gen_spinon_basis, components/build_basis/gen_spinon_basis.cpp:16-156)."""
+def _lattice_arrays(jdata, order):
     from .lattice import greedy_spin_ordering
     W = len(jdata["atoms"])
     tet = np.ascontiguousarray([t["member_spin_idx"] for t in jdata["tetrahedra"]],
@@ -334,7 +381,27 @@ def enumerate_basis(jdata, n_spinon_pairs=0, order="greedy", n_threads=None):
         perm = list(range(W))
     else:
         perm = list(order)
-    perm = np.ascontiguousarray(perm, dtype=np.int32)
+    return W, tet, np.ascontiguousarray(perm, dtype=np.int32)
+
+
+def enumerate_basis_device(jdata, n_spinon_pairs=0, order="greedy", bucket_bits=0):
+    """The same basis as enumerate_basis, built and sorted on the device and
+    kept there (qsi_enumerate_basis_device): a GpuBasis whose states are never
+    copied through the host unless asked for."""
+    W, tet, perm = _lattice_arrays(jdata, order)
+    h = C.c_void_p()
+    dim = C.c_int64(0)
+    check(_lib.load().qsi_enumerate_basis_device(
+        W, tet.shape[0], tet.ctypes.data_as(C.c_void_p), perm.ctypes.data_as(C.c_void_p),
+        int(n_spinon_pairs), int(bucket_bits), C.byref(h), C.byref(dim)),
+        "qsi_enumerate_basis_device")
+    return GpuBasis._adopt(h, W)
+
+
+def enumerate_basis(jdata, n_spinon_pairs=0, order="greedy", n_threads=None):
+    """Sorted ice-rule basis of a lattice as a (D, 2) uint64 [lo, hi] array
+    (gen_spinon_basis, components/build_basis/gen_spinon_basis.cpp:16-156)."""
+    W, tet, perm = _lattice_arrays(jdata, order)
     if n_threads is None:
         n_threads = min(16, os.cpu_count() or 1)
     out = C.POINTER(C.c_uint64)()

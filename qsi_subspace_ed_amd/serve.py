@@ -77,10 +77,13 @@ class Engine:
     """One qsi_engine_t on one GpuOp (see include/qsi_gpu.h)."""
 
     def __init__(self, op: GpuOp, max_cols=16, nev=1, ncv=20, tol=1e-10, max_restarts=500,
-                 cache_connectivity=True):
+                 cache_connectivity=True, autotune=True):
         lib = _lib.load()
         if cache_connectivity:
             op.cache_connectivity(True)
+        if autotune:
+            # block-apply configuration for this operator at max_cols columns
+            self.tuned = op.autotune(max_cols, reps=2)
         h = C.c_void_p()
         check(lib.qsi_engine_create(op.handle, max_cols, nev, ncv, tol, max_restarts,
                                     C.byref(h)), "qsi_engine_create")

@@ -286,3 +286,36 @@ def test_cached_connectivity_block_apply_identical(R, lattices, cuda):
     op.apply_block(X, Y1, accumulate=False)
     assert torch.equal(Y0, Y1)
     assert op.cache_connectivity(False) == -1
+
+
+def test_autotune_keeps_results(lattices, cuda):
+    """The autotuner picks a kernel by timing; every pick gives the same bits."""
+    torch = cuda
+    W = 48
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    Hs = q.lattice.add_ring_terms(q.SymbolicOpSum(), jd, [0.0] * 4, slots=True)
+    q.add_local_field(Hs, jd, [0.0, 0.0, 0.0], slots=True)
+    op = q.GpuOp(q.GpuBasis(b, W), Hs, n_slots=8)
+    x = torch.from_numpy(_rand_x(D, 3)).cuda()
+    y_ref = torch.zeros_like(x)
+    op.set_variant("lane8")
+    op.evaluate(x, y_ref)
+    name, ms = op.autotune(0, reps=2)
+    assert name in q._lib.VARIANTS and ms > 0
+    y = torch.zeros_like(x)
+    op.evaluate(x, y)
+    assert torch.equal(y, y_ref)
+    R = 8
+    X = torch.from_numpy(np.random.default_rng(4).standard_normal((D, R))).cuda()
+    co = torch.from_numpy(np.random.default_rng(5).standard_normal((8, R))).cuda()
+    Y0 = torch.zeros_like(X)
+    op.apply_block(X, Y0, coeffs=co)
+    for cache in (False, True):
+        op.cache_connectivity(cache)
+        cfg, ms = op.autotune(R, reps=2)
+        assert 0 <= cfg < 8 and ms > 0
+        Y1 = torch.zeros_like(X)
+        op.apply_block(X, Y1, coeffs=co)
+        assert torch.equal(Y0, Y1)

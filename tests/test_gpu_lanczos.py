@@ -158,3 +158,29 @@ def test_eigsh_48_vs_sparse(lattices, cuda):
     M = sp.csr_matrix((v, (r, c)), shape=(b.shape[0],) * 2)
     w = np.sort(sla.eigsh(M, k=12, which="SA", tol=1e-13)[0])
     _check_against_dense(evals, w, 5)
+
+
+def test_lanczos_coeffs_match_oracle(lattices, cuda):
+    """qsi_lanczos_coeffs: the 3-term recurrence from a given start vector
+    against the oracle's (first 25 steps, 1e-10 relative; later steps drift
+    as both lose orthogonality), and from a seed against randn(seed)."""
+    torch = cuda
+    from qsi_subspace_ed_amd.device import randn
+    jd = lattices[32]
+    b = load_basis(32)
+    H = q.build_hamiltonian_g(jd, [-0.9, -0.35, -0.6, -0.45])
+    q.add_local_field(H, jd, [0.05, -0.02, 0.04])
+    op = q.GpuOp(q.GpuBasis(b, 32), H)
+    v0 = randn(b.shape[0], 11)
+    a, be = op.lanczos_coeffs(60, v0=v0)
+    oa, ob = orc.lanczos_coeffs(orc.Basis(b), H, v0.cpu().numpy(), 60)
+    k = 25
+    sc = np.max(np.abs(oa[:k]))
+    assert np.max(np.abs(a[:k] - oa[:k])) <= 1e-10 * sc
+    assert np.max(np.abs(be[:k] - ob[:k])) <= 1e-10 * sc
+    a2, b2 = op.lanczos_coeffs(60, seed=11)
+    assert np.array_equal(a, a2) and np.array_equal(be, b2)
+    # the lowest Ritz value of 60 steps is the ground state (dense check)
+    w = np.linalg.eigvalsh(_dense(b, H))[0]
+    T = np.diag(a) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+    assert abs(np.linalg.eigvalsh(T)[0] - w) <= 1e-9 * abs(w)

@@ -162,3 +162,20 @@ def test_golden_e0_sparse(lattices):
     vec = gold["eigenvector"]
     res = M @ vec - e0 * vec
     assert np.linalg.norm(res) < 1e-6 * abs(e0)
+
+
+def test_row_range_apply_sums_to_full(lattices):
+    """Pushes of row ranges add up to the full apply (the sampled CPU
+    baseline of very large bases uses a range)."""
+    from conftest import load_basis
+    b = load_basis(32)
+    D = b.shape[0]
+    H = q.build_hamiltonian_g(lattices[32], [-0.9, -0.35, -0.6, -0.45])
+    q.add_local_field(H, lattices[32], [0.05, -0.02, 0.04])
+    ob = orc.Basis(b)
+    x = np.random.default_rng(1).standard_normal(D)
+    full = orc.evaluate_add(ob, H, x, np.zeros(D))
+    parts = np.zeros(D)
+    for r0 in range(0, D, 1000):
+        orc.evaluate_add_rows(ob, H, x, parts, r0, r0 + 1000)
+    np.testing.assert_allclose(parts, full, rtol=1e-13, atol=1e-13)
