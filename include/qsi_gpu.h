@@ -163,7 +163,7 @@ int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
 
 /* Keep (enable = 1) or drop (0) the operator's connectivity on the device:
  * for every row, in term order, the rank of each connected state and the
- * term's coefficient class and sign (5 bytes per nonzero plus 4 per row).
+ * term's coefficient class and sign (5 bytes per nonzero plus 8 per row).
  * Block applies (qsi_op_apply_block, qsi_lanczos_block, qsi_engine_*) then
  * gather through it instead of testing and searching, with identical
  * results; worth it when the same operator is applied many times.  *nnz

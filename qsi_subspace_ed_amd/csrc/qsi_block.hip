@@ -306,11 +306,11 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_apply(
 // counts[i] = number of connected off-diagonal terms of row i.
 template <int NW>
 __global__ __launch_bounds__(256) void k_cc_count(BasisView b, const OffTerm* __restrict__ off,
-                                                  int n_off, uint32_t* __restrict__ counts) {
+                                                  int n_off, uint64_t* __restrict__ counts) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= b.dim) return;
   const State<NW> st = load_state_cached<NW>(b.states, i);
-  uint32_t n = 0;
+  uint64_t n = 0;
   for (int t = 0; t < n_off; ++t) n += term_ok<NW>(off[t], st) ? 1u : 0u;
   counts[i] = n;
 }
@@ -318,13 +318,13 @@ __global__ __launch_bounds__(256) void k_cc_count(BasisView b, const OffTerm* __
 // Entries of row i at rowptr[i]...: ranks looked up kBlkBatch at a time.
 template <int NW>
 __global__ __launch_bounds__(256) void k_cc_fill(BasisView b, const OffTerm* __restrict__ off,
-                                                 int n_off, const uint32_t* __restrict__ rowptr,
+                                                 int n_off, const uint64_t* __restrict__ rowptr,
                                                  uint32_t* __restrict__ cj,
                                                  uint8_t* __restrict__ cm) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= b.dim) return;
   const State<NW> st = load_state_cached<NW>(b.states, i);
-  uint32_t pos = rowptr[i];
+  uint64_t pos = rowptr[i];
   for (int t0 = 0; t0 < n_off; t0 += 64) {
     const int tn = min(64, n_off - t0);
     uint64_t m = 0;
@@ -409,8 +409,8 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
       const int64_t row = S.row_off + i;
       const int64_t gi = a.y_colmajor ? col * a.ldy + row : row * a.ldy + col;
       double acc = a.accumulate ? Y[gi] : 0.0;
-      const uint32_t e0 = S.rowptr[i], e1 = S.rowptr[i + 1];
-      for (uint32_t e = e0; e < e1; e += PIPE) {
+      const uint64_t e0 = S.rowptr[i], e1 = S.rowptr[i + 1];
+      for (uint64_t e = e0; e < e1; e += PIPE) {
         uint32_t J[PIPE];
         uint8_t mt[PIPE];
         double xj[PIPE];
@@ -620,7 +620,8 @@ int op_cache_connectivity(qsi_op_s* op, bool enable) {
     if (op->d_rowptr) (void)hipFree(op->d_rowptr);
     if (op->d_cj) (void)hipFree(op->d_cj);
     if (op->d_cm) (void)hipFree(op->d_cm);
-    op->d_rowptr = op->d_cj = nullptr;
+    op->d_rowptr = nullptr;
+    op->d_cj = nullptr;
     op->d_cm = nullptr;
     op->conn_nnz = -1;
   };
@@ -638,8 +639,8 @@ int op_cache_connectivity(qsi_op_s* op, bool enable) {
   const uint32_t grid = static_cast<uint32_t>((D + 255) / 256);
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
-  hipError_t e = hipMalloc(&op->d_rowptr, static_cast<size_t>(D + 1) * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMemset(op->d_rowptr, 0, static_cast<size_t>(D + 1) * sizeof(uint32_t));
+  hipError_t e = hipMalloc(&op->d_rowptr, static_cast<size_t>(D + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMemset(op->d_rowptr, 0, static_cast<size_t>(D + 1) * sizeof(uint64_t));
   if (e == hipSuccess && D > 0) {
     if (op->basis->nw == 1)
       hipLaunchKernelGGL(k_cc_count<1>, dim3(grid), dim3(256), 0, 0, view, op->d_off, op->n_off,
@@ -655,8 +656,8 @@ int op_cache_connectivity(qsi_op_s* op, bool enable) {
   if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
   if (e == hipSuccess)
     e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, op->d_rowptr, op->d_rowptr, D + 1);
-  uint32_t nnz = 0;
-  if (e == hipSuccess) e = hipMemcpy(&nnz, op->d_rowptr + D, sizeof(uint32_t), hipMemcpyDeviceToHost);
+  uint64_t nnz = 0;
+  if (e == hipSuccess) e = hipMemcpy(&nnz, op->d_rowptr + D, sizeof(uint64_t), hipMemcpyDeviceToHost);
   if (tmp) (void)hipFree(tmp);
   if (e == hipSuccess) e = hipMalloc(&op->d_cj, std::max<size_t>(nnz, 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&op->d_cm, std::max<size_t>(nnz, 1));
@@ -917,24 +918,36 @@ extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double
       cand.push_back(QSI_APPLY_HASH);
       cand.push_back(QSI_APPLY_HASH4);
     }
-    if (op->conn_nnz >= 0) cand.push_back(QSI_APPLY_CACHED);
-    const int saved = op->variant;
-    for (int v : cand) {
-      op->variant = v;
+    // the cached kernel once per gather depth (4, 8, 16, 32)
+    std::vector<int> pipes(cand.size(), 8);
+    if (op->conn_nnz >= 0) {
+      for (int p : {4, 8, 16, 32}) {
+        cand.push_back(QSI_APPLY_CACHED);
+        pipes.push_back(p);
+      }
+    }
+    const int saved = op->variant, saved_pipe = op->cached_pipe;
+    int pick_pipe = saved_pipe;
+    for (size_t k = 0; k < cand.size(); ++k) {
+      op->variant = cand[k];
+      op->cached_pipe = pipes[k];
       double ms = 0.0;
       const int rc = time_ms(
           [&]() { return launch_apply(op, d_x, d_y, nullptr, 0.0, nullptr, nullptr, nullptr); },
           reps, nullptr, &ms);
       if (rc != QSI_OK) {
         op->variant = saved;
+        op->cached_pipe = saved_pipe;
         return rc;
       }
       if (pick < 0 || ms < pick_ms) {
-        pick = v;
+        pick = cand[k];
+        pick_pipe = pipes[k];
         pick_ms = ms;
       }
     }
     op->variant = pick;
+    op->cached_pipe = pick_pipe;
   } else {
     BlkArgs a{};
     a.n_sec = 1;

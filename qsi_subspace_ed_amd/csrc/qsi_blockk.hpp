@@ -29,7 +29,7 @@ struct BlkSector {
   int64_t tile_begin;  // first global tile of this sector
   int64_t row_off;     // first row of this sector in the concatenated blocks
   // cached connectivity of the operator, or null (see qsi_op_s)
-  const uint32_t* rowptr;
+  const uint64_t* rowptr;
   const uint32_t* cj;
   const uint8_t* cm;
 };

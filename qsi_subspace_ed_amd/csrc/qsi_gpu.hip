@@ -219,12 +219,10 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
 // connectivity (qsi_op_cache_connectivity), so an apply is a gather with no
 // flip tests or searches.  Same terms, order and arithmetic as the lane
 // variant, so y is bit-identical.
-constexpr int kCachedPipe = 8;
-
-template <int NW, bool DOT>
+template <int NW, bool DOT, int PIPE>
 __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
     BasisView b, const CoefClass* __restrict__ cls, int n_cls,
-    const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ cj,
+    const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ cj,
     const uint8_t* __restrict__ cm, const DiagTerm* __restrict__ dg, int n_diag,
     const double* __restrict__ x, double* y, const double* y_in, double yscale,
     const double* __restrict__ yscale_dev, double* __restrict__ partials) {
@@ -238,21 +236,21 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
   if (i < b.dim) {
     const double xi = x[i];
     double acc = y_in ? yscale * y_in[i] : 0.0;
-    const uint32_t e0 = rowptr[i], e1 = rowptr[i + 1];
-    for (uint32_t e = e0; e < e1; e += kCachedPipe) {
-      uint32_t J[kCachedPipe];
-      uint8_t mt[kCachedPipe];
-      double xj[kCachedPipe];
+    const uint64_t e0 = rowptr[i], e1 = rowptr[i + 1];
+    for (uint64_t e = e0; e < e1; e += PIPE) {
+      uint32_t J[PIPE];
+      uint8_t mt[PIPE];
+      double xj[PIPE];
 #pragma unroll
-      for (int u = 0; u < kCachedPipe; ++u) {
+      for (int u = 0; u < PIPE; ++u) {
         const bool in = e + u < e1;
         J[u] = in ? cj[e + u] : 0xFFFFFFFFu;
         mt[u] = in ? cm[e + u] : 0;
       }
 #pragma unroll
-      for (int u = 0; u < kCachedPipe; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] : 0.0;
+      for (int u = 0; u < PIPE; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] : 0.0;
 #pragma unroll
-      for (int u = 0; u < kCachedPipe; ++u) {
+      for (int u = 0; u < PIPE; ++u) {
         if (J[u] != 0xFFFFFFFFu && fabs(xj[u]) > kApplyTol) {
           const int k = mt[u] & 0x7F;
           double c = cc[0];
@@ -806,9 +804,23 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
     variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE8;
   switch (variant) {
     case QSI_APPLY_CACHED:
-      hipLaunchKernelGGL((k_apply_cached<NW, DOT>), grid, block, 0, s, v, op->d_cls, op->n_cls,
-                         op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+      // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)
+      if (op->cached_pipe == 32)
+        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 32>), grid, block, 0, s, v, op->d_cls,
+                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
+                           x, y, y_in, yscale, yscale_dev, partials);
+      else if (op->cached_pipe == 4)
+        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 4>), grid, block, 0, s, v, op->d_cls,
+                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
+                           x, y, y_in, yscale, yscale_dev, partials);
+      else if (op->cached_pipe == 16)
+        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 16>), grid, block, 0, s, v, op->d_cls,
+                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
+                           x, y, y_in, yscale, yscale_dev, partials);
+      else
+        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 8>), grid, block, 0, s, v, op->d_cls,
+                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
+                           x, y, y_in, yscale, yscale_dev, partials);
       break;
     case QSI_APPLY_LANE:
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,

@@ -98,14 +98,16 @@ struct qsi_op_s {
   int n_cls = 0;  // coefficient classes (0: more than kMaxCoefClasses)
   qsi::CoefClass* d_cls = nullptr;
   std::vector<qsi::CoefClass> h_cls;
-  // cached connectivity (qsi_op_cache_connectivity): CSR over the rows, the
+  // cached connectivity (qsi_op_cache_connectivity): CSR over the rows (64-bit
+  // offsets: a 96-spin operator has 3.3e10 nonzeros), the
   // rank of each connected state (0xFFFFFFFF if absent) and its coefficient
   // class | negative-sign bit << 7, off-diagonal terms in term order
-  uint32_t* d_rowptr = nullptr;
+  uint64_t* d_rowptr = nullptr;
   uint32_t* d_cj = nullptr;
   uint8_t* d_cm = nullptr;
   int64_t conn_nnz = -1;  // -1: not cached
   int blk_cfg = 1;        // block-apply configuration (qsi_op_autotune)
+  int cached_pipe = 8;    // gathers in flight per lane of the cached apply (4-32)
 };
 
 namespace qsi {

@@ -194,10 +194,10 @@ class Space:
         """Device bytes when resident: basis words + bucket table (at most
         2^28 + 1 entries, about 16 per state) + cached connectivity (5 bytes
         per nonzero, at most n_spins / 4 nonzeros per row on pyrochlore ring
-        terms, + 4 per row) + terms + engine buffers."""
+        terms, + 8 per row) + terms + engine buffers."""
         words = 1 if self.n_spins <= 64 else 2
         basis = self.dim * 8 * words + 4 * min(16 * self.dim + 1, (1 << 28) + 1)
-        conn = self.dim * (4 + 5 * max(1, self.n_spins // 4))
+        conn = self.dim * (8 + 5 * max(1, self.n_spins // 4))
         terms = 1 << 20  # term tables are a few KB; a MiB covers them
         return basis + conn + terms + Engine.footprint(self.dim, max_cols, ncv)
 
