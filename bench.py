@@ -118,9 +118,8 @@ def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
     for op in ops:
         op.cache_connectivity(True)
     t_cache = time.perf_counter() - tc0
-    # block configuration chosen by the autotuner (one per operator; the
-    # multi-sector launch uses the first sector's)
-    tuned = [op.autotune(R, reps=2) for op in ops]
+    # block configuration chosen by the autotuner on the multi-sector launch
+    tuned = [ftlm.autotune(ops, R, reps=3)]
     ftlm.lanczos_block(ops, V0s, 3)
     t0 = time.perf_counter()
     al, be, dev_ms = ftlm.lanczos_block(ops, V0s, M, use_graph=True)
@@ -403,6 +402,7 @@ def bench_96(args):
         kc = ec0.elapsed_time(ec1) / 1e3 / steps
         cached = {"states_per_s": D * steps / wall_c, "apply_kernel_us": kc * 1e6,
                   "build_s": t_build, "device_bytes": int(need), "autotuned": tuned_c[0],
+                  "cached_pipe": op.cached_pipe(),
                   "algorithmic_GBps": bytes_apply / kc / 1e9,
                   "kernel_bytes_per_apply": cached_kernel_bytes(D, nnz, S),
                   "kernel_GBps": cached_kernel_bytes(D, nnz, S) / kc / 1e9,
@@ -605,6 +605,7 @@ def main():
         kc = ec0.elapsed_time(ec1) / 1e3 / args.steps
         cached = {"states_per_s": D * args.steps / wall_c, "apply_kernel_us": kc * 1e6,
                   "build_s": t_build, "nnz": int(cnnz), "autotuned": tuned_c[0],
+                  "cached_pipe": op.cached_pipe(),
                   "device_bytes": int(cnnz) * 5 + 8 * (D + 1),
                   "algorithmic_GBps": bytes_apply / kc / 1e9,
                   "kernel_bytes_per_apply": cached_kernel_bytes(D, cnnz, S),

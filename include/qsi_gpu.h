@@ -93,6 +93,14 @@ int qsi_basis_destroy(qsi_basis_t b);
 int64_t qsi_basis_dim(qsi_basis_t b);
 int qsi_basis_n_spins(qsi_basis_t b);
 int qsi_basis_bucket_bits(qsi_basis_t b);
+/* The bucket key: the concatenation, highest first, of *n_fields (<= 4) bit
+ * fields [lo[f], lo[f] + width[f]) of a state, K bits in all, covering the
+ * highest bit positions at which neighbouring basis states first differ
+ * (positions fixed by the bits above them are left out between fields); the
+ * top K bits when there are none.  lo and width hold 4 entries.  No
+ * counterpart in the reference (its bucket key is the high 64-bit word,
+ * operator.cpp:73-86). */
+int qsi_basis_key_fields(qsi_basis_t b, int *n_fields, int *lo, int *width);
 /* Device-side state -> rank lookup of n states (test hook and building block;
  * replaces ZBasisInterp::search, operator.cpp:38-70).  d_lohi: n rows of
  * [lo,hi] in device memory; d_rank receives the rank or -1 if absent. */
@@ -173,6 +181,12 @@ int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
  * algorithm, which materialises H (operator_matrix.hpp:79-105) instead. */
 int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
 
+/* Gathers of x in flight per lane of the cached apply (4, 8, 16 or 32).
+ * Chosen by qsi_op_autotune; set it to pin one depth (results are
+ * bit-identical for all).  No counterpart in the reference. */
+int qsi_op_set_cached_pipe(qsi_op_t op, int pipe);
+int qsi_op_get_cached_pipe(qsi_op_t op);
+
 /* Autotuner: times the apply kernels on this operator (reps calls each, after
  * a warm-up, with HIP events) and keeps the fastest.  R = 0 tunes the
  * single-vector kernel (the QSI_APPLY_* variants that apply: hash variants
@@ -185,6 +199,13 @@ int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
  * the reference.  Synchronises. */
 int qsi_op_autotune(qsi_op_t op, int R, int reps, double *d_x, double *d_y,
                     int *best, double *ms);
+/* Autotuner for the multi-sector block apply of qsi_lanczos_block: times every
+ * configuration on the launch over all n_sectors operators (rows
+ * concatenated, R columns, scratch blocks allocated) and sets the chosen one
+ * on every operator.  *best, *ms as for qsi_op_autotune.  No counterpart in
+ * the reference.  Synchronises. */
+int qsi_block_autotune(const qsi_op_t *ops, int n_sectors, int R, int reps, int *best,
+                       double *ms);
 
 /* Connectivity materialiser (the matrix-free equivalent of
  * LazyOpSum::toSparseMatrix, operator_matrix.hpp:79-105): writes the nonzero

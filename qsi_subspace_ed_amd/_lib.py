@@ -91,6 +91,8 @@ SIGNATURES = {
     "qsi_basis_dim": (_I64, [_P]),
     "qsi_basis_n_spins": (C.c_int, [_P]),
     "qsi_basis_bucket_bits": (C.c_int, [_P]),
+    "qsi_basis_key_fields": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
     "qsi_basis_search": (C.c_int, [_P, _P, _I64, _P, _P]),
     "qsi_op_create": (C.c_int, [_P, C.POINTER(Term), C.c_int, C.c_int, C.POINTER(_P)]),
     "qsi_op_destroy": (C.c_int, [_P]),
@@ -99,11 +101,14 @@ SIGNATURES = {
     "qsi_op_n_diag": (C.c_int, [_P]),
     "qsi_op_set_variant": (C.c_int, [_P, C.c_int]),
     "qsi_op_get_variant": (C.c_int, [_P]),
+    "qsi_op_set_cached_pipe": (C.c_int, [_P, C.c_int]),
+    "qsi_op_get_cached_pipe": (C.c_int, [_P]),
     "qsi_op_apply": (C.c_int, [_P, _P, _P, C.c_int, _P]),
     "qsi_op_apply_host": (C.c_int, [_P, _P, _P, C.c_int]),
     "qsi_op_apply_block": (C.c_int, [_P, _P, _I64, _P, _I64, C.c_int, _P, C.c_int, _P]),
     "qsi_op_cache_connectivity": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     "qsi_op_autotune": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.POINTER(C.c_int), _DP]),
+    "qsi_block_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _DP]),
     "qsi_op_connectivity": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P, _P]),
     "qsi_lanczos_eigval0": (C.c_int, [_P, C.POINTER(LanczosSettings), _DP, _P,
                                       C.POINTER(LanczosResult)]),

@@ -68,11 +68,9 @@ __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>
     lo[g] = 0;
     n[g] = 0;
     if (act[g]) {
-      const uint32_t k = bucket_key<NW>(key[g], b.shift);
-      if (k < (1u << b.kbits)) {
-        lo[g] = b.table[k];
-        n[g] = b.table[k + 1] - lo[g];
-      }
+      const uint32_t k = bucket_key<NW>(b, key[g]);
+      lo[g] = b.table[k];
+      n[g] = b.table[k + 1] - lo[g];
     }
   }
   bool more = true;
@@ -684,17 +682,12 @@ int op_cache_connectivity(qsi_op_s* op, bool enable) {
 
 using namespace qsi;
 
-extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
-                                 const double* const* d_V0,
-                                 const double* const* d_coeffs, int M,
-                                 double* h_alpha, double* h_beta, int use_graph,
-                                 double* elapsed_ms) {
-  QSI_REQUIRE(ops && d_V0 && h_alpha && h_beta, "qsi_lanczos_block: null argument");
-  QSI_REQUIRE(n_sectors >= 1 && n_sectors <= kMaxSectors,
-              "qsi_lanczos_block: n_sectors must be in [1, 8]");
-  QSI_REQUIRE(R >= 1 && R <= 64, "qsi_lanczos_block: R must be in [1, 64]");
-  QSI_REQUIRE(M >= 1, "qsi_lanczos_block: M must be >= 1");
-  const int nw = ops[0]->basis->nw;
+namespace {
+
+// Block-apply arguments over the sectors' operators (rows concatenated in
+// sector order), R columns with row stride R; *rows = total rows.
+BlkArgs sector_args(const qsi_op_t* ops, int n_sectors, int R, const double* const* d_coeffs,
+                    int64_t* rows_out) {
   BlkArgs args{};
   args.n_sec = n_sectors;
   args.R = R;
@@ -707,11 +700,6 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
   int64_t tiles = 0, rows = 0;
   for (int s = 0; s < n_sectors; ++s) {
     const qsi_op_s* op = ops[s];
-    QSI_REQUIRE(op != nullptr && op->basis->dim > 0, "qsi_lanczos_block: empty sector");
-    QSI_REQUIRE(op->basis->nw == nw, "qsi_lanczos_block: sectors mix <=64 and >64 spins");
-    QSI_REQUIRE(d_V0[s] != nullptr, "qsi_lanczos_block: null start block");
-    QSI_REQUIRE(!d_coeffs || !d_coeffs[s] || op->n_slots > 0,
-                "qsi_lanczos_block: coefficients given for an operator without slots");
     BlkSector& S = args.sec[s];
     S.b = op->basis->view();
     S.off = op->d_off;
@@ -730,6 +718,34 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
     rows += op->basis->dim;
   }
   args.n_tiles = tiles;
+  args.cfg = ops[0]->blk_cfg;
+  *rows_out = rows;
+  return args;
+}
+
+}  // namespace
+
+extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
+                                 const double* const* d_V0,
+                                 const double* const* d_coeffs, int M,
+                                 double* h_alpha, double* h_beta, int use_graph,
+                                 double* elapsed_ms) {
+  QSI_REQUIRE(ops && d_V0 && h_alpha && h_beta, "qsi_lanczos_block: null argument");
+  QSI_REQUIRE(n_sectors >= 1 && n_sectors <= kMaxSectors,
+              "qsi_lanczos_block: n_sectors must be in [1, 8]");
+  QSI_REQUIRE(R >= 1 && R <= 64, "qsi_lanczos_block: R must be in [1, 64]");
+  QSI_REQUIRE(M >= 1, "qsi_lanczos_block: M must be >= 1");
+  const int nw = ops[0]->basis->nw;
+  for (int s = 0; s < n_sectors; ++s) {
+    QSI_REQUIRE(ops[s] != nullptr && ops[s]->basis->dim > 0, "qsi_lanczos_block: empty sector");
+    QSI_REQUIRE(ops[s]->basis->nw == nw, "qsi_lanczos_block: sectors mix <=64 and >64 spins");
+    QSI_REQUIRE(d_V0[s] != nullptr, "qsi_lanczos_block: null start block");
+    QSI_REQUIRE(!d_coeffs || !d_coeffs[s] || ops[s]->n_slots > 0,
+                "qsi_lanczos_block: coefficients given for an operator without slots");
+  }
+  int64_t rows = 0;
+  BlkArgs args = sector_args(ops, n_sectors, R, d_coeffs, &rows);
+  const int64_t tiles = args.n_tiles;
   // persistent grid: enough wavefronts to fill the chip, never more than tiles
   args.cfg = ops[0]->blk_cfg;
   const int n_blocks = blk_apply_blocks(nw, tiles, blk_cached(args), args.cfg);
@@ -884,7 +900,72 @@ int time_ms(F&& f, int reps, hipStream_t s, double* ms) {
   return rc;
 }
 
+// Times every block-apply configuration (kernel x grid) of `a` on X, Y
+// (rows x R) and returns the fastest.
+int tune_block(BlkArgs a, int nw, int reps, const double* X, double* Y, int* pick,
+               double* pick_ms) {
+  const bool cached = blk_cached(a);
+  const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
+  *pick = -1;
+  for (int k : kernels) {
+    for (int half = 0; half < 2; ++half) {
+      a.cfg = k | (half << 2);
+      const int nb = blk_apply_blocks(nw, a.n_tiles, cached, a.cfg);
+      double ms = 0.0;
+      const int rc = time_ms([&]() { return launch_blk_apply(a, nw, nb, X, Y, nullptr, nullptr); },
+                             reps, nullptr, &ms);
+      if (rc != QSI_OK) return rc;
+      if (*pick < 0 || ms < *pick_ms) {
+        *pick = a.cfg;
+        *pick_ms = ms;
+      }
+    }
+  }
+  return QSI_OK;
+}
+
 }  // namespace
+
+extern "C" int qsi_block_autotune(const qsi_op_t* ops, int n_sectors, int R, int reps, int* best,
+                                  double* best_ms) {
+  QSI_REQUIRE(ops != nullptr, "qsi_block_autotune: null argument");
+  QSI_REQUIRE(n_sectors >= 1 && n_sectors <= kMaxSectors,
+              "qsi_block_autotune: n_sectors must be in [1, 8]");
+  QSI_REQUIRE(R >= 1 && R <= 64, "qsi_block_autotune: R must be in [1, 64]");
+  const int nw = ops[0]->basis->nw;
+  for (int s = 0; s < n_sectors; ++s) {
+    QSI_REQUIRE(ops[s] != nullptr && ops[s]->basis->dim > 0, "qsi_block_autotune: empty sector");
+    QSI_REQUIRE(ops[s]->basis->nw == nw, "qsi_block_autotune: sectors mix <=64 and >64 spins");
+  }
+  reps = std::max(1, reps);
+  DevMem mem;
+  std::vector<const double*> cp(n_sectors, nullptr);
+  for (int s = 0; s < n_sectors; ++s) {
+    if (ops[s]->n_slots > 0) {
+      double* c = nullptr;
+      QSI_CHECK_HIP(mem.alloc(&c, static_cast<size_t>(ops[s]->n_slots) * R));
+      const int rc = launch_randn(c, static_cast<int64_t>(ops[s]->n_slots) * R, 7 + s, nullptr);
+      if (rc != QSI_OK) return rc;
+      cp[s] = c;
+    }
+  }
+  int64_t rows = 0;
+  BlkArgs a = sector_args(ops, n_sectors, R, cp.data(), &rows);
+  double *X = nullptr, *Y = nullptr;
+  QSI_CHECK_HIP(mem.alloc(&X, static_cast<size_t>(rows) * R));
+  QSI_CHECK_HIP(mem.alloc(&Y, static_cast<size_t>(rows) * R));
+  int rc = launch_randn(X, rows * R, 12345, nullptr);
+  if (rc != QSI_OK) return rc;
+  QSI_CHECK_HIP(hipDeviceSynchronize());
+  int pick = -1;
+  double pick_ms = 0.0;
+  rc = tune_block(a, nw, reps, X, Y, &pick, &pick_ms);
+  if (rc != QSI_OK) return rc;
+  for (int s = 0; s < n_sectors; ++s) ops[s]->blk_cfg = pick;
+  if (best) *best = pick;
+  if (best_ms) *best_ms = pick_ms;
+  return QSI_OK;
+}
 
 extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double* d_y,
                                int* best, double* best_ms) {
@@ -949,42 +1030,11 @@ extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double
     op->variant = pick;
     op->cached_pipe = pick_pipe;
   } else {
-    BlkArgs a{};
-    a.n_sec = 1;
-    a.R = R;
-    int G = 64;
-    while (G / 2 >= R && G > 1) G >>= 1;
-    a.G = G;
-    a.ldx = a.ldy = R;
-    a.sec[0].b = op->basis->view();
-    a.sec[0].off = op->d_off;
-    a.sec[0].dg = op->d_diag;
-    a.sec[0].cls = op->d_cls;
-    a.sec[0].n_cls = op->n_cls;
-    a.sec[0].coeffs = coeffs;
-    a.sec[0].n_off = op->n_off;
-    a.sec[0].n_diag = op->n_diag;
-    a.sec[0].rowptr = op->d_rowptr;
-    a.sec[0].cj = op->d_cj;
-    a.sec[0].cm = op->d_cm;
-    a.n_tiles = (D + kTileRows - 1) / kTileRows;
-    const bool cached = blk_cached(a);
-    const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
-    for (int k : kernels) {
-      for (int half = 0; half < 2; ++half) {
-        a.cfg = k | (half << 2);
-        const int nb = blk_apply_blocks(op->basis->nw, a.n_tiles, cached, a.cfg);
-        double ms = 0.0;
-        const int rc = time_ms(
-            [&]() { return launch_blk_apply(a, op->basis->nw, nb, d_x, d_y, nullptr, nullptr); },
-            reps, nullptr, &ms);
-        if (rc != QSI_OK) return rc;
-        if (pick < 0 || ms < pick_ms) {
-          pick = a.cfg;
-          pick_ms = ms;
-        }
-      }
-    }
+    const double* cp[1] = {coeffs};
+    int64_t rows = 0;
+    BlkArgs a = sector_args(&op, 1, R, cp, &rows);
+    const int rc = tune_block(a, op->basis->nw, reps, d_x, d_y, &pick, &pick_ms);
+    if (rc != QSI_OK) return rc;
     op->blk_cfg = pick;
   }
   if (best) *best = pick;

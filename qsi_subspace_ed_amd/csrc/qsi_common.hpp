@@ -95,27 +95,51 @@ __device__ __forceinline__ bool st_eq(const State<NW>& a, const State<NW>& b) {
   }
 }
 
-// Top `kbits` of an n_spins-bit state: (state >> (n_spins - kbits)).
-template <int NW>
-__device__ __forceinline__ uint32_t bucket_key(const State<NW>& s, int shift) {
-  if constexpr (NW == 1) {
-    return static_cast<uint32_t>(s.w[0] >> shift);
-  } else {
-    if (shift >= 64) return static_cast<uint32_t>(s.w[1] >> (shift - 64));
-    if (shift == 0) return static_cast<uint32_t>(s.w[0]);
-    return static_cast<uint32_t>((s.w[1] << (64 - shift)) | (s.w[0] >> shift));
-  }
-}
+// Bucket keys.  The key of a state is the concatenation, highest field
+// first, of up to kKeySegs bit fields [lo, lo + w) of the state.  The fields
+// cover the K highest bit positions at which two neighbouring states of the
+// sorted basis first differ ("informative" positions): a position where no
+// neighbouring pair first differs is fixed by the bits above it in every
+// basis state, so leaving it out keeps the key non-decreasing along the basis
+// while the same K key bits split the basis into smaller buckets than the top
+// K bits do (ice-rule bases fix many positions).  With no informative
+// positions the key is the top K bits.
+constexpr int kKeySegs = 4;
 
-// Device view of a basis: states + top-bit bucket table.
+// Device view of a basis: states + bucket table.
 struct BasisView {
   const uint64_t* states;  // D * NW words
   const uint32_t* table;   // 2^K + 1 entries: table[k] = #states with key < k
   int64_t dim;
-  int shift;               // n_spins - K
-  int kbits;
+  int kbits;               // K = sum of the field widths
   int n_spins;
+  int n_seg;
+  uint8_t seg_lo[kKeySegs];
+  uint8_t seg_w[kKeySegs];
 };
+
+template <int NW>
+__host__ __device__ __forceinline__ uint32_t state_field(const State<NW>& s, int lo, int w) {
+  uint64_t v;
+  if constexpr (NW == 1) {
+    v = s.w[0] >> lo;
+  } else {
+    if (lo >= 64) v = s.w[1] >> (lo - 64);
+    else if (lo == 0) v = s.w[0];
+    else v = (s.w[1] << (64 - lo)) | (s.w[0] >> lo);
+  }
+  return static_cast<uint32_t>(v & ((uint64_t{1} << w) - 1));
+}
+
+template <int NW>
+__host__ __device__ __forceinline__ uint32_t bucket_key(const BasisView& b, const State<NW>& s) {
+  uint32_t k = 0;
+#pragma unroll
+  for (int q = 0; q < kKeySegs; ++q) {
+    if (q < b.n_seg) k = (k << b.seg_w[q]) | state_field<NW>(s, b.seg_lo[q], b.seg_w[q]);
+  }
+  return k;
+}
 
 // true if the state has a bit at or above n_spins (it is then larger than
 // every basis state)
@@ -138,12 +162,12 @@ __device__ __forceinline__ bool st_lt(const State<NW>& a, const State<NW>& b) {
 }
 
 // Rank of `key` in the basis, or -1.  The bucket [table[k], table[k+1]) is
-// searched with a branch-free lower-bound bisection.
+// searched with a branch-free lower-bound bisection.  A basis state's key
+// selects the bucket that holds it; any other state is reported absent.
 template <int NW>
 __device__ __forceinline__ int64_t basis_lookup(const BasisView& b,
                                                 const State<NW>& key) {
-  const uint32_t k = bucket_key<NW>(key, b.shift);
-  if (k >= (1u << b.kbits)) return -1;
+  const uint32_t k = bucket_key<NW>(b, key);
   uint32_t lo = b.table[k];
   uint32_t n = b.table[k + 1] - lo;
   if (n == 0) return -1;
@@ -197,25 +221,34 @@ __device__ __forceinline__ int64_t hash_lookup(const HashView& h,
 }
 
 // First rank whose state is >= key (dim if none): bucket from the table,
-// then a lower-bound bisection inside it.
+// then a lower-bound bisection inside it.  A state outside the basis may
+// differ from the basis at a position the key leaves out, so its bucket need
+// not bound its place: the result is checked against its neighbours and, if
+// it is off, the bisection is redone over the whole basis.
 template <int NW>
 __device__ __forceinline__ int64_t basis_lower_bound(const BasisView& b,
                                                      const State<NW>& key) {
   if (above_range<NW>(key, b.n_spins)) return b.dim;
-  const uint32_t k = bucket_key<NW>(key, b.shift);
-  uint32_t lo = b.table[k];
-  uint32_t n = b.table[k + 1] - lo;
-  while (n > 0) {
-    const uint32_t half = n >> 1;
-    const State<NW> m = load_state_cached<NW>(b.states, lo + half);
-    if (st_lt<NW>(m, key)) {
-      lo += half + 1;
-      n -= half + 1;
-    } else {
-      n = half;
+  auto bisect = [&](int64_t lo, int64_t n) {
+    while (n > 0) {
+      const int64_t half = n >> 1;
+      const State<NW> m = load_state_cached<NW>(b.states, lo + half);
+      if (st_lt<NW>(m, key)) {
+        lo += half + 1;
+        n -= half + 1;
+      } else {
+        n = half;
+      }
     }
-  }
-  return static_cast<int64_t>(lo);
+    return lo;
+  };
+  const uint32_t k = bucket_key<NW>(b, key);
+  const int64_t lo0 = b.table[k];
+  const int64_t lo = bisect(lo0, static_cast<int64_t>(b.table[k + 1]) - lo0);
+  const bool below_ok = lo == 0 || st_lt<NW>(load_state_cached<NW>(b.states, lo - 1), key);
+  const bool above_ok = lo == b.dim || !st_lt<NW>(load_state_cached<NW>(b.states, lo), key);
+  if (below_ok && above_ok) return lo;
+  return bisect(0, b.dim);
 }
 
 }  // namespace qsi

@@ -40,18 +40,16 @@ constexpr double kSparseTol = 1e-14;  // toSparseMatrix default tol
 // Basis: bucket table
 // ============================================================================
 
-// table[k] = first rank whose top-K key is >= k, for k in [0, 2^K].
+// table[k] = first rank whose bucket key is >= k, for k in [0, 2^K].
 template <int NW>
-__global__ void k_build_table(const uint64_t* __restrict__ states, int64_t dim,
-                              int shift, uint32_t nkeys,
-                              uint32_t* __restrict__ table) {
+__global__ void k_build_table(BasisView b, uint32_t nkeys, uint32_t* __restrict__ table) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k > nkeys) return;
-  int64_t lo = 0, n = dim;
+  int64_t lo = 0, n = b.dim;
   while (n > 0) {
     const int64_t half = n >> 1;
-    const State<NW> s = load_state_cached<NW>(states, lo + half);
-    if (static_cast<int64_t>(bucket_key<NW>(s, shift)) < k) {
+    const State<NW> s = load_state_cached<NW>(b.states, lo + half);
+    if (static_cast<int64_t>(bucket_key<NW>(b, s)) < k) {
       lo = lo + half + 1;
       n = n - half - 1;
     } else {
@@ -59,6 +57,37 @@ __global__ void k_build_table(const uint64_t* __restrict__ states, int64_t dim,
     }
   }
   table[k] = static_cast<uint32_t>(lo);
+}
+
+// mask[p / 64] bit p % 64 = 1 if some neighbouring pair of the sorted basis
+// first differs at bit p.  Grid-stride; one global OR per word per block.
+template <int NW>
+__global__ __launch_bounds__(256) void k_diff_bits(const uint64_t* __restrict__ states,
+                                                   int64_t dim,
+                                                   unsigned long long* __restrict__ mask) {
+  __shared__ unsigned long long sm[2];
+  if (threadIdx.x < 2) sm[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long m0 = 0, m1 = 0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i + 1 < dim;
+       i += stride) {
+    const State<NW> a = load_state_cached<NW>(states, i);
+    const State<NW> c = load_state_cached<NW>(states, i + 1);
+    if constexpr (NW == 2) {
+      const uint64_t hi = a.w[1] ^ c.w[1];
+      if (hi) {
+        m1 |= 1ull << (63 - __clzll(hi));
+        continue;
+      }
+    }
+    const uint64_t lo = a.w[0] ^ c.w[0];
+    if (lo) m0 |= 1ull << (63 - __clzll(lo));
+  }
+  if (m0) atomicOr(&sm[0], m0);
+  if (m1) atomicOr(&sm[1], m1);
+  __syncthreads();
+  if (threadIdx.x < 2 && sm[threadIdx.x]) atomicOr(&mask[threadIdx.x], sm[threadIdx.x]);
 }
 
 // Insert every basis state into the hash index (slots pre-filled with ones).
@@ -156,11 +185,9 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
           lo[g] = 0;
           n[g] = 0;
           if (act[g]) {
-            const uint32_t k = bucket_key<NW>(key[g], b.shift);
-            if (k < (1u << b.kbits)) {
-              lo[g] = b.table[k];
-              n[g] = b.table[k + 1] - lo[g];
-            }
+            const uint32_t k = bucket_key<NW>(b, key[g]);
+            lo[g] = b.table[k];
+            n[g] = b.table[k + 1] - lo[g];
           }
         }
         // interleaved bisection down to one candidate per key
@@ -1062,6 +1089,81 @@ static int auto_bucket_bits(int64_t dim, int n_spins) {
   return std::max(k, 1);
 }
 
+
+// Chooses the bucket key fields from the basis's informative bit positions
+// (k_diff_bits) and builds the bucket table.  K = bucket_bits, or automatic;
+// at most the informative positions (a smaller K then splits the basis into
+// single states) and at most kKeySegs fields.  Synchronises.
+static hipError_t basis_build_index(qsi_basis_s* b, int bucket_bits) {
+  int K = bucket_bits ? bucket_bits : auto_bucket_bits(b->dim, b->n_spins);
+  unsigned long long hmask[2] = {0, 0};
+  hipError_t e = hipSuccess;
+  if (b->dim >= 2) {
+    unsigned long long* dmask = nullptr;
+    e = hipMalloc(&dmask, 2 * sizeof(unsigned long long));
+    if (e != hipSuccess) return e;
+    e = hipMemset(dmask, 0, 2 * sizeof(unsigned long long));
+    const int64_t want = (b->dim + 255) / 256;
+    const uint32_t grid = static_cast<uint32_t>(std::min<int64_t>(want, 8192));
+    if (e == hipSuccess) {
+      if (b->nw == 1)
+        hipLaunchKernelGGL(k_diff_bits<1>, dim3(grid), dim3(256), 0, 0, b->d_states, b->dim, dmask);
+      else
+        hipLaunchKernelGGL(k_diff_bits<2>, dim3(grid), dim3(256), 0, 0, b->d_states, b->dim, dmask);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpy(hmask, dmask, sizeof(hmask), hipMemcpyDeviceToHost);
+    (void)hipFree(dmask);
+    if (e != hipSuccess) return e;
+  }
+  // informative positions from the top; a run starts a new field, and once
+  // kKeySegs fields exist the last one is extended down over the gap (the
+  // fixed positions it then covers do not break the ordering)
+  b->n_seg = 0;
+  int taken = 0;  // key width so far
+  for (int p = b->n_spins - 1; p >= 0 && taken < K; --p) {
+    if (!((hmask[p >> 6] >> (p & 63)) & 1ull)) continue;
+    const int q = b->n_seg - 1;
+    if (q >= 0 && b->seg_lo[q] == p + 1) {
+      b->seg_lo[q] = static_cast<uint8_t>(p);
+      b->seg_w[q] += 1;
+      taken += 1;
+    } else if (b->n_seg < kKeySegs) {
+      b->seg_lo[b->n_seg] = static_cast<uint8_t>(p);
+      b->seg_w[b->n_seg] = 1;
+      ++b->n_seg;
+      taken += 1;
+    } else {
+      const int grow = b->seg_lo[q] - p;
+      if (taken + grow > K) break;
+      b->seg_lo[q] = static_cast<uint8_t>(p);
+      b->seg_w[q] += grow;
+      taken += grow;
+    }
+  }
+  if (taken == 0) {  // no informative position (dim < 2): the top K bits
+    K = std::min(K, b->n_spins);
+    b->n_seg = 1;
+    b->seg_lo[0] = static_cast<uint8_t>(b->n_spins - K);
+    b->seg_w[0] = static_cast<uint8_t>(K);
+    taken = K;
+  }
+  b->kbits = taken;
+  const size_t nkeys = size_t{1} << b->kbits;
+  e = hipMalloc(&b->d_table, (nkeys + 1) * sizeof(uint32_t));
+  if (e != hipSuccess) return e;
+  const uint32_t grid = static_cast<uint32_t>((nkeys + 1 + 255) / 256);
+  const uint32_t nk = static_cast<uint32_t>(nkeys);
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_build_table<1>, dim3(grid), dim3(256), 0, 0, b->view(), nk, b->d_table);
+  else
+    hipLaunchKernelGGL(k_build_table<2>, dim3(grid), dim3(256), 0, 0, b->view(), nk, b->d_table);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return e;
+}
+
 int qsi_basis_create_ex(const uint64_t* h_lohi, int64_t dim, int n_spins,
                         int bucket_bits, int with_hash, qsi_basis_t* out) {
   QSI_REQUIRE(out != nullptr, "qsi_basis_create: out is null");
@@ -1102,11 +1204,8 @@ int qsi_basis_create_ex(const uint64_t* h_lohi, int64_t dim, int n_spins,
   b->dim = dim;
   b->n_spins = n_spins;
   b->nw = nw;
-  b->kbits = bucket_bits ? bucket_bits : auto_bucket_bits(dim, n_spins);
-  b->shift = n_spins - b->kbits;
   (void)hipGetDevice(&b->device);
   const size_t words = static_cast<size_t>(std::max<int64_t>(dim, 1)) * nw;
-  const size_t nkeys = size_t{1} << b->kbits;
   auto fail = [&](hipError_t e, const char* what) {
     set_error(std::string("qsi_basis_create: ") + what + ": " + hipGetErrorString(e));
     basis_release(b);
@@ -1114,8 +1213,6 @@ int qsi_basis_create_ex(const uint64_t* h_lohi, int64_t dim, int n_spins,
   };
   hipError_t e = hipMalloc(&b->d_states, words * sizeof(uint64_t));
   if (e != hipSuccess) return fail(e, "hipMalloc(states)");
-  e = hipMalloc(&b->d_table, (nkeys + 1) * sizeof(uint32_t));
-  if (e != hipSuccess) return fail(e, "hipMalloc(table)");
   if (dim > 0) {
     if (nw == 2) {
       e = hipMemcpy(b->d_states, h_lohi, words * sizeof(uint64_t),
@@ -1131,18 +1228,8 @@ int qsi_basis_create_ex(const uint64_t* h_lohi, int64_t dim, int n_spins,
     e = hipMemset(b->d_states, 0, words * sizeof(uint64_t));
     if (e != hipSuccess) return fail(e, "hipMemset(states)");
   }
-  const uint32_t nk = static_cast<uint32_t>(nkeys);
-  const uint32_t grid = static_cast<uint32_t>((nkeys + 1 + 255) / 256);
-  if (nw == 1)
-    hipLaunchKernelGGL(k_build_table<1>, dim3(grid), dim3(256), 0, 0,
-                       b->d_states, dim, b->shift, nk, b->d_table);
-  else
-    hipLaunchKernelGGL(k_build_table<2>, dim3(grid), dim3(256), 0, 0,
-                       b->d_states, dim, b->shift, nk, b->d_table);
-  e = hipGetLastError();
-  if (e != hipSuccess) return fail(e, "k_build_table launch");
-  e = hipDeviceSynchronize();
-  if (e != hipSuccess) return fail(e, "k_build_table");
+  e = basis_build_index(b, bucket_bits);
+  if (e != hipSuccess) return fail(e, "bucket table");
   if (with_hash && dim > 0) {
     int hb = 1;
     while ((int64_t{1} << hb) < 2 * dim) ++hb;  // load factor <= 1/2
@@ -1214,8 +1301,6 @@ int basis_adopt_device(uint64_t* d_states, int64_t dim, int n_spins, int bucket_
   b->dim = dim;
   b->n_spins = n_spins;
   b->nw = n_spins <= 64 ? 1 : 2;
-  b->kbits = bucket_bits ? bucket_bits : auto_bucket_bits(dim, n_spins);
-  b->shift = n_spins - b->kbits;
   (void)hipGetDevice(&b->device);
   b->d_states = d_states;
   auto fail = [&](hipError_t e, const char* what) {
@@ -1245,20 +1330,8 @@ int basis_adopt_device(uint64_t* d_states, int64_t dim, int n_spins, int bucket_
     basis_release(b);
     return QSI_E_INVALID;
   }
-  const size_t nkeys = size_t{1} << b->kbits;
-  e = hipMalloc(&b->d_table, (nkeys + 1) * sizeof(uint32_t));
-  if (e != hipSuccess) return fail(e, "hipMalloc(table)");
-  const uint32_t grid = static_cast<uint32_t>((nkeys + 1 + 255) / 256);
-  const uint32_t nk = static_cast<uint32_t>(nkeys);
-  if (b->nw == 1)
-    hipLaunchKernelGGL(k_build_table<1>, dim3(grid), dim3(256), 0, 0, b->d_states, dim, b->shift,
-                       nk, b->d_table);
-  else
-    hipLaunchKernelGGL(k_build_table<2>, dim3(grid), dim3(256), 0, 0, b->d_states, dim, b->shift,
-                       nk, b->d_table);
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e != hipSuccess) return fail(e, "k_build_table");
+  e = basis_build_index(b, bucket_bits);
+  if (e != hipSuccess) return fail(e, "bucket table");
   *out = b;
   return QSI_OK;
 }
@@ -1292,6 +1365,16 @@ int qsi_basis_destroy(qsi_basis_t b) {
 int64_t qsi_basis_dim(qsi_basis_t b) { return b ? b->dim : -1; }
 int qsi_basis_n_spins(qsi_basis_t b) { return b ? b->n_spins : -1; }
 int qsi_basis_bucket_bits(qsi_basis_t b) { return b ? b->kbits : -1; }
+
+int qsi_basis_key_fields(qsi_basis_t b, int* n_fields, int* lo, int* width) {
+  QSI_REQUIRE(b && n_fields && lo && width, "qsi_basis_key_fields: null argument");
+  *n_fields = b->n_seg;
+  for (int q = 0; q < kKeySegs; ++q) {
+    lo[q] = q < b->n_seg ? b->seg_lo[q] : 0;
+    width[q] = q < b->n_seg ? b->seg_w[q] : 0;
+  }
+  return QSI_OK;
+}
 
 int qsi_basis_search(qsi_basis_t b, const uint64_t* d_lohi, int64_t n,
                      int64_t* d_rank, qsi_stream_t stream) {
@@ -1484,6 +1567,16 @@ int qsi_op_set_variant(qsi_op_t op, int variant) {
 }
 
 int qsi_op_get_variant(qsi_op_t op) { return op ? op->variant : -1; }
+
+int qsi_op_set_cached_pipe(qsi_op_t op, int pipe) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_set_cached_pipe: null operator");
+  QSI_REQUIRE(pipe == 4 || pipe == 8 || pipe == 16 || pipe == 32,
+              "qsi_op_set_cached_pipe: pipe must be 4, 8, 16 or 32");
+  op->cached_pipe = pipe;
+  return QSI_OK;
+}
+
+int qsi_op_get_cached_pipe(qsi_op_t op) { return op ? op->cached_pipe : -1; }
 int qsi_op_n_offdiag(qsi_op_t op) { return op ? op->n_off : -1; }
 int qsi_op_n_diag(qsi_op_t op) { return op ? op->n_diag : -1; }
 

@@ -56,7 +56,9 @@ struct qsi_basis_s {
   int n_spins = 0;
   int nw = 1;
   int kbits = 0;
-  int shift = 0;
+  int n_seg = 0;  // bucket key fields (see BasisView)
+  uint8_t seg_lo[qsi::kKeySegs] = {};
+  uint8_t seg_w[qsi::kKeySegs] = {};
   int device = 0;
   uint64_t* d_states = nullptr;  // dim * nw words
   uint32_t* d_table = nullptr;   // 2^kbits + 1
@@ -77,9 +79,13 @@ struct qsi_basis_s {
     v.states = d_states;
     v.table = d_table;
     v.dim = dim;
-    v.shift = shift;
     v.kbits = kbits;
     v.n_spins = n_spins;
+    v.n_seg = n_seg;
+    for (int q = 0; q < qsi::kKeySegs; ++q) {
+      v.seg_lo[q] = seg_lo[q];
+      v.seg_w[q] = seg_w[q];
+    }
     return v;
   }
 };

@@ -8,9 +8,11 @@
 // column by one Lanczos step:
 //   gather    X[:, c] = V_c[:, j_c]                (D x R row-major block)
 //   apply     W = H_c X  (one block launch, per-column coefficients)
-//   transpose W -> column-major w_c
-//   2 x (h_c = V_c[:, 0..j_c]^T w_c ; w_c -= V_c h_c)   (all columns per launch)
-//   beta_c = |w_c| ; V_c[:, j_c+1] = w_c / beta_c
+//   h_c = V_c[:, 0..j_c]^T w_c ; u_c = w_c - V_c h_c ; h2_c = V_c^T u_c
+//   if |h2_c| > 1e-12 |u_c|: u_c -= V_c h2_c (second Gram-Schmidt pass)
+//   beta_c = |u_c| ; V_c[:, j_c+1] = u_c / beta_c
+// Active columns take apply slots 0..n_act in column order, so a step's
+// launches cost what its active columns need, not what the engine can hold.
 // then the host updates each column's projected matrix and, when a column's
 // basis is full, runs its Rayleigh-Ritz step, convergence test and restart.
 // Requests join free columns between steps and leave when converged.  Every
@@ -35,22 +37,27 @@ namespace {
 constexpr int kSrvThreads = 256;
 constexpr int kSrvRowsPerBlock = 4096;
 constexpr int kHMax = 64;
+// partials slot holding |u|^2 (projections use slots 0..ncv-1 <= 62)
+constexpr int kNormSlot = kHMax - 1;
+// the second Gram-Schmidt pass runs when |h2|^2 > kReorthTol2 |u|^2
+constexpr double kReorthTol2 = 1e-24;
 
-// Start of a step, one block per 64-row tile: for every active column c,
-// finish normalising V_c[j_c] (scale invb[c] when pend[c]) and copy it to
-// the row-major apply input X[row, c]; X[row, c] = 0 for idle columns.
+// Start of a step, one block per 64-row tile: for every apply slot sl <
+// n_act (column c = slotcol[sl]), finish normalising V_c[j_c] (scale invb[c]
+// when pend[c]) and copy it to the row-major apply input X[row, sl].
 __global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
     double* __restrict__ Vb, int64_t ld, int ncv1, const int* __restrict__ colj,
-    const int* __restrict__ pend, const double* __restrict__ invb, int R, int64_t D,
-    double* __restrict__ X) {
+    const int* __restrict__ pend, const double* __restrict__ invb,
+    const int* __restrict__ slotcol, int n_act, int64_t D, double* __restrict__ X) {
   __shared__ double tile[64][65];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
   const int64_t r = r0 + lane;
-  for (int c = wid; c < R; c += kSrvThreads / 64) {
+  for (int sl = wid; sl < n_act; sl += kSrvThreads / 64) {
+    const int c = slotcol[sl];
     const int j = colj[c];
     double v = 0.0;
-    if (j >= 0 && r < D) {
+    if (r < D) {
       double* vp = Vb + (static_cast<int64_t>(c) * ncv1 + j) * ld + r;
       v = *vp;
       if (pend[c]) {
@@ -58,12 +65,12 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
         *vp = v;
       }
     }
-    tile[lane][c] = v;
+    tile[lane][sl] = v;
   }
   __syncthreads();
   const int64_t rows = min<int64_t>(64, D - r0);
-  for (int t = threadIdx.x; t < rows * R; t += kSrvThreads)
-    X[r0 * R + t] = tile[t / R][t % R];
+  for (int t = threadIdx.x; t < rows * n_act; t += kSrvThreads)
+    X[r0 * n_act + t] = tile[t / n_act][t % n_act];
 }
 
 // Per-block partial dot products of column c's w with its basis vectors
@@ -89,17 +96,19 @@ __device__ __forceinline__ void srv_block_dots(double (&acc)[MMAX], int m, int c
   }
 }
 
-// pass 1: partials of V_c^T w_c.  Grid (nblk, R).
+// pass 1: partials of V_c^T w_c.  Grid (nblk, n_act); w_c is apply slot
+// blockIdx.y.
 template <int MMAX>
 __global__ __launch_bounds__(kSrvThreads) void k_srv_mdot(
     const double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ Wc,
-    const int* __restrict__ colj, int64_t D, double* __restrict__ partials) {
-  const int c = blockIdx.y;
+    const int* __restrict__ colj, const int* __restrict__ slotcol, int64_t D,
+    double* __restrict__ partials) {
+  const int sl = blockIdx.y;
+  const int c = slotcol[sl];
   const int j = colj[c];
-  if (j < 0) return;
   const int m = j + 1;
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
-  const double* w = Wc + static_cast<int64_t>(c) * ld;
+  const double* w = Wc + static_cast<int64_t>(sl) * ld;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
   const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
   double acc[MMAX];
@@ -114,39 +123,41 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_mdot(
   srv_block_dots<MMAX>(acc, m, c, gridDim.x, partials);
 }
 
-// pass 1 -> 2: w_c -= V_c h_c (h from pass 1), then partials of V_c^T w_c.
-// Grid (nblk, R).  The block walks its rows in 64-row tiles; wave k holds
+// pass 1 -> 2: u_c = w_c - V_c h_c (h from pass 1) written to V_c[j_c + 1],
+// then partials of V_c^T u_c (slots 0..j_c) and of |u_c|^2 (kNormSlot).
+// Grid (nblk, n_act).  The block walks its rows in 64-row tiles; wave k holds
 // basis vectors i = k, k + 4, ... in registers, the four waves' parts of
 // (V h)[row] meet in LDS, and each wave reuses its V entries for the dots.
 template <int MMAX>
 __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
-    const double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
-    const int* __restrict__ colj, int64_t D, double* __restrict__ Wc,
-    double* __restrict__ partials) {
+    double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
+    const int* __restrict__ colj, const int* __restrict__ slotcol, int64_t D,
+    const double* __restrict__ Wc, double* __restrict__ partials) {
   constexpr int NWV = kSrvThreads / 64;
   constexpr int Q = (MMAX + NWV - 1) / NWV;
-  const int c = blockIdx.y;
+  const int sl = blockIdx.y;
+  const int c = slotcol[sl];
   const int j = colj[c];
-  if (j < 0) return;
   const int m = j + 1;
   __shared__ double hs[MMAX];
   __shared__ double ared[NWV][64];
   __shared__ double red[MMAX];
+  __shared__ double nsh;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x];
   __syncthreads();
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
-  double* w = Wc + static_cast<int64_t>(c) * ld;
+  double* out = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
+  const double* w = Wc + static_cast<int64_t>(sl) * ld;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
   const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
   double acc[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+  double nrm = 0.0;
   for (int64_t rt = r0; rt < r1; rt += 64) {
     const int64_t r = rt + lane;
     const bool ok = r < r1;
-    // every wave reads w[r] before the barrier; wave 0 overwrites it after
-    const double w_old = ok ? w[r] : 0.0;
     double v[Q];
     double a = 0.0;
 #pragma unroll
@@ -157,16 +168,19 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
     }
     ared[wid][lane] = a;
     __syncthreads();
-    double wr = 0.0;
+    double ur = 0.0;
     if (ok) {
-      double s = 0.0;
+      double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < NWV; ++k) s += ared[k][lane];
-      wr = w_old - s;
-      if (wid == 0) w[r] = wr;
+      for (int k = 0; k < NWV; ++k) t += ared[k][lane];
+      ur = w[r] - t;
+      if (wid == 0) {
+        out[r] = ur;
+        nrm += ur * ur;
+      }
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) acc[q] += v[q] * wr;
+    for (int q = 0; q < Q; ++q) acc[q] += v[q] * ur;
     __syncthreads();
   }
 #pragma unroll
@@ -175,29 +189,59 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
     const double t = wave_sum(acc[q]);
     if (lane == 0 && i < m) red[i] = t;
   }
+  if (wid == 0) {
+    const double t = wave_sum(nrm);
+    if (lane == 0) nsh = t;
+  }
   __syncthreads();
-  if (threadIdx.x < m)
-    partials[(static_cast<int64_t>(c) * gridDim.x + blockIdx.x) * kHMax + threadIdx.x] =
-        red[threadIdx.x];
+  double* pp = partials + (static_cast<int64_t>(c) * gridDim.x + blockIdx.x) * kHMax;
+  if (threadIdx.x < m) pp[threadIdx.x] = red[threadIdx.x];
+  if (threadIdx.x == 0) pp[kNormSlot] = nsh;
 }
 
-// pass 2: u = w_c - V_c h_c written to V_c[j_c + 1] (normalised by the next
-// step's prep), partials of |u|^2 in partials[c*nblk + b].
+// After pass 2's projections h2_c are summed: beta_c = |u_c| from the
+// kNormSlot partials, and flag_c = 1 when |h2_c|^2 > kReorthTol2 |u_c|^2 (u_c
+// needs the second Gram-Schmidt pass).  Grid R, one wave each.
+__global__ void k_srv_check(const double* __restrict__ partials, int nblk,
+                            const int* __restrict__ colj, const double* __restrict__ h,
+                            int* __restrict__ flag, double* __restrict__ beta,
+                            double* __restrict__ invb) {
+  const int c = blockIdx.x;
+  const int j = colj[c];
+  if (j < 0) return;
+  double n2 = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64)
+    n2 += partials[(static_cast<int64_t>(c) * nblk + b) * kHMax + kNormSlot];
+  n2 = wave_sum(n2);
+  double s2 = 0.0;
+  for (int i = threadIdx.x; i <= j; i += 64) {
+    const double v = h[c * kHMax + i];
+    s2 += v * v;
+  }
+  s2 = wave_sum(s2);
+  if (threadIdx.x == 0) {
+    flag[c] = s2 > kReorthTol2 * n2 ? 1 : 0;
+    const double r = sqrt(n2);
+    beta[c] = r;
+    invb[c] = r > 1e-300 ? 1.0 / r : 0.0;
+  }
+}
+
+// pass 3, flagged columns only: u_c -= V_c h2_c in place in V_c[j_c + 1],
+// partials of |u_c|^2 in partials[c*nblk + b].  Grid (nblk, n_act).
 template <int MMAX>
 __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
     double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
-    const int* __restrict__ colj, int64_t D, const double* __restrict__ Wc,
-    double* __restrict__ partials) {
-  const int c = blockIdx.y;
-  const int j = colj[c];
-  if (j < 0) return;
-  const int m = j + 1;
+    const int* __restrict__ colj, const int* __restrict__ slotcol,
+    const int* __restrict__ flag, int64_t D, double* __restrict__ partials) {
+  const int c = slotcol[blockIdx.y];
+  if (!flag[c]) return;
+  const int m = colj[c] + 1;
   __shared__ double hs[MMAX];
   if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x];
   __syncthreads();
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
-  double* out = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
-  const double* w = Wc + static_cast<int64_t>(c) * ld;
+  double* u = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
   const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
   double nrm = 0.0;
@@ -206,9 +250,9 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
 #pragma unroll
     for (int i = 0; i < MMAX; ++i)
       if (i < m) a += V[i * ld + r] * hs[i];
-    const double u = w[r] - a;
-    out[r] = u;
-    nrm += u * u;
+    const double ur = u[r] - a;
+    u[r] = ur;
+    nrm += ur * ur;
   }
   nrm = block_sum<kSrvThreads>(nrm);
   if (threadIdx.x == 0) partials[static_cast<int64_t>(c) * gridDim.x + blockIdx.x] = nrm;
@@ -232,12 +276,13 @@ __global__ void k_srv_fin(const double* __restrict__ partials, int nblk,
   }
 }
 
-// beta[c] = sqrt(sum), invb[c] = 1/beta (0 if it underflows).  Grid R.
+// Flagged columns after pass 3: beta[c] = sqrt(sum), invb[c] = 1/beta (0 if
+// it underflows).  Grid R.
 __global__ void k_srv_beta(const double* __restrict__ partials, int nblk,
-                           const int* __restrict__ colj, double* __restrict__ beta,
-                           double* __restrict__ invb) {
+                           const int* __restrict__ colj, const int* __restrict__ flag,
+                           double* __restrict__ beta, double* __restrict__ invb) {
   const int c = blockIdx.x;
-  if (colj[c] < 0) return;
+  if (colj[c] < 0 || !flag[c]) return;
   double v = 0.0;
   for (int b = threadIdx.x; b < nblk; b += 64) v += partials[static_cast<int64_t>(c) * nblk + b];
   v = wave_sum(v);
@@ -301,13 +346,14 @@ struct qsi_engine_s {
   double *Vb = nullptr, *X = nullptr, *Wc = nullptr, *coeffs = nullptr;
   double *h = nullptr, *hacc = nullptr, *beta = nullptr, *invb = nullptr, *part = nullptr;
   double *tmp = nullptr, *Ydev = nullptr;
-  int *colj_d = nullptr, *pend_d = nullptr;
+  int *colj_d = nullptr, *pend_d = nullptr, *slot_d = nullptr, *flag_d = nullptr;
   std::vector<void*> allocs;
   // pinned staging of the restarts' Ritz coefficients, kHMax^2 per column
   double* Ystage = nullptr;
   // pinned per-step exchange: column state down, projections and norms up
   int* colj_p = nullptr;
   int* pend_p = nullptr;
+  int* slot_p = nullptr;  // slotcol: apply slot -> column
   double* coeffs_p = nullptr;
   double* hh_p = nullptr;
   double* bb_p = nullptr;
@@ -322,7 +368,7 @@ struct qsi_engine_s {
   std::vector<Restart> restarts;
   // host
   std::vector<Col> cols;
-  std::vector<int> colj, pend;
+  std::vector<int> colj, pend, slotcol;
   std::vector<double> coeffs_h;
   std::deque<Request> queue;
   std::deque<Result> done;
@@ -352,10 +398,10 @@ struct qsi_engine_s {
 
 namespace {
 
-// Free columns take queued requests: coefficients, a seeded Gaussian start
-// vector, and its normalisation left pending for the next prep.
+// Free columns take queued requests: coefficients (uploaded per step in slot
+// order), a seeded Gaussian start vector, and its normalisation left pending
+// for the next prep.
 int engine_admit(qsi_engine_s* E) {
-  bool any = false;
   for (int c = 0; c < E->R && !E->queue.empty(); ++c) {
     auto& col = E->cols[c];
     if (col.active) continue;
@@ -374,14 +420,6 @@ int engine_admit(qsi_engine_s* E) {
     if (rc != QSI_OK) return rc;
     hipLaunchKernelGGL(k_srv_setinv, dim3(1), dim3(64), 0, E->st, E->part, c, E->invb);
     QSI_CHECK_LAUNCH();
-    any = true;
-  }
-  if (any) {
-    // through pinned memory: an async copy from pageable memory is not
-    // ordered with the kernels of a non-blocking stream
-    std::copy(E->coeffs_h.begin(), E->coeffs_h.end(), E->coeffs_p);
-    QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_p, E->coeffs_h.size() * sizeof(double),
-                                 hipMemcpyHostToDevice, E->st));
   }
   return QSI_OK;
 }
@@ -458,15 +496,19 @@ template <int MMAX>
 void launch_reorth(qsi_engine_s* E, dim3 gcol, hipStream_t s) {
   const int R = E->R;
   hipLaunchKernelGGL(k_srv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld, E->ncv1,
-                     E->Wc, E->colj_d, E->D, E->part);
+                     E->Wc, E->colj_d, E->slot_d, E->D, E->part);
   hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
                      E->h, E->hacc);
   hipLaunchKernelGGL(k_srv_gemv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
-                     E->ncv1, E->h, E->colj_d, E->D, E->Wc, E->part);
+                     E->ncv1, E->h, E->colj_d, E->slot_d, E->D, E->Wc, E->part);
   hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
                      E->h, E->hacc);
+  hipLaunchKernelGGL(k_srv_check, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d, E->h,
+                     E->flag_d, E->beta, E->invb);
   hipLaunchKernelGGL(k_srv_gemv_norm<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
-                     E->ncv1, E->h, E->colj_d, E->D, E->Wc, E->part);
+                     E->ncv1, E->h, E->colj_d, E->slot_d, E->flag_d, E->D, E->part);
+  hipLaunchKernelGGL(k_srv_beta, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
+                     E->flag_d, E->beta, E->invb);
 }
 
 }  // namespace
@@ -540,7 +582,7 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
     e = hipHostMalloc(reinterpret_cast<void**>(&E->Ystage), R * kHMax * kHMax * sizeof(double),
                       hipHostMallocDefault);
   if (e == hipSuccess)
-    e = hipHostMalloc(&E->pin, R * (kHMax + 2 + ns) * sizeof(double),
+    e = hipHostMalloc(&E->pin, R * (kHMax + 3 + ns) * sizeof(double),
                       hipHostMallocMapped | hipHostMallocCoherent);
   if (e == hipSuccess) {
     E->hh_p = static_cast<double*>(E->pin);
@@ -548,6 +590,7 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
     E->coeffs_p = E->bb_p + R;
     E->colj_p = reinterpret_cast<int*>(E->coeffs_p + R * ns);
     E->pend_p = E->colj_p + R;
+    E->slot_p = E->pend_p + R;
     void* dp = nullptr;
     e = hipHostGetDevicePointer(&dp, E->pin, 0);
     E->hh_dev = static_cast<double*>(dp);
@@ -556,6 +599,8 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   if (e == hipSuccess) e = hipEventCreateWithFlags(&E->done_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = E->alloc(&E->colj_d, R);
   if (e == hipSuccess) e = E->alloc(&E->pend_d, R);
+  if (e == hipSuccess) e = E->alloc(&E->slot_d, R);
+  if (e == hipSuccess) e = E->alloc(&E->flag_d, R);
   // on the engine's own stream: it does not wait for the null stream, so a
   // null-stream memset could still be running under the first step
   if (e == hipSuccess) e = hipMemsetAsync(E->Vb, 0, R * E->ncv1 * E->ld * sizeof(double), E->st);
@@ -569,6 +614,7 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   E->cols.assign(E->R, qsi_engine_s::Col());
   E->colj.assign(E->R, -1);
   E->pend.assign(E->R, 0);
+  E->slotcol.assign(E->R, 0);
   E->coeffs_h.assign(static_cast<size_t>(ns) * R, 0.0);
   E->args.sec[0].coeffs = op->n_slots > 0 ? E->coeffs : nullptr;
   *out = E;
@@ -623,33 +669,51 @@ int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   if (launched) *launched = 0;
   int rc = engine_admit(E);
   if (rc != QSI_OK) return rc;
+  // active columns take apply slots 0..nact in column order
   int nact = 0;
   for (int c = 0; c < E->R; ++c) {
     E->colj[c] = E->cols[c].active ? E->cols[c].j : -1;
-    nact += E->cols[c].active ? 1 : 0;
+    if (E->cols[c].active) E->slotcol[nact++] = c;
   }
   if (nact == 0) return QSI_OK;
   hipStream_t s = E->st;
   const int64_t D = E->D;
   const int R = E->R;
+  const int ns = E->op->n_slots;
+  // through pinned memory: an async copy from pageable memory is not ordered
+  // with the kernels of a non-blocking stream
   std::copy(E->colj.begin(), E->colj.end(), E->colj_p);
   std::copy(E->pend.begin(), E->pend.end(), E->pend_p);
+  std::copy(E->slotcol.begin(), E->slotcol.begin() + nact, E->slot_p);
+  for (int k = 0; k < ns; ++k)
+    for (int sl = 0; sl < nact; ++sl)
+      E->coeffs_p[k * nact + sl] = E->coeffs_h[static_cast<size_t>(k) * R + E->slotcol[sl]];
   QSI_CHECK_HIP(hipMemcpyAsync(E->colj_d, E->colj_p, R * sizeof(int), hipMemcpyHostToDevice, s));
   QSI_CHECK_HIP(hipMemcpyAsync(E->pend_d, E->pend_p, R * sizeof(int), hipMemcpyHostToDevice, s));
+  QSI_CHECK_HIP(hipMemcpyAsync(E->slot_d, E->slot_p, nact * sizeof(int), hipMemcpyHostToDevice, s));
+  if (ns > 0)
+    QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_p, static_cast<size_t>(ns) * nact * sizeof(double),
+                                 hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>((D + 63) / 64)), dim3(kSrvThreads), 0,
-                     s, E->Vb, E->ld, E->ncv1, E->colj_d, E->pend_d, E->invb, R, D, E->X);
+                     s, E->Vb, E->ld, E->ncv1, E->colj_d, E->pend_d, E->invb, E->slot_d, nact, D,
+                     E->X);
   QSI_CHECK_LAUNCH();
-  rc = launch_blk_apply(E->args, E->op->basis->nw, E->n_blocks, E->X, E->Wc, nullptr, s);
+  // the block apply over the nact slots: X is D x nact row-major, slot sl's
+  // output is column sl of Wc
+  BlkArgs a = E->args;
+  a.R = nact;
+  int G = 64;
+  while (G / 2 >= nact && G > 1) G >>= 1;
+  a.G = G;
+  a.ldx = nact;
+  rc = launch_blk_apply(a, E->op->basis->nw, E->n_blocks, E->X, E->Wc, nullptr, s);
   if (rc != QSI_OK) return rc;
-  const dim3 gcol(E->nblk, R);
+  const dim3 gcol(E->nblk, nact);
   QSI_CHECK_HIP(hipMemsetAsync(E->hacc, 0, static_cast<size_t>(R) * kHMax * sizeof(double), s));
   if (E->mmax == 16) launch_reorth<16>(E, gcol, s);
   else if (E->mmax == 24) launch_reorth<24>(E, gcol, s);
   else if (E->mmax == 32) launch_reorth<32>(E, gcol, s);
   else launch_reorth<64>(E, gcol, s);
-  QSI_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_srv_beta, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d, E->beta,
-                     E->invb);
   QSI_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_srv_out, dim3((R * kHMax + 255) / 256), dim3(256), 0, s, E->hacc, E->beta,
                      R, E->hh_dev, E->bb_dev);

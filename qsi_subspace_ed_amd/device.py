@@ -94,6 +94,14 @@ class GpuBasis:
     def bucket_bits(self):
         return int(_lib.load().qsi_basis_bucket_bits(self._h))
 
+    def key_fields(self):
+        """Bucket key fields [(lo, width), ...], highest first."""
+        n = C.c_int(0)
+        lo = (C.c_int * 4)()
+        w = (C.c_int * 4)()
+        check(_lib.load().qsi_basis_key_fields(self._h, C.byref(n), lo, w), "qsi_basis_key_fields")
+        return [(lo[i], w[i]) for i in range(n.value)]
+
     def states(self):
         """(D, 2) uint64 [lo, hi] host copy of the states."""
         if self._states_host is None:
@@ -188,6 +196,14 @@ class GpuOp:
         per lane) or 'window' (LDS-staged bucket windows per wavefront)."""
         v = _lib.VARIANTS[variant]
         check(_lib.load().qsi_op_set_variant(self._h, v), "qsi_op_set_variant")
+
+    def set_cached_pipe(self, pipe):
+        """Pin the cached apply's gathers in flight per lane (4, 8, 16, 32)."""
+        check(_lib.load().qsi_op_set_cached_pipe(self._h, int(pipe)), "qsi_op_set_cached_pipe")
+
+    def cached_pipe(self):
+        """Gathers in flight per lane of the cached apply."""
+        return _lib.load().qsi_op_get_cached_pipe(self._h)
 
     # -- Eigen / Spectra glue
     def rows(self):

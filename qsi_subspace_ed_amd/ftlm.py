@@ -50,6 +50,19 @@ def lanczos_block(ops, V0s, M, coeffs=None, use_graph=True):
     return alpha.reshape(S, R, M), beta.reshape(S, R, M), ms.value
 
 
+def autotune(ops, R, reps=3):
+    """Choose the block-apply configuration of lanczos_block(ops, ...) with R
+    columns by timing every one on the multi-sector launch
+    (qsi_block_autotune).  Returns (configuration, ms per apply)."""
+    S = len(ops)
+    op_arr = (C.c_void_p * S)(*[o._h.value for o in ops])
+    best = C.c_int(-1)
+    ms = C.c_double(0.0)
+    check(_lib.load().qsi_block_autotune(op_arr, S, int(R), int(reps), C.byref(best),
+                                         C.byref(ms)), "qsi_block_autotune")
+    return best.value, ms.value
+
+
 def tridiag_eig(alpha, beta):
     """Ritz values and first-row eigenvector components of one T_M
     (diagonal alpha[0:M], off-diagonal beta[0:M-1])."""

@@ -128,6 +128,13 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
         y = torch.zeros_like(x)
         op.evaluate(x, y)
         ys.append(y)
+    # every gather depth of the cached kernel
+    for pipe in (4, 8, 16, 32):
+        op.set_cached_pipe(pipe)
+        assert op.cached_pipe() == pipe
+        y = torch.zeros_like(x)
+        op.evaluate(x, y)
+        ys.append(y)
     for y in ys[1:]:
         assert torch.equal(ys[0], y)
     # y += H x and y = 0.5 y + H x through the cached kernel match too
@@ -319,3 +326,67 @@ def test_autotune_keeps_results(lattices, cuda):
         Y1 = torch.zeros_like(X)
         op.apply_block(X, Y1, coeffs=co)
         assert torch.equal(Y0, Y1)
+
+
+def _expected_key_fields(ints, n_spins, K):
+    """The bucket key fields the library documents, from the sorted states as
+    Python integers: informative positions (where a neighbouring pair first
+    differs) from the top, runs as fields, at most 4 fields (the last one then
+    grows down over gaps), K bits in all."""
+    info = set()
+    for a, c in zip(ints[:-1], ints[1:]):
+        info.add((a ^ c).bit_length() - 1)
+    fields, width = [], 0
+    for p in range(n_spins - 1, -1, -1):
+        if width >= K:
+            break
+        if p not in info:
+            continue
+        if fields and fields[-1][0] == p + 1:
+            fields[-1] = (p, fields[-1][1] + 1)
+            width += 1
+        elif len(fields) < 4:
+            fields.append((p, 1))
+            width += 1
+        else:
+            grow = fields[-1][0] - p
+            if width + grow > K:
+                break
+            fields[-1] = (p, fields[-1][1] + grow)
+            width += grow
+    return fields
+
+
+@pytest.mark.parametrize("W,kbits", [(32, 0), (48, 0), (48, 7), (48, 30)])
+def test_bucket_key_fields(W, kbits, cuda):
+    """The device's choice of bucket key fields matches the rule recomputed on
+    the host, the key is non-decreasing along the basis, and every state is
+    found at its rank."""
+    b = load_basis(W)
+    basis = q.GpuBasis(b, W, bucket_bits=kbits)
+    ints = [int(v) for v in b[:, 0]]
+    K = basis.bucket_bits()
+    fields = basis.key_fields()
+    assert fields == _expected_key_fields(ints, W, K if kbits == 0 else kbits)
+    assert sum(w for _, w in fields) == K
+    keys = []
+    for s in ints:
+        k = 0
+        for lo, w in fields:
+            k = (k << w) | ((s >> lo) & ((1 << w) - 1))
+        keys.append(k)
+    assert all(a <= c for a, c in zip(keys[:-1], keys[1:]))
+    assert np.array_equal(basis.search(b), np.arange(b.shape[0]))
+
+
+def test_bucket_key_fields_two_words(lattices, cuda):
+    """Scattered informative positions over both words (48 spins on every
+    other bit of 20..114): four fields, the last grown over the gaps."""
+    f = lambda i: 2 * i + 20  # noqa: E731
+    b48 = load_basis(48)
+    ints = sorted(sum(1 << f(i) for i in range(48) if (int(lo) >> i) & 1) for lo in b48[:, 0])
+    basis = q.GpuBasis(q.as_lohi(ints), 115)
+    fields = basis.key_fields()
+    K = basis.bucket_bits()
+    assert len(fields) == 4 and fields == _expected_key_fields(ints, 115, K)
+    assert np.array_equal(basis.search(q.as_lohi(ints)), np.arange(len(ints)))

@@ -184,3 +184,22 @@ def test_lanczos_coeffs_match_oracle(lattices, cuda):
     w = np.linalg.eigvalsh(_dense(b, H))[0]
     T = np.diag(a) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
     assert abs(np.linalg.eigvalsh(T)[0] - w) <= 1e-9 * abs(w)
+
+
+def test_lanczos_coeffs_cached_forms_identical(lattices, cuda):
+    """The Lanczos step's fused apply + alpha partials through every gather
+    depth of the cached kernel gives the same coefficients, bit for bit, as
+    the searching kernel."""
+    jd = lattices[48]
+    b = load_basis(48)
+    H = q.build_hamiltonian_g(jd, [-0.9, -0.35, -0.6, -0.45])
+    q.add_local_field(H, jd, [0.05, -0.02, 0.04])
+    op = q.GpuOp(q.GpuBasis(b, 48), H)
+    op.set_variant("lane8")
+    a0, b0 = op.lanczos_coeffs(30, seed=5)
+    op.cache_connectivity(True)
+    op.set_variant("cached")
+    for pipe in (4, 8, 16, 32):
+        op.set_cached_pipe(pipe)
+        a1, b1 = op.lanczos_coeffs(30, seed=5)
+        assert np.array_equal(a0, a1) and np.array_equal(b0, b1), pipe
