@@ -13,6 +13,9 @@
 //   beta_c = |u_c| ; V_c[:, j_c+1] = u_c / beta_c
 // Active columns take apply slots 0..n_act in column order, so a step's
 // launches cost what its active columns need, not what the engine can hold.
+// Basis vectors are stored unnormalised with a scale per vector (vs): the
+// Lanczos vector v_i is vs_i V_c[:, i], and every kernel that reads V_c
+// applies the scales, so no pass rewrites a vector just to normalise it.
 // then the host updates each column's projected matrix and, when a column's
 // basis is full, runs its Rayleigh-Ritz step, convergence test and restart.
 // Requests join free columns between steps and leave when converged.  Every
@@ -35,42 +38,45 @@ namespace qsi {
 namespace {
 
 constexpr int kSrvThreads = 256;
-constexpr int kSrvRowsPerBlock = 4096;
+// rows per workgroup of the reorthogonalisation kernels: D / 256 rounded up
+// to a multiple of 64, between 256 and 2048 (small bases still get many
+// workgroups; the grid stays a few hundred workgroups per column)
+constexpr int kSrvRowsMin = 256;
+constexpr int kSrvRowsMax = 2048;
 constexpr int kHMax = 64;
 // partials slot holding |u|^2 (projections use slots 0..ncv-1 <= 62)
 constexpr int kNormSlot = kHMax - 1;
 // the second Gram-Schmidt pass runs when |h2|^2 > kReorthTol2 |u|^2
 constexpr double kReorthTol2 = 1e-24;
 
-// Start of a step, one block per 64-row tile: for every apply slot sl <
-// n_act (column c = slotcol[sl]), finish normalising V_c[j_c] (scale invb[c]
-// when pend[c]) and copy it to the row-major apply input X[row, sl].
+// Start of a step: for every apply slot sl < n_act (column c = slotcol[sl]),
+// the Lanczos vector vs_j V_c[j_c] into the row-major apply input
+// X[row * ldx + sl] (ldx >= n_act: rows padded to an aligned size; the
+// padding is not written).  Workgroups stride over 64-row tiles; a tile goes
+// through LDS so that both the column reads and the row writes coalesce.
 __global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
-    double* __restrict__ Vb, int64_t ld, int ncv1, const int* __restrict__ colj,
-    const int* __restrict__ pend, const double* __restrict__ invb,
-    const int* __restrict__ slotcol, int n_act, int64_t D, double* __restrict__ X) {
+    const double* __restrict__ Vb, int64_t ld, int ncv1, const int* __restrict__ colj,
+    const double* __restrict__ vs, const int* __restrict__ slotcol, int n_act, int ldx,
+    int64_t D, double* __restrict__ X) {
   __shared__ double tile[64][65];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
-  const int64_t r = r0 + lane;
-  for (int sl = wid; sl < n_act; sl += kSrvThreads / 64) {
-    const int c = slotcol[sl];
-    const int j = colj[c];
-    double v = 0.0;
-    if (r < D) {
-      double* vp = Vb + (static_cast<int64_t>(c) * ncv1 + j) * ld + r;
-      v = *vp;
-      if (pend[c]) {
-        v *= invb[c];
-        *vp = v;
-      }
+  const int64_t n_tiles = (D + 63) / 64;
+  for (int64_t tl = blockIdx.x; tl < n_tiles; tl += gridDim.x) {
+    const int64_t r0 = tl * 64;
+    const int64_t r = r0 + lane;
+    for (int sl = wid; sl < n_act; sl += kSrvThreads / 64) {
+      const int c = slotcol[sl];
+      const int j = colj[c];
+      double v = 0.0;
+      if (r < D) v = Vb[(static_cast<int64_t>(c) * ncv1 + j) * ld + r] * vs[c * kHMax + j];
+      tile[lane][sl] = v;
     }
-    tile[lane][sl] = v;
+    __syncthreads();
+    const int64_t rows = min<int64_t>(64, D - r0);
+    for (int t = threadIdx.x; t < rows * n_act; t += kSrvThreads)
+      X[(r0 + t / n_act) * ldx + t % n_act] = tile[t / n_act][t % n_act];
+    __syncthreads();
   }
-  __syncthreads();
-  const int64_t rows = min<int64_t>(64, D - r0);
-  for (int t = threadIdx.x; t < rows * n_act; t += kSrvThreads)
-    X[r0 * n_act + t] = tile[t / n_act][t % n_act];
 }
 
 // Per-block partial dot products of column c's w with its basis vectors
@@ -101,7 +107,7 @@ __device__ __forceinline__ void srv_block_dots(double (&acc)[MMAX], int m, int c
 template <int MMAX>
 __global__ __launch_bounds__(kSrvThreads) void k_srv_mdot(
     const double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ Wc,
-    const int* __restrict__ colj, const int* __restrict__ slotcol, int64_t D,
+    const int* __restrict__ colj, const int* __restrict__ slotcol, int64_t D, int rpb,
     double* __restrict__ partials) {
   const int sl = blockIdx.y;
   const int c = slotcol[sl];
@@ -109,16 +115,25 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_mdot(
   const int m = j + 1;
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
   const double* w = Wc + static_cast<int64_t>(sl) * ld;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
-  const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rpb;
+  const int64_t r1 = min<int64_t>(r0 + rpb, D);
   double acc[MMAX];
 #pragma unroll
   for (int i = 0; i < MMAX; ++i) acc[i] = 0.0;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kSrvThreads) {
     const double wr = w[r];
+    if constexpr (MMAX <= 32) {
+      // all loads first, then unpredicated updates (entries i >= m are zero)
+      double v[MMAX];
 #pragma unroll
-    for (int i = 0; i < MMAX; ++i)
-      if (i < m) acc[i] += V[i * ld + r] * wr;
+      for (int i = 0; i < MMAX; ++i) v[i] = i < m ? V[i * ld + r] : 0.0;
+#pragma unroll
+      for (int i = 0; i < MMAX; ++i) acc[i] += v[i] * wr;
+    } else {
+#pragma unroll
+      for (int i = 0; i < MMAX; ++i)
+        if (i < m) acc[i] += V[i * ld + r] * wr;
+    }
   }
   srv_block_dots<MMAX>(acc, m, c, gridDim.x, partials);
 }
@@ -131,8 +146,8 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_mdot(
 template <int MMAX>
 __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
     double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
-    const int* __restrict__ colj, const int* __restrict__ slotcol, int64_t D,
-    const double* __restrict__ Wc, double* __restrict__ partials) {
+    const double* __restrict__ vs, const int* __restrict__ colj, const int* __restrict__ slotcol,
+    int64_t D, int rpb, const double* __restrict__ Wc, double* __restrict__ partials) {
   constexpr int NWV = kSrvThreads / 64;
   constexpr int Q = (MMAX + NWV - 1) / NWV;
   const int sl = blockIdx.y;
@@ -144,13 +159,13 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
   __shared__ double red[MMAX];
   __shared__ double nsh;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x];
+  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x] * vs[c * kHMax + threadIdx.x];
   __syncthreads();
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
   double* out = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
   const double* w = Wc + static_cast<int64_t>(sl) * ld;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
-  const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rpb;
+  const int64_t r1 = min<int64_t>(r0 + rpb, D);
   double acc[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) acc[q] = 0.0;
@@ -199,13 +214,92 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
   if (threadIdx.x == 0) pp[kNormSlot] = nsh;
 }
 
+// Pass 1 -> 2 as k_srv_gemv_mdot for MMAX <= 24: one thread per row keeps
+// the row's m basis entries in registers for both the update and the dots
+// (no workgroup barrier per row tile).
+template <int MMAX>
+__global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot_rows(
+    double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
+    const double* __restrict__ vs, const int* __restrict__ colj, const int* __restrict__ slotcol,
+    int64_t D, int rpb, const double* __restrict__ Wc, double* __restrict__ partials) {
+  const int sl = blockIdx.y;
+  const int c = slotcol[sl];
+  const int m = colj[c] + 1;
+  // h in registers (uniform); entries i >= m are zero, so the loops below
+  // run unpredicated
+  double hr[MMAX];
+#pragma unroll
+  for (int i = 0; i < MMAX; ++i) hr[i] = i < m ? h[c * kHMax + i] * vs[c * kHMax + i] : 0.0;
+  const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
+  double* out = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
+  const double* w = Wc + static_cast<int64_t>(sl) * ld;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rpb;
+  const int64_t r1 = min<int64_t>(r0 + rpb, D);
+  double acc[MMAX];
+#pragma unroll
+  for (int i = 0; i < MMAX; ++i) acc[i] = 0.0;
+  double nrm = 0.0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kSrvThreads) {
+    double v[MMAX];
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) v[i] = i < m ? V[i * ld + r] : 0.0;
+    const double wr = w[r];
+    double a = 0.0;
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) a += v[i] * hr[i];
+    const double ur = wr - a;
+    out[r] = ur;
+    nrm += ur * ur;
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) acc[i] += v[i] * ur;
+  }
+  srv_block_dots<MMAX>(acc, m, c, gridDim.x, partials);
+  nrm = block_sum<kSrvThreads>(nrm);
+  if (threadIdx.x == 0)
+    partials[(static_cast<int64_t>(c) * gridDim.x + blockIdx.x) * kHMax + kNormSlot] = nrm;
+}
+
+// Thick restart of one column, in place: row by row, V[0..kn) = v[0..m) Y
+// (Y column-major m x kn; v_i = vsc_i V[i]) and V[kn] = V[m] (the residual
+// direction, unscaled).  Every thread reads its row's m + 1 entries before
+// writing any of them.
+template <int MMAX>
+__global__ __launch_bounds__(kSrvThreads) void k_srv_restart(double* __restrict__ V, int64_t ld,
+                                                              int m, int kn,
+                                                              const double* __restrict__ Y,
+                                                              const double* __restrict__ vsc,
+                                                              int64_t D) {
+  __shared__ double ys[MMAX * MMAX];
+  for (int t = threadIdx.x; t < m * kn; t += kSrvThreads) ys[t] = Y[t] * vsc[t % m];
+  __syncthreads();
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kSrvThreads + threadIdx.x; r < D;
+       r += static_cast<int64_t>(gridDim.x) * kSrvThreads) {
+    double v[MMAX + 1];
+#pragma unroll
+    for (int i = 0; i <= MMAX; ++i) v[i] = i <= m ? V[i * ld + r] : 0.0;
+    for (int k = 0; k < kn; ++k) {
+      double a = 0.0;
+#pragma unroll
+      for (int i = 0; i < MMAX; ++i)
+        if (i < m) a += v[i] * ys[k * m + i];
+      V[k * ld + r] = a;
+    }
+    double res = 0.0;
+#pragma unroll
+    for (int i = 0; i <= MMAX; ++i)
+      if (i == m) res = v[i];
+    V[kn * ld + r] = res;
+  }
+}
+
 // After pass 2's projections h2_c are summed: beta_c = |u_c| from the
-// kNormSlot partials, and flag_c = 1 when |h2_c|^2 > kReorthTol2 |u_c|^2 (u_c
-// needs the second Gram-Schmidt pass).  Grid R, one wave each.
+// kNormSlot partials, the scale 1/beta_c of the new vector u_c, and flag_c = 1
+// when |h2_c|^2 > kReorthTol2 |u_c|^2 (u_c needs the second Gram-Schmidt
+// pass).  Grid R, one wave each.
 __global__ void k_srv_check(const double* __restrict__ partials, int nblk,
                             const int* __restrict__ colj, const double* __restrict__ h,
                             int* __restrict__ flag, double* __restrict__ beta,
-                            double* __restrict__ invb) {
+                            double* __restrict__ vs) {
   const int c = blockIdx.x;
   const int j = colj[c];
   if (j < 0) return;
@@ -223,7 +317,7 @@ __global__ void k_srv_check(const double* __restrict__ partials, int nblk,
     flag[c] = s2 > kReorthTol2 * n2 ? 1 : 0;
     const double r = sqrt(n2);
     beta[c] = r;
-    invb[c] = r > 1e-300 ? 1.0 / r : 0.0;
+    vs[c * kHMax + j + 1] = r > 1e-300 ? 1.0 / r : 0.0;
   }
 }
 
@@ -232,18 +326,19 @@ __global__ void k_srv_check(const double* __restrict__ partials, int nblk,
 template <int MMAX>
 __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
     double* __restrict__ Vb, int64_t ld, int ncv1, const double* __restrict__ h,
-    const int* __restrict__ colj, const int* __restrict__ slotcol,
-    const int* __restrict__ flag, int64_t D, double* __restrict__ partials) {
+    const double* __restrict__ vs, const int* __restrict__ colj,
+    const int* __restrict__ slotcol, const int* __restrict__ flag, int64_t D, int rpb,
+    double* __restrict__ partials) {
   const int c = slotcol[blockIdx.y];
   if (!flag[c]) return;
   const int m = colj[c] + 1;
   __shared__ double hs[MMAX];
-  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x];
+  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x] * vs[c * kHMax + threadIdx.x];
   __syncthreads();
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
   double* u = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSrvRowsPerBlock;
-  const int64_t r1 = min<int64_t>(r0 + kSrvRowsPerBlock, D);
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rpb;
+  const int64_t r1 = min<int64_t>(r0 + rpb, D);
   double nrm = 0.0;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kSrvThreads) {
     double a = 0.0;
@@ -258,29 +353,29 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
   if (threadIdx.x == 0) partials[static_cast<int64_t>(c) * gridDim.x + blockIdx.x] = nrm;
 }
 
-// h[c*kHMax + i] = sum_b partials (fixed order); acc (optional) += h.
+// h[c*kHMax + i] = vs_i * sum_b partials (fixed order); acc (optional) += h.
 // Grid (R, kHMax), one wave each.
 __global__ void k_srv_fin(const double* __restrict__ partials, int nblk,
-                          const int* __restrict__ colj, double* __restrict__ h,
-                          double* __restrict__ acc) {
+                          const int* __restrict__ colj, const double* __restrict__ vs,
+                          double* __restrict__ h, double* __restrict__ acc) {
   const int c = blockIdx.x, i = blockIdx.y;
   const int j = colj[c];
   if (j < 0 || i > j) return;
   double v = 0.0;
   for (int b = threadIdx.x; b < nblk; b += 64)
     v += partials[(static_cast<int64_t>(c) * nblk + b) * kHMax + i];
-  v = wave_sum(v);
+  v = wave_sum(v) * vs[c * kHMax + i];
   if (threadIdx.x == 0) {
     h[c * kHMax + i] = v;
     if (acc) acc[c * kHMax + i] += v;
   }
 }
 
-// Flagged columns after pass 3: beta[c] = sqrt(sum), invb[c] = 1/beta (0 if
-// it underflows).  Grid R.
+// Flagged columns after pass 3: beta[c] = sqrt(sum) and the new vector's
+// scale 1/beta (0 if it underflows).  Grid R.
 __global__ void k_srv_beta(const double* __restrict__ partials, int nblk,
                            const int* __restrict__ colj, const int* __restrict__ flag,
-                           double* __restrict__ beta, double* __restrict__ invb) {
+                           double* __restrict__ beta, double* __restrict__ vs) {
   const int c = blockIdx.x;
   if (colj[c] < 0 || !flag[c]) return;
   double v = 0.0;
@@ -289,7 +384,7 @@ __global__ void k_srv_beta(const double* __restrict__ partials, int nblk,
   if (threadIdx.x == 0) {
     const double r = sqrt(v);
     beta[c] = r;
-    invb[c] = r > 1e-300 ? 1.0 / r : 0.0;
+    vs[c * kHMax + colj[c] + 1] = r > 1e-300 ? 1.0 / r : 0.0;
   }
 }
 
@@ -302,9 +397,19 @@ __global__ void k_srv_out(const double* __restrict__ hacc, const double* __restr
   if (t < R) bb[t] = beta[t];
 }
 
-// invb[c] = 1 / sqrt(*n2) (start vector normalisation).
-__global__ void k_srv_setinv(const double* __restrict__ n2, int c, double* __restrict__ invb) {
-  if (threadIdx.x == 0) invb[c] = 1.0 / sqrt(*n2);
+// vs[c][0] = 1 / sqrt(*n2) (start vector normalisation).
+__global__ void k_srv_setinv(const double* __restrict__ n2, int c, double* __restrict__ vs) {
+  if (threadIdx.x == 0) vs[c * kHMax] = 1.0 / sqrt(*n2);
+}
+
+// After column c's restart: the kept Ritz vectors are normalised and the
+// residual direction moved from index m to kn keeps its scale.
+__global__ void k_srv_rescale(double* __restrict__ vs, int c, int m, int kn) {
+  if (threadIdx.x == 0) {
+    const double sm = vs[c * kHMax + m];
+    for (int k = 0; k < kn; ++k) vs[c * kHMax + k] = 1.0;
+    vs[c * kHMax + kn] = sm;
+  }
 }
 
 }  // namespace
@@ -339,20 +444,20 @@ struct qsi_engine_s {
   double tol = 0.0;
   int64_t D = 0, ld = 0;
   int nblk = 0;
+  int rpb = 0;  // rows per workgroup of the reorthogonalisation kernels
   hipStream_t st = nullptr;
   // device: Krylov bases (column c: ncv + 1 vectors of stride ld), the
   // row-major apply input X (D x R), the apply output w (column-major),
   // slot coefficients, projections, norms and reduction partials
   double *Vb = nullptr, *X = nullptr, *Wc = nullptr, *coeffs = nullptr;
-  double *h = nullptr, *hacc = nullptr, *beta = nullptr, *invb = nullptr, *part = nullptr;
-  double *tmp = nullptr, *Ydev = nullptr;
-  int *colj_d = nullptr, *pend_d = nullptr, *slot_d = nullptr, *flag_d = nullptr;
+  double *h = nullptr, *hacc = nullptr, *beta = nullptr, *vs = nullptr, *part = nullptr;
+  double* Ydev = nullptr;
+  int *colj_d = nullptr, *slot_d = nullptr, *flag_d = nullptr;
   std::vector<void*> allocs;
   // pinned staging of the restarts' Ritz coefficients, kHMax^2 per column
   double* Ystage = nullptr;
   // pinned per-step exchange: column state down, projections and norms up
   int* colj_p = nullptr;
-  int* pend_p = nullptr;
   int* slot_p = nullptr;  // slotcol: apply slot -> column
   double* coeffs_p = nullptr;
   double* hh_p = nullptr;
@@ -368,7 +473,7 @@ struct qsi_engine_s {
   std::vector<Restart> restarts;
   // host
   std::vector<Col> cols;
-  std::vector<int> colj, pend, slotcol;
+  std::vector<int> colj, slotcol;
   std::vector<double> coeffs_h;
   std::deque<Request> queue;
   std::deque<Result> done;
@@ -399,8 +504,7 @@ struct qsi_engine_s {
 namespace {
 
 // Free columns take queued requests: coefficients (uploaded per step in slot
-// order), a seeded Gaussian start vector, and its normalisation left pending
-// for the next prep.
+// order), a seeded Gaussian start vector and its scale 1/|v0|.
 int engine_admit(qsi_engine_s* E) {
   for (int c = 0; c < E->R && !E->queue.empty(); ++c) {
     auto& col = E->cols[c];
@@ -412,13 +516,12 @@ int engine_admit(qsi_engine_s* E) {
     col.req = rq.id;
     col.T.assign(static_cast<size_t>(E->ncv) * E->ncv, 0.0);
     for (int s = 0; s < E->op->n_slots; ++s) E->coeffs_h[s * E->R + c] = rq.coeffs[s];
-    E->pend[c] = 1;
     double* v0 = E->Vb + static_cast<int64_t>(c) * E->ncv1 * E->ld;
     int rc = launch_randn(v0, E->D, rq.seed, E->st);
     if (rc != QSI_OK) return rc;
     rc = launch_dot(v0, v0, E->D, E->part + 8, E->part, E->st);
     if (rc != QSI_OK) return rc;
-    hipLaunchKernelGGL(k_srv_setinv, dim3(1), dim3(64), 0, E->st, E->part, c, E->invb);
+    hipLaunchKernelGGL(k_srv_setinv, dim3(1), dim3(64), 0, E->st, E->part, c, E->vs);
     QSI_CHECK_LAUNCH();
   }
   return QSI_OK;
@@ -451,7 +554,6 @@ int engine_ritz(qsi_engine_s* E, int c, int m, bool exhausted) {
     r.steps = col.steps;
     E->done.push_back(std::move(r));
     col.active = false;
-    E->pend[c] = 0;
     return QSI_OK;
   }
   // keep the kn lowest Ritz vectors, then the residual direction; the
@@ -473,20 +575,30 @@ int engine_ritz(qsi_engine_s* E, int c, int m, bool exhausted) {
 // The staging buffer is rewritten only after the next step synchronises.
 int engine_restarts(qsi_engine_s* E) {
   const size_t per = static_cast<size_t>(kHMax) * kHMax;
+  const uint32_t grid = static_cast<uint32_t>(
+      std::min<int64_t>((E->D + kSrvThreads - 1) / kSrvThreads, 4096));
   for (const auto& r : E->restarts) {
     double* V = E->Vb + static_cast<int64_t>(r.c) * E->ncv1 * E->ld;
     double* Yd = E->Ydev + r.c * per;
+    const double* vsc = E->vs + r.c * kHMax;
     QSI_CHECK_HIP(hipMemcpyAsync(Yd, E->Ystage + r.c * per,
                                  static_cast<size_t>(r.kn) * r.m * sizeof(double),
                                  hipMemcpyHostToDevice, E->st));
-    int rc = launch_combine(V, E->ld, r.m, Yd, r.kn, E->tmp, E->ld, E->D, E->st);
-    if (rc != QSI_OK) return rc;
-    QSI_CHECK_HIP(hipMemcpyAsync(V, E->tmp, static_cast<size_t>(r.kn) * E->ld * sizeof(double),
-                                 hipMemcpyDeviceToDevice, E->st));
-    QSI_CHECK_HIP(hipMemcpyAsync(V + static_cast<int64_t>(r.kn) * E->ld,
-                                 V + static_cast<int64_t>(r.m) * E->ld,
-                                 static_cast<size_t>(E->D) * sizeof(double),
-                                 hipMemcpyDeviceToDevice, E->st));
+    if (E->mmax == 16)
+      hipLaunchKernelGGL(k_srv_restart<16>, dim3(grid), dim3(kSrvThreads), 0, E->st, V, E->ld,
+                         r.m, r.kn, Yd, vsc, E->D);
+    else if (E->mmax == 24)
+      hipLaunchKernelGGL(k_srv_restart<24>, dim3(grid), dim3(kSrvThreads), 0, E->st, V, E->ld,
+                         r.m, r.kn, Yd, vsc, E->D);
+    else if (E->mmax == 32)
+      hipLaunchKernelGGL(k_srv_restart<32>, dim3(grid), dim3(kSrvThreads), 0, E->st, V, E->ld,
+                         r.m, r.kn, Yd, vsc, E->D);
+    else
+      hipLaunchKernelGGL(k_srv_restart<64>, dim3(grid), dim3(kSrvThreads), 0, E->st, V, E->ld,
+                         r.m, r.kn, Yd, vsc, E->D);
+    QSI_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_srv_rescale, dim3(1), dim3(64), 0, E->st, E->vs, r.c, r.m, r.kn);
+    QSI_CHECK_LAUNCH();
   }
   E->restarts.clear();
   return QSI_OK;
@@ -496,19 +608,26 @@ template <int MMAX>
 void launch_reorth(qsi_engine_s* E, dim3 gcol, hipStream_t s) {
   const int R = E->R;
   hipLaunchKernelGGL(k_srv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld, E->ncv1,
-                     E->Wc, E->colj_d, E->slot_d, E->D, E->part);
+                     E->Wc, E->colj_d, E->slot_d, E->D, E->rpb, E->part);
   hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
-                     E->h, E->hacc);
-  hipLaunchKernelGGL(k_srv_gemv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
-                     E->ncv1, E->h, E->colj_d, E->slot_d, E->D, E->Wc, E->part);
+                     E->vs, E->h, E->hacc);
+  if constexpr (MMAX <= 24)
+    hipLaunchKernelGGL(k_srv_gemv_mdot_rows<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
+                       E->ncv1, E->h, E->vs, E->colj_d, E->slot_d, E->D, E->rpb, E->Wc,
+                       E->part);
+  else
+    hipLaunchKernelGGL(k_srv_gemv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
+                       E->ncv1, E->h, E->vs, E->colj_d, E->slot_d, E->D, E->rpb, E->Wc,
+                       E->part);
   hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
-                     E->h, E->hacc);
+                     E->vs, E->h, E->hacc);
   hipLaunchKernelGGL(k_srv_check, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d, E->h,
-                     E->flag_d, E->beta, E->invb);
+                     E->flag_d, E->beta, E->vs);
   hipLaunchKernelGGL(k_srv_gemv_norm<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
-                     E->ncv1, E->h, E->colj_d, E->slot_d, E->flag_d, E->D, E->part);
+                     E->ncv1, E->h, E->vs, E->colj_d, E->slot_d, E->flag_d, E->D, E->rpb,
+                     E->part);
   hipLaunchKernelGGL(k_srv_beta, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
-                     E->flag_d, E->beta, E->invb);
+                     E->flag_d, E->beta, E->vs);
 }
 
 }  // namespace
@@ -537,7 +656,12 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   E->tol = tol;
   E->max_restarts = max_restarts;
   E->ld = (E->D + 63) / 64 * 64;
-  E->nblk = static_cast<int>((E->D + kSrvRowsPerBlock - 1) / kSrvRowsPerBlock);
+  {
+    int64_t rpb = ((E->D + 255) / 256 + 63) / 64 * 64;
+    rpb = std::min<int64_t>(std::max<int64_t>(rpb, kSrvRowsMin), kSrvRowsMax);
+    E->rpb = static_cast<int>(rpb);
+    E->nblk = static_cast<int>((E->D + rpb - 1) / rpb);
+  }
   BlkArgs& a = E->args;
   a.n_sec = 1;
   a.R = E->R;
@@ -569,12 +693,19 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   if (e == hipSuccess) e = E->alloc(&E->Vb, R * E->ncv1 * E->ld);
   if (e == hipSuccess) e = E->alloc(&E->X, static_cast<size_t>(E->D) * R);
   if (e == hipSuccess) e = E->alloc(&E->Wc, R * E->ld);
-  if (e == hipSuccess) e = E->alloc(&E->tmp, static_cast<size_t>(E->ncv1) * E->ld);
-  if (e == hipSuccess) e = E->alloc(&E->coeffs, static_cast<size_t>(ns) * R);
+  // per-step state sent down in one copy: coeffs (ns x R doubles), then colj,
+  // slotcol (R ints each), the same layout as the pinned block
+  double* down = nullptr;
+  if (e == hipSuccess) e = E->alloc(&down, static_cast<size_t>(ns) * R + R);
+  if (e == hipSuccess) {
+    E->coeffs = down;
+    E->colj_d = reinterpret_cast<int*>(down + static_cast<size_t>(ns) * R);
+    E->slot_d = E->colj_d + R;
+  }
   if (e == hipSuccess) e = E->alloc(&E->h, R * kHMax);
   if (e == hipSuccess) e = E->alloc(&E->hacc, R * kHMax);
   if (e == hipSuccess) e = E->alloc(&E->beta, R);
-  if (e == hipSuccess) e = E->alloc(&E->invb, R);
+  if (e == hipSuccess) e = E->alloc(&E->vs, R * kHMax);
   if (e == hipSuccess)
     e = E->alloc(&E->part, std::max<size_t>(R * E->nblk * kHMax, static_cast<size_t>(max_vec_partials()) + 8));
   if (e == hipSuccess) e = E->alloc(&E->Ydev, R * kHMax * kHMax);
@@ -589,22 +720,18 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
     E->bb_p = E->hh_p + R * kHMax;
     E->coeffs_p = E->bb_p + R;
     E->colj_p = reinterpret_cast<int*>(E->coeffs_p + R * ns);
-    E->pend_p = E->colj_p + R;
-    E->slot_p = E->pend_p + R;
+    E->slot_p = E->colj_p + R;
     void* dp = nullptr;
     e = hipHostGetDevicePointer(&dp, E->pin, 0);
     E->hh_dev = static_cast<double*>(dp);
     E->bb_dev = E->hh_dev + R * kHMax;
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&E->done_ev, hipEventDisableTiming);
-  if (e == hipSuccess) e = E->alloc(&E->colj_d, R);
-  if (e == hipSuccess) e = E->alloc(&E->pend_d, R);
-  if (e == hipSuccess) e = E->alloc(&E->slot_d, R);
   if (e == hipSuccess) e = E->alloc(&E->flag_d, R);
   // on the engine's own stream: it does not wait for the null stream, so a
   // null-stream memset could still be running under the first step
   if (e == hipSuccess) e = hipMemsetAsync(E->Vb, 0, R * E->ncv1 * E->ld * sizeof(double), E->st);
-  if (e == hipSuccess) e = hipMemsetAsync(E->invb, 0, R * sizeof(double), E->st);
+  if (e == hipSuccess) e = hipMemsetAsync(E->vs, 0, R * kHMax * sizeof(double), E->st);
   if (e == hipSuccess) e = hipStreamSynchronize(E->st);
   if (e != hipSuccess) {
     set_error(std::string("qsi_engine_create: ") + hipGetErrorString(e));
@@ -613,7 +740,6 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   }
   E->cols.assign(E->R, qsi_engine_s::Col());
   E->colj.assign(E->R, -1);
-  E->pend.assign(E->R, 0);
   E->slotcol.assign(E->R, 0);
   E->coeffs_h.assign(static_cast<size_t>(ns) * R, 0.0);
   E->args.sec[0].coeffs = op->n_slots > 0 ? E->coeffs : nullptr;
@@ -630,11 +756,11 @@ int qsi_engine_destroy(qsi_engine_t E) {
 }
 
 // Bytes of device memory an engine with these parameters allocates (the
-// Krylov bases, X, w and the restart scratch; the small arrays aside).
+// Krylov bases, X and w; the small arrays aside).
 int64_t qsi_engine_footprint(int64_t dim, int max_cols, int ncv) {
   const int64_t ld = (dim + 63) / 64 * 64;
   const int64_t R = max_cols;
-  return 8 * (R * (ncv + 1) * ld + dim * R + R * ld + (ncv + 1) * ld);
+  return 8 * (R * (ncv + 1) * ld + dim * R + R * ld);
 }
 
 int qsi_engine_submit(qsi_engine_t E, const double* h_coeffs, uint64_t seed, int64_t* id) {
@@ -683,29 +809,32 @@ int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   // through pinned memory: an async copy from pageable memory is not ordered
   // with the kernels of a non-blocking stream
   std::copy(E->colj.begin(), E->colj.end(), E->colj_p);
-  std::copy(E->pend.begin(), E->pend.end(), E->pend_p);
   std::copy(E->slotcol.begin(), E->slotcol.begin() + nact, E->slot_p);
   for (int k = 0; k < ns; ++k)
     for (int sl = 0; sl < nact; ++sl)
       E->coeffs_p[k * nact + sl] = E->coeffs_h[static_cast<size_t>(k) * R + E->slotcol[sl]];
-  QSI_CHECK_HIP(hipMemcpyAsync(E->colj_d, E->colj_p, R * sizeof(int), hipMemcpyHostToDevice, s));
-  QSI_CHECK_HIP(hipMemcpyAsync(E->pend_d, E->pend_p, R * sizeof(int), hipMemcpyHostToDevice, s));
-  QSI_CHECK_HIP(hipMemcpyAsync(E->slot_d, E->slot_p, nact * sizeof(int), hipMemcpyHostToDevice, s));
-  if (ns > 0)
-    QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_p, static_cast<size_t>(ns) * nact * sizeof(double),
-                                 hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>((D + 63) / 64)), dim3(kSrvThreads), 0,
-                     s, E->Vb, E->ld, E->ncv1, E->colj_d, E->pend_d, E->invb, E->slot_d, nact, D,
-                     E->X);
+  QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_p,
+                               static_cast<size_t>(std::max(ns, 1)) * R * sizeof(double) +
+                                   2 * R * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+  // X rows padded to a power of two (one aligned 64- or 128-byte segment per
+  // gathered row for up to 16 columns) when the buffer allows
+  int ldx = 1;
+  while (ldx < nact) ldx <<= 1;
+  if (ldx > R) ldx = nact;
+  hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>(std::min<int64_t>((D + 63) / 64, 4096))),
+                     dim3(kSrvThreads), 0,
+                     s, E->Vb, E->ld, E->ncv1, E->colj_d, E->vs, E->slot_d, nact, ldx,
+                     D, E->X);
   QSI_CHECK_LAUNCH();
-  // the block apply over the nact slots: X is D x nact row-major, slot sl's
+  // the block apply over the nact slots: X is D x ldx row-major, slot sl's
   // output is column sl of Wc
   BlkArgs a = E->args;
   a.R = nact;
   int G = 64;
   while (G / 2 >= nact && G > 1) G >>= 1;
   a.G = G;
-  a.ldx = nact;
+  a.ldx = ldx;
   rc = launch_blk_apply(a, E->op->basis->nw, E->n_blocks, E->X, E->Wc, nullptr, s);
   if (rc != QSI_OK) return rc;
   const dim3 gcol(E->nblk, nact);
@@ -755,7 +884,6 @@ int qsi_engine_step_end(qsi_engine_t E, int* n_active) {
       const bool exhausted = beta <= 1e-13 * std::max(tn, 1e-300);
       col.beta_last = exhausted ? 0.0 : beta;
       col.j = j + 1;
-      E->pend[c] = 1;  // V_c[j + 1] waits for its 1/beta in the next prep
       if (col.j == ncv || exhausted) {
         const int rc = engine_ritz(E, c, col.j, exhausted);
         if (rc != QSI_OK) return rc;

@@ -51,8 +51,9 @@ def test_engine_footprint_formula():
     lib = _lib.load()
     for D, R, ncv in ((2970, 8, 24), (2756394, 16, 20), (1, 1, 2)):
         ld = (D + 63) // 64 * 64
-        # Krylov bases R x (ncv + 1) x ld, X (D x R), w (R x ld), restart scratch
-        want = 8 * (R * (ncv + 1) * ld + D * R + R * ld + (ncv + 1) * ld)
+        # Krylov bases R x (ncv + 1) x ld, X (D x R), w (R x ld); restarts are
+        # done in place
+        want = 8 * (R * (ncv + 1) * ld + D * R + R * ld)
         assert lib.qsi_engine_footprint(D, R, ncv) == want
 
 
