@@ -157,7 +157,7 @@ def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
         "batched_device_ms": dev_ms, "batched_device_ms_no_graph": dev_ms_nograph,
         "batched_wall_s": t_batched,
         "connectivity_cache_build_s": t_cache,
-        "autotuned_block_cfg": [{"cfg": c, "ms_per_apply": m} for c, m in tuned],
+        "autotuned_block_cfg": [{"cfg": c, "ms_per_step": m} for c, m in tuned],
         "batched_search_solves_per_s": n_solves / t_batched_search,
         "batched_search_device_ms": dev_ms_search,
         "plain_solves_per_s": n_solves / t_plain,

@@ -186,6 +186,12 @@ int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
  * bit-identical for all).  No counterpart in the reference. */
 int qsi_op_set_cached_pipe(qsi_op_t op, int pipe);
 int qsi_op_get_cached_pipe(qsi_op_t op);
+/* Block-apply configuration (bits 0-1: kernel 0-2, bit 2: half-size grid) of
+ * qsi_op_apply_block, qsi_lanczos_block (the first sector's) and engines
+ * created afterwards.  Chosen by qsi_op_autotune / qsi_block_autotune; set it
+ * to pin one.  No counterpart in the reference. */
+int qsi_op_set_block_cfg(qsi_op_t op, int cfg);
+int qsi_op_get_block_cfg(qsi_op_t op);
 
 /* Autotuner: times the apply kernels on this operator (reps calls each, after
  * a warm-up, with HIP events) and keeps the fastest.  R = 0 tunes the
@@ -199,11 +205,12 @@ int qsi_op_get_cached_pipe(qsi_op_t op);
  * the reference.  Synchronises. */
 int qsi_op_autotune(qsi_op_t op, int R, int reps, double *d_x, double *d_y,
                     int *best, double *ms);
-/* Autotuner for the multi-sector block apply of qsi_lanczos_block: times every
- * configuration on the launch over all n_sectors operators (rows
- * concatenated, R columns, scratch blocks allocated) and sets the chosen one
- * on every operator.  *best, *ms as for qsi_op_autotune.  No counterpart in
- * the reference.  Synchronises. */
+/* Autotuner for the multi-sector block apply of qsi_lanczos_block: times
+ * every configuration on a few steps of the block Lanczos recurrence over all
+ * n_sectors operators (R columns, Gaussian start blocks and slot coefficients
+ * allocated here; reps runs each after a warm-up) and sets the fastest on
+ * every operator.  *best receives it, *ms the time per recurrence step.  No
+ * counterpart in the reference.  Synchronises. */
 int qsi_block_autotune(const qsi_op_t *ops, int n_sectors, int R, int reps, int *best,
                        double *ms);
 

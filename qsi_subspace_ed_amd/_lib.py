@@ -103,6 +103,8 @@ SIGNATURES = {
     "qsi_op_get_variant": (C.c_int, [_P]),
     "qsi_op_set_cached_pipe": (C.c_int, [_P, C.c_int]),
     "qsi_op_get_cached_pipe": (C.c_int, [_P]),
+    "qsi_op_set_block_cfg": (C.c_int, [_P, C.c_int]),
+    "qsi_op_get_block_cfg": (C.c_int, [_P]),
     "qsi_op_apply": (C.c_int, [_P, _P, _P, C.c_int, _P]),
     "qsi_op_apply_host": (C.c_int, [_P, _P, _P, C.c_int]),
     "qsi_op_apply_block": (C.c_int, [_P, _P, _I64, _P, _I64, C.c_int, _P, C.c_int, _P]),

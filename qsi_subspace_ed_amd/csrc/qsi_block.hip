@@ -878,15 +878,16 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
 // (Gauss quadrature nodes and weights for FTLM).
 namespace {
 
-// Average milliseconds of `reps` calls of f() (after one warm-up), timed with
-// events on stream s.
+// Average milliseconds per call of f(), timed with events on stream s: after
+// one warm-up call, `reps` calls, repeated with more calls when those took
+// under 5 ms.
 template <typename F>
 int time_ms(F&& f, int reps, hipStream_t s, double* ms) {
   hipEvent_t e0, e1;
   QSI_CHECK_HIP(hipEventCreate(&e0));
   QSI_CHECK_HIP(hipEventCreate(&e1));
   int rc = f();
-  if (rc == QSI_OK) {
+  for (int pass = 0; pass < 2 && rc == QSI_OK; ++pass) {
     (void)hipEventRecord(e0, s);
     for (int r = 0; r < reps && rc == QSI_OK; ++r) rc = f();
     (void)hipEventRecord(e1, s);
@@ -894,6 +895,11 @@ int time_ms(F&& f, int reps, hipStream_t s, double* ms) {
     float t = 0.f;
     (void)hipEventElapsedTime(&t, e0, e1);
     *ms = t / reps;
+    if (pass == 0) {
+      const int want = *ms > 0.0 ? std::min(200, static_cast<int>(5.0 / *ms) + 1) : reps;
+      if (want <= reps) break;
+      reps = want;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -904,6 +910,7 @@ int time_ms(F&& f, int reps, hipStream_t s, double* ms) {
 // (rows x R) and returns the fastest.
 int tune_block(BlkArgs a, int nw, int reps, const double* X, double* Y, int* pick,
                double* pick_ms) {
+
   const bool cached = blk_cached(a);
   const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
   *pick = -1;
@@ -938,29 +945,58 @@ extern "C" int qsi_block_autotune(const qsi_op_t* ops, int n_sectors, int R, int
     QSI_REQUIRE(ops[s]->basis->nw == nw, "qsi_block_autotune: sectors mix <=64 and >64 spins");
   }
   reps = std::max(1, reps);
+  // each configuration runs a few steps of the block Lanczos recurrence
+  // itself (the apply between the recurrence's vector kernels behaves
+  // differently from back-to-back applies), from Gaussian start blocks
+  constexpr int kSteps = 8;
   DevMem mem;
-  std::vector<const double*> cp(n_sectors, nullptr);
+  std::vector<const double*> v0(n_sectors, nullptr), cp(n_sectors, nullptr);
   for (int s = 0; s < n_sectors; ++s) {
+    double* v = nullptr;
+    const int64_t n = ops[s]->basis->dim * R;
+    QSI_CHECK_HIP(mem.alloc(&v, static_cast<size_t>(n)));
+    int rc = launch_randn(v, n, 12345 + s, nullptr);
+    if (rc != QSI_OK) return rc;
+    v0[s] = v;
     if (ops[s]->n_slots > 0) {
       double* c = nullptr;
       QSI_CHECK_HIP(mem.alloc(&c, static_cast<size_t>(ops[s]->n_slots) * R));
-      const int rc = launch_randn(c, static_cast<int64_t>(ops[s]->n_slots) * R, 7 + s, nullptr);
+      rc = launch_randn(c, static_cast<int64_t>(ops[s]->n_slots) * R, 7 + s, nullptr);
       if (rc != QSI_OK) return rc;
       cp[s] = c;
     }
   }
-  int64_t rows = 0;
-  BlkArgs a = sector_args(ops, n_sectors, R, cp.data(), &rows);
-  double *X = nullptr, *Y = nullptr;
-  QSI_CHECK_HIP(mem.alloc(&X, static_cast<size_t>(rows) * R));
-  QSI_CHECK_HIP(mem.alloc(&Y, static_cast<size_t>(rows) * R));
-  int rc = launch_randn(X, rows * R, 12345, nullptr);
-  if (rc != QSI_OK) return rc;
   QSI_CHECK_HIP(hipDeviceSynchronize());
+  std::vector<double> ha(static_cast<size_t>(n_sectors) * R * kSteps),
+      hb(static_cast<size_t>(n_sectors) * R * kSteps);
+  std::vector<int> saved(n_sectors);
+  for (int s = 0; s < n_sectors; ++s) saved[s] = ops[s]->blk_cfg;
+  int64_t rows = 0;
+  const bool cached = blk_cached(sector_args(ops, n_sectors, R, cp.data(), &rows));
+  const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
   int pick = -1;
   double pick_ms = 0.0;
-  rc = tune_block(a, nw, reps, X, Y, &pick, &pick_ms);
-  if (rc != QSI_OK) return rc;
+  for (int k : kernels) {
+    for (int half = 0; half < 2; ++half) {
+      for (int s = 0; s < n_sectors; ++s) ops[s]->blk_cfg = k | (half << 2);
+      double ms = 0.0;
+      for (int r = 0; r <= reps; ++r) {  // the first run warms up
+        double t = 0.0;
+        const int rc = qsi_lanczos_block(ops, n_sectors, R, v0.data(), cp.data(), kSteps,
+                                         ha.data(), hb.data(), 0, &t);
+        if (rc != QSI_OK) {
+          for (int q = 0; q < n_sectors; ++q) ops[q]->blk_cfg = saved[q];
+          return rc;
+        }
+        if (r > 0) ms += t / kSteps;
+      }
+      ms /= reps;
+      if (pick < 0 || ms < pick_ms) {
+        pick = k | (half << 2);
+        pick_ms = ms;
+      }
+    }
+  }
   for (int s = 0; s < n_sectors; ++s) ops[s]->blk_cfg = pick;
   if (best) *best = pick;
   if (best_ms) *best_ms = pick_ms;

@@ -205,6 +205,13 @@ class GpuOp:
         """Gathers in flight per lane of the cached apply."""
         return _lib.load().qsi_op_get_cached_pipe(self._h)
 
+    def set_block_cfg(self, cfg):
+        """Pin the block-apply configuration (kernel 0-2, + 4 for a half grid)."""
+        check(_lib.load().qsi_op_set_block_cfg(self._h, int(cfg)), "qsi_op_set_block_cfg")
+
+    def block_cfg(self):
+        return _lib.load().qsi_op_get_block_cfg(self._h)
+
     # -- Eigen / Spectra glue
     def rows(self):
         return self._n

@@ -52,8 +52,9 @@ def lanczos_block(ops, V0s, M, coeffs=None, use_graph=True):
 
 def autotune(ops, R, reps=3):
     """Choose the block-apply configuration of lanczos_block(ops, ...) with R
-    columns by timing every one on the multi-sector launch
-    (qsi_block_autotune).  Returns (configuration, ms per apply)."""
+    columns by timing every one on a few steps of the multi-sector block
+    Lanczos recurrence (qsi_block_autotune).  Returns (configuration, ms per
+    recurrence step)."""
     S = len(ops)
     op_arr = (C.c_void_p * S)(*[o._h.value for o in ops])
     best = C.c_int(-1)
