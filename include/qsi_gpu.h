@@ -77,9 +77,10 @@ int qsi_device_count(void);
  * Replaces ZBasisInterp / ZBasisBST (components/operator/include/operator.hpp:
  * 153-204; load_from_file + find_bounds, components/operator/src/operator.cpp:
  * 73-86,138-159).  The host array (D rows of [lo,hi], strictly increasing as
- * 128-bit integers) is copied to the device once; a top-bit bucket table
- * (2^K+1 uint32 ranks) is built on the device.  n_spins bounds the state bits
- * (1..128).  bucket_bits = 0 picks K automatically.  Synchronises. */
+ * 128-bit integers) is copied to the device once; a bucket table (2^K+1
+ * uint32 ranks over a K-bit key, see qsi_basis_key_fields) is built on the
+ * device.  n_spins bounds the state bits (1..128).  bucket_bits = 0 picks K
+ * automatically.  Synchronises. */
 int qsi_basis_create(const uint64_t *h_lohi, int64_t dim, int n_spins,
                      int bucket_bits, qsi_basis_t *out);
 /* As qsi_basis_create; with_hash = 1 also builds a device hash index
@@ -122,7 +123,7 @@ int qsi_op_n_diag(qsi_op_t op);
 /* Apply kernel choice (no counterpart in the reference; a tuning knob).
  * QSI_APPLY_HASH{,4}: each lane owns a row and finds its connected states in
  * the hash index, 1/4 lookups in flight per lane.  QSI_APPLY_LANE{,2,4,8}: the
- * same through the top-bit bucket table and a bisection of the bucket.
+ * same through the bucket table and a bisection of the bucket.
  * QSI_APPLY_WINDOW: per wavefront, one bucket lookup per term, then the
  * connected rows search a 64-state window staged in LDS.  QSI_APPLY_CACHED:
  * gathers through the cached connectivity (qsi_op_cache_connectivity; setting
@@ -301,8 +302,10 @@ int qsi_tridiag_eig(const double *d, const double *e, int n, double *evals,
                     double *first_row);
 
 /* ---- solve engine: continuous batching of eigensolves on one basis -------
- * Each request is one qsi_eigsh solve (lowest nev eigenvalues, thick-restart
- * Lanczos with full reorthogonalisation; counterpart of the reference's
+ * Each request is one qsi_eigsh-style solve (lowest nev eigenvalues,
+ * thick-restart Lanczos with full reorthogonalisation: one Gram-Schmidt pass
+ * against the column's basis, a second only when the first leaves
+ * projections above 1e-12 of the residual; counterpart of the reference's
  * mfsparse solve, include/matrix_diag_bits.hpp:51-95) with its own slot
  * coefficients, i.e. its own (g, B) point on a shared device-resident basis
  * and term list.  Up to max_cols requests run at once, one per column of a

@@ -60,8 +60,8 @@ def as_lohi(states):
 
 
 class GpuBasis:
-    """Sorted Z-basis resident in GPU memory with its top-bit bucket table
-    (replaces ZBasisInterp, components/operator/include/operator.hpp:197-204)."""
+    """Sorted Z-basis resident in GPU memory with its bucket table (replaces
+    ZBasisInterp, components/operator/include/operator.hpp:197-204)."""
 
     @classmethod
     def _adopt(cls, handle, n_spins):
