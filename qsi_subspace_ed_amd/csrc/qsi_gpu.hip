@@ -205,20 +205,21 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
             }
           }
         }
+        // the candidate's state and x are loaded together (lo < dim whenever
+        // the bucket is not empty); x counts only if the state matches
         double xj[G];
         bool hit[G];
+        State<NW> f[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          hit[g] = false;
           xj[g] = 0.0;
           if (n[g] == 1) {
-            const State<NW> f = load_state_cached<NW>(b.states, lo[g]);
-            hit[g] = st_eq<NW>(f, key[g]);
+            f[g] = load_state_cached<NW>(b.states, lo[g]);
+            xj[g] = x[lo[g]];
           }
         }
 #pragma unroll
-        for (int g = 0; g < G; ++g)
-          if (hit[g]) xj[g] = x[lo[g]];
+        for (int g = 0; g < G; ++g) hit[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if (hit[g] && fabs(xj[g]) > kApplyTol) {
@@ -1083,7 +1084,7 @@ int qsi_device_count(void) {
 static int auto_bucket_bits(int64_t dim, int n_spins) {
   int lg = 0;
   while ((int64_t{1} << lg) < dim) ++lg;
-  int k = lg + 4;
+  int k = lg + 5;  // 64 spins: 27 informative bits, ~6 states per bucket
   k = std::min(k, 28);
   k = std::min(k, n_spins);
   return std::max(k, 1);
