@@ -326,16 +326,23 @@ int qsi_engine_submit(qsi_engine_t e, const double *h_coeffs, uint64_t seed,
                       int64_t *id);
 int qsi_engine_active(qsi_engine_t e);  /* requests holding a column */
 int qsi_engine_queued(qsi_engine_t e);  /* requests waiting for one  */
-/* admit + one Lanczos step for every active request; synchronises */
+/* keeps up to two steps enqueued (admitting requests), then finishes the
+ * oldest: one Lanczos step for every active request; synchronises with it */
 int qsi_engine_step(qsi_engine_t e, int *n_active);
-/* The two halves of qsi_engine_step, so that several engines (each on its
- * own stream) overlap: step_begin admits and enqueues (*launched = 0 when no
- * request is active), step_end waits and does the host work. */
+/* The halves of qsi_engine_step, so that several engines (each on its own
+ * stream) overlap and the device never waits for the host: step_begin admits
+ * and enqueues a step (*launched = 0 when no request can step; at most two
+ * steps in flight, the second continuing from the first, a request whose
+ * Krylov basis the first fills sitting out until its restart), step_end waits
+ * for the oldest step and does its host work (projected matrices, Ritz
+ * values, restarts, finished requests). */
 int qsi_engine_step_begin(qsi_engine_t e, int *launched);
 int qsi_engine_step_end(qsi_engine_t e, int *n_active);
-/* 1 when step_end would not wait (no step enqueued, or it has completed),
- * 0 while the enqueued step runs; does not block */
+/* 1 when step_end would not wait (no step enqueued, or the oldest has
+ * completed), 0 while it runs; does not block */
 int qsi_engine_step_ready(qsi_engine_t e);
+/* steps enqueued and not yet finished by step_end (0, 1 or 2) */
+int qsi_engine_inflight(qsi_engine_t e);
 /* pop one finished request: returns 1 and fills id, h_evals (nev), nconv and
  * steps (Lanczos steps taken); returns 0 when none has finished */
 int qsi_engine_poll(qsi_engine_t e, int64_t *id, double *h_evals, int *nconv,

@@ -131,6 +131,7 @@ SIGNATURES = {
     "qsi_engine_step_begin": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "qsi_engine_step_end": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "qsi_engine_step_ready": (C.c_int, [_P]),
+    "qsi_engine_inflight": (C.c_int, [_P]),
     "qsi_engine_poll": (C.c_int, [_P, C.POINTER(_I64), _DP, C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),
     "qsi_vec_dot": (C.c_int, [_P, _P, _I64, _DP, _P]),

@@ -431,8 +431,9 @@ struct qsi_engine_s {
   };
   struct Col {
     bool active = false;
+    bool held = false;  // basis full: out of the steps until its restart
     int64_t req = -1;
-    int j = 0;        // index of the vector being expanded
+    int j = 0;        // index of the next vector to expand
     int k = 0;        // kept Ritz vectors after the last restart
     int restarts = 0;
     int steps = 0;
@@ -456,17 +457,24 @@ struct qsi_engine_s {
   std::vector<void*> allocs;
   // pinned staging of the restarts' Ritz coefficients, kHMax^2 per column
   double* Ystage = nullptr;
-  // pinned per-step exchange: column state down, projections and norms up
-  int* colj_p = nullptr;
-  int* slot_p = nullptr;  // slotcol: apply slot -> column
-  double* coeffs_p = nullptr;
-  double* hh_p = nullptr;
-  double* bb_p = nullptr;
+  // Up to two steps are in flight; each has its own pinned exchange (column
+  // state down, projections and norms up) and completion event.
+  struct StepBuf {
+    int* colj_p = nullptr;
+    int* slot_p = nullptr;  // slotcol: apply slot -> column
+    double* coeffs_p = nullptr;
+    double* hh_p = nullptr;
+    double* bb_p = nullptr;
+    double* hh_dev = nullptr;  // device addresses of hh_p / bb_p (mapped)
+    double* bb_dev = nullptr;
+    hipEvent_t ev = nullptr;   // recorded after the step's last command
+    std::vector<int> colj;     // j of each column in this step (-1: not in it)
+    std::vector<int64_t> req;  // request of each column when it was enqueued
+  };
+  StepBuf sb[2];
   void* pin = nullptr;
-  double* hh_dev = nullptr;  // device addresses of hh_p / bb_p (mapped)
-  double* bb_dev = nullptr;
-  hipEvent_t done_ev = nullptr;  // recorded after a step's last command
-  bool inflight = false;  // a step is enqueued and not yet finished
+  std::deque<int> inflight;  // StepBuf indices, oldest first
+  int next_buf = 0;
   struct Restart {
     int c, m, kn;
   };
@@ -486,7 +494,8 @@ struct qsi_engine_s {
     for (void* p : allocs) (void)hipFree(p);
     if (Ystage) (void)hipHostFree(Ystage);
     if (pin) (void)hipHostFree(pin);
-    if (done_ev) (void)hipEventDestroy(done_ev);
+    for (auto& b : sb)
+      if (b.ev) (void)hipEventDestroy(b.ev);
     if (st) (void)hipStreamDestroy(st);
   }
   template <typename T>
@@ -554,6 +563,7 @@ int engine_ritz(qsi_engine_s* E, int c, int m, bool exhausted) {
     r.steps = col.steps;
     E->done.push_back(std::move(r));
     col.active = false;
+    col.held = false;
     return QSI_OK;
   }
   // keep the kn lowest Ritz vectors, then the residual direction; the
@@ -567,6 +577,7 @@ int engine_ritz(qsi_engine_s* E, int c, int m, bool exhausted) {
   for (int i = 0; i < kn; ++i) col.T[static_cast<size_t>(i) * ncv + i] = theta[i];
   col.k = kn;
   col.j = kn;
+  col.held = false;
   col.restarts += 1;
   return QSI_OK;
 }
@@ -712,25 +723,34 @@ int qsi_engine_create(qsi_op_t op, int max_cols, int nev, int ncv, double tol,
   if (e == hipSuccess)
     e = hipHostMalloc(reinterpret_cast<void**>(&E->Ystage), R * kHMax * kHMax * sizeof(double),
                       hipHostMallocDefault);
+  // pinned: per step buffer hh (R x kHMax), bb (R), coeffs (ns x R), colj and
+  // slotcol (R ints each, in R more doubles)
+  const size_t per_buf = R * (kHMax + 2 + ns);
   if (e == hipSuccess)
-    e = hipHostMalloc(&E->pin, R * (kHMax + 3 + ns) * sizeof(double),
+    e = hipHostMalloc(&E->pin, 2 * per_buf * sizeof(double),
                       hipHostMallocMapped | hipHostMallocCoherent);
-  if (e == hipSuccess) {
-    E->hh_p = static_cast<double*>(E->pin);
-    E->bb_p = E->hh_p + R * kHMax;
-    E->coeffs_p = E->bb_p + R;
-    E->colj_p = reinterpret_cast<int*>(E->coeffs_p + R * ns);
-    E->slot_p = E->colj_p + R;
-    void* dp = nullptr;
-    e = hipHostGetDevicePointer(&dp, E->pin, 0);
-    E->hh_dev = static_cast<double*>(dp);
-    E->bb_dev = E->hh_dev + R * kHMax;
+  void* dp = nullptr;
+  if (e == hipSuccess) e = hipHostGetDevicePointer(&dp, E->pin, 0);
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    auto& B = E->sb[b];
+    B.hh_p = static_cast<double*>(E->pin) + b * per_buf;
+    B.bb_p = B.hh_p + R * kHMax;
+    B.coeffs_p = B.bb_p + R;
+    B.colj_p = reinterpret_cast<int*>(B.coeffs_p + R * ns);
+    B.slot_p = B.colj_p + R;
+    B.hh_dev = static_cast<double*>(dp) + b * per_buf;
+    B.bb_dev = B.hh_dev + R * kHMax;
+    B.colj.assign(R, -1);
+    B.req.assign(R, -1);
+    e = hipEventCreateWithFlags(&B.ev, hipEventDisableTiming);
   }
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&E->done_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = E->alloc(&E->flag_d, R);
   // on the engine's own stream: it does not wait for the null stream, so a
   // null-stream memset could still be running under the first step
   if (e == hipSuccess) e = hipMemsetAsync(E->Vb, 0, R * E->ncv1 * E->ld * sizeof(double), E->st);
+  // first touch of the step buffers here rather than in the first full step
+  if (e == hipSuccess) e = hipMemsetAsync(E->X, 0, static_cast<size_t>(E->D) * R * sizeof(double), E->st);
+  if (e == hipSuccess) e = hipMemsetAsync(E->Wc, 0, R * E->ld * sizeof(double), E->st);
   if (e == hipSuccess) e = hipMemsetAsync(E->vs, 0, R * kHMax * sizeof(double), E->st);
   if (e == hipSuccess) e = hipStreamSynchronize(E->st);
   if (e != hipSuccess) {
@@ -786,34 +806,44 @@ int qsi_engine_active(qsi_engine_t E) {
 int qsi_engine_queued(qsi_engine_t E) { return E ? static_cast<int>(E->queue.size()) : -1; }
 
 // First half of an engine step: admit queued requests into free columns and
-// enqueue one Lanczos step for every active column on the engine's stream.
-// *launched (optional) = 0 when no column is active (nothing enqueued).
-// Engines on different streams overlap between step_begin and step_end.
+// enqueue one Lanczos step for every active column whose basis is not full
+// on the engine's stream.  A second step may be enqueued before the first one
+// is finished (its columns continue from where the first leaves them; a
+// column whose basis the first fills waits for its restart).  *launched
+// (optional) = 0 when no column can step (nothing enqueued).  Engines on
+// different streams overlap between step_begin and step_end.
 int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   QSI_REQUIRE(E != nullptr, "qsi_engine_step_begin: null engine");
-  QSI_REQUIRE(!E->inflight, "qsi_engine_step_begin: the previous step is not finished");
+  QSI_REQUIRE(E->inflight.size() < 2, "qsi_engine_step_begin: two steps are already in flight");
   if (launched) *launched = 0;
   int rc = engine_admit(E);
   if (rc != QSI_OK) return rc;
-  // active columns take apply slots 0..nact in column order
+  // stepping columns take apply slots 0..nact in column order
   int nact = 0;
   for (int c = 0; c < E->R; ++c) {
-    E->colj[c] = E->cols[c].active ? E->cols[c].j : -1;
-    if (E->cols[c].active) E->slotcol[nact++] = c;
+    const auto& col = E->cols[c];
+    const bool in = col.active && !col.held;
+    E->colj[c] = in ? col.j : -1;
+    if (in) E->slotcol[nact++] = c;
   }
   if (nact == 0) return QSI_OK;
   hipStream_t s = E->st;
   const int64_t D = E->D;
   const int R = E->R;
   const int ns = E->op->n_slots;
+  const int bi = E->next_buf;
+  E->next_buf ^= 1;
+  auto& B = E->sb[bi];
   // through pinned memory: an async copy from pageable memory is not ordered
-  // with the kernels of a non-blocking stream
-  std::copy(E->colj.begin(), E->colj.end(), E->colj_p);
-  std::copy(E->slotcol.begin(), E->slotcol.begin() + nact, E->slot_p);
+  // with the kernels of a non-blocking stream.  The device copy is read by
+  // this step's kernels only: the previous step's kernels come first on the
+  // stream.
+  std::copy(E->colj.begin(), E->colj.end(), B.colj_p);
+  std::copy(E->slotcol.begin(), E->slotcol.begin() + nact, B.slot_p);
   for (int k = 0; k < ns; ++k)
     for (int sl = 0; sl < nact; ++sl)
-      E->coeffs_p[k * nact + sl] = E->coeffs_h[static_cast<size_t>(k) * R + E->slotcol[sl]];
-  QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, E->coeffs_p,
+      B.coeffs_p[k * nact + sl] = E->coeffs_h[static_cast<size_t>(k) * R + E->slotcol[sl]];
+  QSI_CHECK_HIP(hipMemcpyAsync(E->coeffs, B.coeffs_p,
                                static_cast<size_t>(std::max(ns, 1)) * R * sizeof(double) +
                                    2 * R * sizeof(int),
                                hipMemcpyHostToDevice, s));
@@ -845,33 +875,46 @@ int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   else launch_reorth<64>(E, gcol, s);
   QSI_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_srv_out, dim3((R * kHMax + 255) / 256), dim3(256), 0, s, E->hacc, E->beta,
-                     R, E->hh_dev, E->bb_dev);
+                     R, B.hh_dev, B.bb_dev);
   QSI_CHECK_LAUNCH();
-  QSI_CHECK_HIP(hipEventRecord(E->done_ev, s));
-  E->inflight = true;
+  QSI_CHECK_HIP(hipEventRecord(B.ev, s));
+  for (int c = 0; c < R; ++c) {
+    B.colj[c] = E->colj[c];
+    B.req[c] = E->cols[c].req;
+    if (E->colj[c] >= 0) {
+      auto& col = E->cols[c];
+      col.j += 1;
+      if (col.j == E->ncv) col.held = true;
+    }
+  }
+  E->inflight.push_back(bi);
   if (launched) *launched = 1;
   return QSI_OK;
 }
 
-// Second half: wait for the enqueued step, update every column's projected
-// matrix, run Rayleigh-Ritz / convergence / restart where a column's basis is
-// full, and enqueue the restarts.  *n_active (optional) = columns active
-// afterwards.  Without an enqueued step it only reports *n_active.
+// Second half: wait for the oldest enqueued step, update its columns'
+// projected matrices, run Rayleigh-Ritz / convergence / restart where a
+// column's basis is full or exhausted, and enqueue the restarts.  Results of
+// a column that finished in between (its request left) are dropped.
+// *n_active (optional) = columns active afterwards.  Without an enqueued step
+// it only reports *n_active.
 int qsi_engine_step_end(qsi_engine_t E, int* n_active) {
   QSI_REQUIRE(E != nullptr, "qsi_engine_step_end: null engine");
-  if (E->inflight) {
-    E->inflight = false;
-    QSI_CHECK_HIP(hipEventSynchronize(E->done_ev));
+  if (!E->inflight.empty()) {
+    const int bi = E->inflight.front();
+    E->inflight.pop_front();
+    auto& B = E->sb[bi];
+    QSI_CHECK_HIP(hipEventSynchronize(B.ev));
     const int ncv = E->ncv;
     for (int c = 0; c < E->R; ++c) {
       auto& col = E->cols[c];
-      if (!col.active) continue;
-      const int j = col.j;
+      const int j = B.colj[c];
+      if (j < 0 || !col.active || col.req != B.req[c]) continue;
       col.steps += 1;
       // projections onto the kept Ritz vectors only couple to the first new
       // vector (arrowhead); later ones are rounding noise
       for (int i = 0; i <= j; ++i) {
-        const double v = E->hh_p[static_cast<size_t>(c) * kHMax + i];
+        const double v = B.hh_p[static_cast<size_t>(c) * kHMax + i];
         if ((i < col.k && j == col.k) || i >= col.k) {
           col.T[static_cast<size_t>(j) * ncv + i] = v;
           col.T[static_cast<size_t>(i) * ncv + j] = v;
@@ -880,12 +923,11 @@ int qsi_engine_step_end(qsi_engine_t E, int* n_active) {
       double tn = 0.0;
       for (int i = 0; i <= j; ++i)
         tn = std::max(tn, std::fabs(col.T[static_cast<size_t>(j) * ncv + i]));
-      const double beta = E->bb_p[c];
+      const double beta = B.bb_p[c];
       const bool exhausted = beta <= 1e-13 * std::max(tn, 1e-300);
       col.beta_last = exhausted ? 0.0 : beta;
-      col.j = j + 1;
-      if (col.j == ncv || exhausted) {
-        const int rc = engine_ritz(E, c, col.j, exhausted);
+      if (j + 1 == ncv || exhausted) {
+        const int rc = engine_ritz(E, c, j + 1, exhausted);
         if (rc != QSI_OK) return rc;
       }
     }
@@ -898,22 +940,34 @@ int qsi_engine_step_end(qsi_engine_t E, int* n_active) {
   return QSI_OK;
 }
 
-// 1 when no step is enqueued or the enqueued one has completed (step_end
-// will not wait), 0 while it runs, negative on error.
+// 1 when no step is enqueued or the oldest enqueued one has completed
+// (step_end will not wait), 0 while it runs, negative on error.
 int qsi_engine_step_ready(qsi_engine_t E) {
   if (!E) return QSI_E_INVALID;
-  if (!E->inflight) return 1;
-  const hipError_t e = hipEventQuery(E->done_ev);
+  if (E->inflight.empty()) return 1;
+  const hipError_t e = hipEventQuery(E->sb[E->inflight.front()].ev);
   if (e == hipSuccess) return 1;
   if (e == hipErrorNotReady) return 0;
   set_error(std::string("qsi_engine_step_ready: ") + hipGetErrorString(e));
   return QSI_E_HIP;
 }
 
-// One engine step: step_begin then step_end.  Synchronises.
+// Steps enqueued and not yet finished by step_end (0, 1 or 2).
+int qsi_engine_inflight(qsi_engine_t E) {
+  return E ? static_cast<int>(E->inflight.size()) : -1;
+}
+
+// One engine step: keeps up to two steps enqueued (step_begin while fewer are
+// in flight and a column can step), then finishes the oldest (step_end).
+// Synchronises with that step.
 int qsi_engine_step(qsi_engine_t E, int* n_active) {
-  const int rc = qsi_engine_step_begin(E, nullptr);
-  if (rc != QSI_OK) return rc;
+  QSI_REQUIRE(E != nullptr, "qsi_engine_step: null engine");
+  while (E->inflight.size() < 2) {
+    int launched = 0;
+    const int rc = qsi_engine_step_begin(E, &launched);
+    if (rc != QSI_OK) return rc;
+    if (!launched) break;
+  }
   return qsi_engine_step_end(E, n_active);
 }
 

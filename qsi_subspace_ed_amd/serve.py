@@ -91,7 +91,6 @@ class Engine:
         self.op = op
         self.nev = nev
         self.max_cols = max_cols
-        self.inflight = False
 
     @staticmethod
     def footprint(dim, max_cols, ncv):
@@ -110,16 +109,22 @@ class Engine:
         return rid.value
 
     def step(self):
+        """Keep up to two steps enqueued and finish the oldest (qsi_engine_step)."""
         n = C.c_int()
         check(_lib.load().qsi_engine_step(self._h, C.byref(n)), "qsi_engine_step")
         return n.value
 
     def step_begin(self):
-        """Admit and enqueue one step; False when nothing is active."""
+        """Admit and enqueue one step (up to two may be in flight); False when
+        no column can step."""
         n = C.c_int()
         check(_lib.load().qsi_engine_step_begin(self._h, C.byref(n)), "qsi_engine_step_begin")
-        self.inflight = bool(n.value)
-        return self.inflight
+        return bool(n.value)
+
+    @property
+    def inflight(self):
+        """Steps enqueued and not yet finished (0, 1 or 2)."""
+        return int(_lib.load().qsi_engine_inflight(self._h))
 
     def ready(self):
         """True when step_end would not wait."""
@@ -129,8 +134,8 @@ class Engine:
         return rc == 1
 
     def step_end(self):
+        """Finish the oldest enqueued step."""
         n = C.c_int()
-        self.inflight = False
         check(_lib.load().qsi_engine_step_end(self._h, C.byref(n)), "qsi_engine_step_end")
         return n.value
 
@@ -152,7 +157,7 @@ class Engine:
         return int(_lib.load().qsi_engine_queued(self._h))
 
     def busy(self):
-        return self.inflight or self.active + self.queued > 0
+        return self.inflight > 0 or self.active + self.queued > 0
 
     def run_all(self):
         """Step until every submitted request finished; returns them."""
@@ -191,12 +196,14 @@ class Space:
         return int(self.states.shape[0])
 
     def footprint(self, max_cols, ncv):
-        """Device bytes when resident: basis words + bucket table (at most
-        2^28 + 1 entries, about 16 per state) + cached connectivity (5 bytes
-        per nonzero, at most n_spins / 4 nonzeros per row on pyrochlore ring
-        terms, + 8 per row) + terms + engine buffers."""
+        """Device bytes when resident: basis words + bucket table (2^K + 1
+        entries, K at most ceil(log2 D) + 5, 28 and n_spins) + cached
+        connectivity (5 bytes per nonzero, at most n_spins / 4 nonzeros per row
+        on pyrochlore ring terms, + 8 per row) + terms + engine buffers."""
         words = 1 if self.n_spins <= 64 else 2
-        basis = self.dim * 8 * words + 4 * min(16 * self.dim + 1, (1 << 28) + 1)
+        lg = max(0, int(self.dim - 1).bit_length())
+        kbits = max(1, min(lg + 5, 28, self.n_spins))
+        basis = self.dim * 8 * words + 4 * ((1 << kbits) + 1)
         conn = self.dim * (8 + 5 * max(1, self.n_spins // 4))
         terms = 1 << 20  # term tables are a few KB; a MiB covers them
         return basis + conn + terms + Engine.footprint(self.dim, max_cols, ncv)
@@ -326,18 +333,21 @@ class SolveServer:
         # each engine runs on its own stream at its own pace: finish a step
         # whose work has completed, then enqueue the next one; engines whose
         # step is still running are left alone (nothing here blocks)
-        for sp in self.spaces.values():
+        # largest basis first: its steps are the longest, so they are enqueued
+        # before the small engines' ones in each pass
+        for sp in sorted(self.spaces.values(), key=lambda x: -x.dim):
             eng = sp.engine
             if eng is None:
                 continue
-            if eng.inflight:
-                if not eng.ready():
-                    continue
+            # each engine keeps up to two steps enqueued: the next one runs
+            # while the host finishes the previous
+            if eng.inflight and eng.ready():
                 eng.step_end()
                 sp.last_used = time.perf_counter()
                 self.stats.steps += 1
-            if eng.busy():
-                eng.step_begin()
+            while eng.inflight < 2 and eng.active + eng.queued > 0:
+                if not eng.step_begin():
+                    break
             for f in eng.poll():
                 r = self.inflight.pop((sp.key, f.id))
                 r.result = f
