@@ -251,3 +251,35 @@ def test_engine_first_step_after_create(lattices, cuda):
         assert a.nconv == 3 and b2.nconv == 3
         assert a.steps > 1
         np.testing.assert_allclose(a.evals, b2.evals, rtol=1e-9, atol=0)
+
+
+def test_lookahead_same_as_one_step_at_a_time(lattices, cuda):
+    """Two steps in flight (qsi_engine_step keeps the next step enqueued while
+    the host finishes the previous; a request whose basis fills sits out until
+    its restart) give every request the same bits and step counts as strictly
+    one step at a time."""
+    jd = lattices[48]
+    b = q.enumerate_basis(jd, 0, n_threads=8)
+    op = q.GpuOp(q.GpuBasis(b, 48), serve.slotted_hamiltonian(jd))
+
+    def run(lookahead):
+        eng = serve.Engine(op, max_cols=4, nev=3, ncv=12, tol=1e-10)
+        for k, (g, B) in enumerate(POINTS):
+            eng.submit(serve.point_coeffs(g, _h_of(B)), seed=70 + k)
+        done = []
+        while eng.busy():
+            if lookahead:
+                eng.step()
+            else:
+                eng.step_begin()
+                assert eng.inflight <= 1
+                eng.step_end()
+            done += eng.poll()
+        eng.close()
+        return {f.id: f for f in done}
+
+    one, two = run(False), run(True)
+    assert one.keys() == two.keys() and len(one) == len(POINTS)
+    for i in one:
+        assert one[i].steps == two[i].steps, i
+        assert np.array_equal(one[i].evals, two[i].evals), i
