@@ -264,6 +264,9 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
   if (i < b.dim) {
     const double xi = x[i];
     double acc = y_in ? yscale * y_in[i] : 0.0;
+    // the row's state (for the diagonal) is loaded before the gathers so its
+    // latency overlaps theirs
+    const State<NW> si = load_state_cached<NW>(b.states, i);
     const uint64_t e0 = rowptr[i], e1 = rowptr[i + 1];
     for (uint64_t e = e0; e < e1; e += PIPE) {
       uint32_t J[PIPE];
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
         }
       }
     }
-    if (n_diag > 0) acc = diag_accumulate<NW>(acc, xi, load_state_cached<NW>(b.states, i), dg, n_diag);
+    if (n_diag > 0) acc = diag_accumulate<NW>(acc, xi, si, dg, n_diag);
     y[i] = acc;
     if constexpr (DOT) prod = xi * acc;
   }
