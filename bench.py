@@ -412,6 +412,7 @@ def bench_96(args):
         print(f"[bench] cached connectivity: {kc * 1e3:.1f} ms/apply (built in {t_build:.1f} s)",
               file=sys.stderr, flush=True)
         del yc
+        torch.cuda.empty_cache()  # the Lanczos vectors below are allocated by the library
     elif not args.no_cached:
         cached = {"skipped": f"needs {need / 1e9:.0f} GB, {free / 1e9:.0f} GB free"}
     lz = None

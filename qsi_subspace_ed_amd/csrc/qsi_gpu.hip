@@ -1088,7 +1088,7 @@ static int auto_bucket_bits(int64_t dim, int n_spins) {
   int lg = 0;
   while ((int64_t{1} << lg) < dim) ++lg;
   int k = lg + 5;  // 64 spins: 27 informative bits, ~6 states per bucket
-  k = std::min(k, 28);
+  k = std::min(k, 30);  // 96 spins: a 4 GB table, 6 % faster than 28 bits
   k = std::min(k, n_spins);
   return std::max(k, 1);
 }
