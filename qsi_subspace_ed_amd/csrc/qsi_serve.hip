@@ -58,7 +58,10 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
     const double* __restrict__ Vb, int64_t ld, int ncv1, const int* __restrict__ colj,
     const double* __restrict__ vs, const int* __restrict__ slotcol, int n_act, int ldx,
     int64_t D, double* __restrict__ X) {
-  __shared__ double tile[64][65];
+  // tile[row][sl] with row stride n_act + 1 (dynamic LDS sized to the step's
+  // columns, so more workgroups fit on a CU)
+  extern __shared__ double tile[];
+  const int ts = n_act + 1;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t n_tiles = (D + 63) / 64;
   for (int64_t tl = blockIdx.x; tl < n_tiles; tl += gridDim.x) {
@@ -69,12 +72,12 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
       const int j = colj[c];
       double v = 0.0;
       if (r < D) v = Vb[(static_cast<int64_t>(c) * ncv1 + j) * ld + r] * vs[c * kHMax + j];
-      tile[lane][sl] = v;
+      tile[lane * ts + sl] = v;
     }
     __syncthreads();
     const int64_t rows = min<int64_t>(64, D - r0);
     for (int t = threadIdx.x; t < rows * n_act; t += kSrvThreads)
-      X[(r0 + t / n_act) * ldx + t % n_act] = tile[t / n_act][t % n_act];
+      X[(r0 + t / n_act) * ldx + t % n_act] = tile[(t / n_act) * ts + t % n_act];
     __syncthreads();
   }
 }
@@ -852,8 +855,8 @@ int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   int ldx = 1;
   while (ldx < nact) ldx <<= 1;
   if (ldx > R) ldx = nact;
-  hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>(std::min<int64_t>((D + 63) / 64, 4096))),
-                     dim3(kSrvThreads), 0,
+  hipLaunchKernelGGL(k_srv_prep, dim3(static_cast<uint32_t>(std::min<int64_t>((D + 63) / 64, 8192))),
+                     dim3(kSrvThreads), static_cast<size_t>(64) * (nact + 1) * sizeof(double),
                      s, E->Vb, E->ld, E->ncv1, E->colj_d, E->vs, E->slot_d, nact, ldx,
                      D, E->X);
   QSI_CHECK_LAUNCH();
