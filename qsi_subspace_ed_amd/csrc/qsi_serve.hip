@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_prep(
 
 // Per-block partial dot products of column c's w with its basis vectors
 // V_c[0..m), m = j_c + 1 <= MMAX, reduced over the block's rows into
-// partials[(c*nblk + b)*kHMax + i].
+// partials[(c*kHMax + i)*nblk + b] (per projection, contiguous over blocks).
 template <int MMAX>
 __device__ __forceinline__ void srv_block_dots(double (&acc)[MMAX], int m, int c, int nblk,
                                                double* __restrict__ partials) {
@@ -101,7 +101,7 @@ __device__ __forceinline__ void srv_block_dots(double (&acc)[MMAX], int m, int c
   if (threadIdx.x < m) {
     double v = 0.0;
     for (int k = 0; k < kSrvThreads / 64; ++k) v += red[k][threadIdx.x];
-    partials[(static_cast<int64_t>(c) * nblk + blockIdx.x) * kHMax + threadIdx.x] = v;
+    partials[(static_cast<int64_t>(c) * kHMax + threadIdx.x) * nblk + blockIdx.x] = v;
   }
 }
 
@@ -212,9 +212,10 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot(
     if (lane == 0) nsh = t;
   }
   __syncthreads();
-  double* pp = partials + (static_cast<int64_t>(c) * gridDim.x + blockIdx.x) * kHMax;
-  if (threadIdx.x < m) pp[threadIdx.x] = red[threadIdx.x];
-  if (threadIdx.x == 0) pp[kNormSlot] = nsh;
+  const int64_t nb = gridDim.x;
+  double* pp = partials + static_cast<int64_t>(c) * kHMax * nb + blockIdx.x;
+  if (threadIdx.x < m) pp[threadIdx.x * nb] = red[threadIdx.x];
+  if (threadIdx.x == 0) pp[kNormSlot * nb] = nsh;
 }
 
 // Pass 1 -> 2 as k_srv_gemv_mdot for MMAX <= 24: one thread per row keeps
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_mdot_rows(
   srv_block_dots<MMAX>(acc, m, c, gridDim.x, partials);
   nrm = block_sum<kSrvThreads>(nrm);
   if (threadIdx.x == 0)
-    partials[(static_cast<int64_t>(c) * gridDim.x + blockIdx.x) * kHMax + kNormSlot] = nrm;
+    partials[(static_cast<int64_t>(c) * kHMax + kNormSlot) * gridDim.x + blockIdx.x] = nrm;
 }
 
 // Thick restart of one column, in place: row by row, V[0..kn) = v[0..m) Y
@@ -308,7 +309,7 @@ __global__ void k_srv_check(const double* __restrict__ partials, int nblk,
   if (j < 0) return;
   double n2 = 0.0;
   for (int b = threadIdx.x; b < nblk; b += 64)
-    n2 += partials[(static_cast<int64_t>(c) * nblk + b) * kHMax + kNormSlot];
+    n2 += partials[(static_cast<int64_t>(c) * kHMax + kNormSlot) * nblk + b];
   n2 = wave_sum(n2);
   double s2 = 0.0;
   for (int i = threadIdx.x; i <= j; i += 64) {
@@ -356,21 +357,24 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
   if (threadIdx.x == 0) partials[static_cast<int64_t>(c) * gridDim.x + blockIdx.x] = nrm;
 }
 
-// h[c*kHMax + i] = vs_i * sum_b partials (fixed order); acc (optional) += h.
-// Grid (R, kHMax), one wave each.
-__global__ void k_srv_fin(const double* __restrict__ partials, int nblk,
-                          const int* __restrict__ colj, const double* __restrict__ vs,
-                          double* __restrict__ h, double* __restrict__ acc) {
+// h[c*kHMax + i] = vs_i * sum_b partials (fixed order); acc[c*kHMax + i] = h
+// (first pass, add = 0) or += h.  Grid (R, kHMax), one workgroup each.
+__global__ __launch_bounds__(kSrvThreads) void k_srv_fin(const double* __restrict__ partials,
+                                                         int nblk, const int* __restrict__ colj,
+                                                         const double* __restrict__ vs,
+                                                         double* __restrict__ h,
+                                                         double* __restrict__ acc, int add) {
   const int c = blockIdx.x, i = blockIdx.y;
   const int j = colj[c];
   if (j < 0 || i > j) return;
+  const double* p = partials + (static_cast<int64_t>(c) * kHMax + i) * nblk;
   double v = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += 64)
-    v += partials[(static_cast<int64_t>(c) * nblk + b) * kHMax + i];
-  v = wave_sum(v) * vs[c * kHMax + i];
+  for (int b = threadIdx.x; b < nblk; b += kSrvThreads) v += p[b];
+  v = block_sum<kSrvThreads>(v);
   if (threadIdx.x == 0) {
+    v *= vs[c * kHMax + i];
     h[c * kHMax + i] = v;
-    if (acc) acc[c * kHMax + i] += v;
+    acc[c * kHMax + i] = add ? acc[c * kHMax + i] + v : v;
   }
 }
 
@@ -623,8 +627,8 @@ void launch_reorth(qsi_engine_s* E, dim3 gcol, hipStream_t s) {
   const int R = E->R;
   hipLaunchKernelGGL(k_srv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld, E->ncv1,
                      E->Wc, E->colj_d, E->slot_d, E->D, E->rpb, E->part);
-  hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
-                     E->vs, E->h, E->hacc);
+  hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(kSrvThreads), 0, s, E->part, E->nblk,
+                     E->colj_d, E->vs, E->h, E->hacc, 0);
   if constexpr (MMAX <= 24)
     hipLaunchKernelGGL(k_srv_gemv_mdot_rows<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
                        E->ncv1, E->h, E->vs, E->colj_d, E->slot_d, E->D, E->rpb, E->Wc,
@@ -633,8 +637,8 @@ void launch_reorth(qsi_engine_s* E, dim3 gcol, hipStream_t s) {
     hipLaunchKernelGGL(k_srv_gemv_mdot<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
                        E->ncv1, E->h, E->vs, E->colj_d, E->slot_d, E->D, E->rpb, E->Wc,
                        E->part);
-  hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(64), 0, s, E->part, E->nblk, E->colj_d,
-                     E->vs, E->h, E->hacc);
+  hipLaunchKernelGGL(k_srv_fin, dim3(R, kHMax), dim3(kSrvThreads), 0, s, E->part, E->nblk,
+                     E->colj_d, E->vs, E->h, E->hacc, 1);
   hipLaunchKernelGGL(k_srv_check, dim3(R), dim3(64), 0, s, E->part, E->nblk, E->colj_d, E->h,
                      E->flag_d, E->beta, E->vs);
   hipLaunchKernelGGL(k_srv_gemv_norm<MMAX>, gcol, dim3(kSrvThreads), 0, s, E->Vb, E->ld,
@@ -871,7 +875,8 @@ int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   rc = launch_blk_apply(a, E->op->basis->nw, E->n_blocks, E->X, E->Wc, nullptr, s);
   if (rc != QSI_OK) return rc;
   const dim3 gcol(E->nblk, nact);
-  QSI_CHECK_HIP(hipMemsetAsync(E->hacc, 0, static_cast<size_t>(R) * kHMax * sizeof(double), s));
+  // hacc is written by the first projection pass (entries past a column's
+  // j are stale and never read)
   if (E->mmax == 16) launch_reorth<16>(E, gcol, s);
   else if (E->mmax == 24) launch_reorth<24>(E, gcol, s);
   else if (E->mmax == 32) launch_reorth<32>(E, gcol, s);
