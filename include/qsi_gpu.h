@@ -301,6 +301,23 @@ int qsi_lanczos_block(const qsi_op_t *ops, int n_sectors, int R,
 int qsi_tridiag_eig(const double *d, const double *e, int n, double *evals,
                     double *first_row);
 
+/* ---- observables ----------------------------------------------------------
+ * compute_all_expectations (src/expectation_eval.cpp:28-53):
+ *   h_out[(l*n_vec + i)*n_vec + j] = psi_j . (O_l psi_i)
+ * compute_cross_correlation (src/expectation_eval.cpp:75-107):
+ *   h_out[((l*n_ops + m)*n_vec + i)*n_vec + j] = (O_l psi_i) . (O_m psi_j)
+ * as eval_observables uses them for the ring, ring-ring and partial-volume
+ * observables (src/eval_observables.cpp:136-206).  ops: n_ops operators on
+ * one basis (each O_l is its own qsi_op_t, e.g. one ring term); d_psi: n_vec
+ * device vectors of the basis dimension, column stride ld.  The reference
+ * materialises each operator as a sparse matrix first; here every O_l psi_i
+ * is one device apply and the dot products are device multi-dots in a fixed
+ * order.  Synchronises. */
+int qsi_expectations(const qsi_op_t *ops, int n_ops, const double *d_psi, int64_t ld, int n_vec,
+                     double *h_out);
+int qsi_cross_correlations(const qsi_op_t *ops, int n_ops, const double *d_psi, int64_t ld,
+                           int n_vec, double *h_out);
+
 /* ---- solve engine: continuous batching of eigensolves on one basis -------
  * Each request is one qsi_eigsh-style solve (lowest nev eigenvalues,
  * thick-restart Lanczos with full reorthogonalisation: one Gram-Schmidt pass

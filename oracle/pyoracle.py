@@ -210,6 +210,33 @@ def sparse_triplets(basis: Basis, H, tol=1e-14):
     return r, c, v
 
 
+def expectations(basis: Basis, op_sums, psi):
+    """compute_all_expectations (src/expectation_eval.cpp:28-53) for one or
+    more vectors: [l, i, j] = psi_j . (O_l psi_i); psi (D,) or (D, n_vec).
+    Each O_l psi_i is the push apply above (the reference materialises O_l as
+    a sparse matrix, operator_matrix.hpp:79-105; same products)."""
+    P = psi.reshape(psi.shape[0], -1)
+    Y = _apply_all(basis, op_sums, P)
+    return np.einsum("lir,rj->lij", Y, P)
+
+
+def cross_correlations(basis: Basis, op_sums, psi):
+    """compute_cross_correlation (src/expectation_eval.cpp:75-107):
+    [l, m, i, j] = (O_l psi_i) . (O_m psi_j)."""
+    P = psi.reshape(psi.shape[0], -1)
+    Y = _apply_all(basis, op_sums, P)
+    return np.einsum("lir,mjr->lmij", Y, Y)
+
+
+def _apply_all(basis, op_sums, P):
+    D, nv = P.shape
+    Y = np.zeros((len(op_sums), nv, D))
+    for l, H in enumerate(op_sums):
+        for i in range(nv):
+            Y[l, i] = evaluate_add(basis, H, np.ascontiguousarray(P[:, i]), np.zeros(D))
+    return Y
+
+
 def normal_stream(seed, n):
     v = np.empty(n, dtype=np.float64)
     load().oracle_normal_stream(seed, _ptr(v), n)

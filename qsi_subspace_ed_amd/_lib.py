@@ -132,6 +132,8 @@ SIGNATURES = {
     "qsi_engine_step_end": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "qsi_engine_step_ready": (C.c_int, [_P]),
     "qsi_engine_inflight": (C.c_int, [_P]),
+    "qsi_expectations": (C.c_int, [_P, C.c_int, _P, _I64, C.c_int, _DP]),
+    "qsi_cross_correlations": (C.c_int, [_P, C.c_int, _P, _I64, C.c_int, _DP]),
     "qsi_engine_poll": (C.c_int, [_P, C.POINTER(_I64), _DP, C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),
     "qsi_vec_dot": (C.c_int, [_P, _P, _I64, _DP, _P]),

@@ -446,6 +446,82 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
 
 }  // extern "C"
 
+namespace {
+
+// Y[:, l*n_vec + i] = O_l psi_i for every operator and vector (Y column-major
+// with leading dimension ldy), then out[b*P + a] = Y[:, a] . W[:, b] for the
+// P = n_ops*n_vec columns a and the n_w columns b of W (ldw), by multi-dots
+// of up to 64 columns.  Synchronises.
+int observe(const qsi_op_t* ops, int n_ops, const double* d_psi, int64_t ld, int n_vec,
+            bool gram, std::vector<double>& out, int* P_out) {
+  QSI_REQUIRE(ops && d_psi && n_ops >= 1 && n_vec >= 1, "observables: bad arguments");
+  const qsi_basis_s* basis = ops[0] ? ops[0]->basis : nullptr;
+  QSI_REQUIRE(basis != nullptr, "observables: null operator");
+  for (int l = 0; l < n_ops; ++l)
+    QSI_REQUIRE(ops[l] && ops[l]->basis == basis, "observables: operators on different bases");
+  const int64_t D = basis->dim;
+  QSI_REQUIRE(D > 0 && ld >= D, "observables: ld < dim");
+  const int P = n_ops * n_vec;
+  const int64_t ldy = (D + 63) / 64 * 64;
+  hipStream_t s = nullptr;
+  DevBuf bY, bpart, bout;
+  QSI_CHECK_HIP(bY.alloc(static_cast<size_t>(ldy) * P * sizeof(double)));
+  QSI_CHECK_HIP(bpart.alloc(static_cast<size_t>(multidot_partials()) * sizeof(double)));
+  const int n_w = gram ? P : n_vec;
+  QSI_CHECK_HIP(bout.alloc(static_cast<size_t>(P) * n_w * sizeof(double)));
+  double* Y = bY.d();
+  for (int l = 0; l < n_ops; ++l)
+    for (int i = 0; i < n_vec; ++i)
+      QSI_TRY(launch_apply(ops[l], d_psi + static_cast<int64_t>(i) * ld,
+                           Y + static_cast<int64_t>(l * n_vec + i) * ldy, nullptr, 0.0, nullptr,
+                           nullptr, s));
+  for (int b = 0; b < n_w; ++b) {
+    const double* w = gram ? Y + static_cast<int64_t>(b) * ldy : d_psi + static_cast<int64_t>(b) * ld;
+    for (int a0 = 0; a0 < P; a0 += 64)
+      QSI_TRY(launch_multidot(Y + static_cast<int64_t>(a0) * ldy, ldy, std::min(64, P - a0), w, D,
+                              bpart.d(), bout.d() + static_cast<int64_t>(b) * P + a0, nullptr, s));
+  }
+  out.resize(static_cast<size_t>(P) * n_w);
+  QSI_CHECK_HIP(hipMemcpy(out.data(), bout.d(), out.size() * sizeof(double), hipMemcpyDeviceToHost));
+  *P_out = P;
+  return QSI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsi_expectations(const qsi_op_t* ops, int n_ops, const double* d_psi, int64_t ld, int n_vec,
+                     double* h_out) {
+  QSI_REQUIRE(h_out != nullptr, "qsi_expectations: null output");
+  std::vector<double> g;
+  int P = 0;
+  QSI_TRY(observe(ops, n_ops, d_psi, ld, n_vec, false, g, &P));
+  for (int l = 0; l < n_ops; ++l)
+    for (int i = 0; i < n_vec; ++i)
+      for (int j = 0; j < n_vec; ++j)
+        h_out[(static_cast<int64_t>(l) * n_vec + i) * n_vec + j] =
+            g[static_cast<size_t>(j) * P + l * n_vec + i];
+  return QSI_OK;
+}
+
+int qsi_cross_correlations(const qsi_op_t* ops, int n_ops, const double* d_psi, int64_t ld,
+                           int n_vec, double* h_out) {
+  QSI_REQUIRE(h_out != nullptr, "qsi_cross_correlations: null output");
+  std::vector<double> g;
+  int P = 0;
+  QSI_TRY(observe(ops, n_ops, d_psi, ld, n_vec, true, g, &P));
+  for (int l = 0; l < n_ops; ++l)
+    for (int m = 0; m < n_ops; ++m)
+      for (int i = 0; i < n_vec; ++i)
+        for (int j = 0; j < n_vec; ++j)
+          h_out[((static_cast<int64_t>(l) * n_ops + m) * n_vec + i) * n_vec + j] =
+              g[static_cast<size_t>(m * n_vec + j) * P + l * n_vec + i];
+  return QSI_OK;
+}
+
+}  // extern "C"
+
 // ---------------------------------------------------------------------------
 // Basis enumeration
 // ---------------------------------------------------------------------------

@@ -14,7 +14,8 @@ Outputs (under tests/golden/ unless noted):
                                     lattice specs (copied byte for byte; the
                                     package ships them as workload inputs)
     basis_<W>.npy / .npz            sorted (D, 2) uint64 [lo, hi] ice bases
-    eigs48_jpm0.02_B0.1.npz         E0 + eigenvector (+ .out twin values)
+    eigs48_jpm0.02_B0.1.npz         E0 + eigenvector (+ .out twin values and the
+                                    ring, ring_ring, partial_vol_sl0..3 observables)
     MANIFEST.json                   sha256 of every fixture + provenance
 """
 import hashlib
@@ -91,9 +92,11 @@ def main(ref_root):
     e0_out = _h5dump_raw(os.path.join(ref_root, EIGS_OUT), "/eigenvalues", "<f8")
     ring = _h5dump_raw(os.path.join(ref_root, EIGS_OUT), "/ring", "<f8")
     ring_ring = _h5dump_raw(os.path.join(ref_root, EIGS_OUT), "/ring_ring", "<f8").reshape(48, 48)
+    partial_vol = np.stack([_h5dump_raw(os.path.join(ref_root, EIGS_OUT), f"/partial_vol_sl{sl}", "<f8")
+                            for sl in range(4)])
     dst = os.path.join(HERE, "eigs48_jpm0.02_B0.1.npz")
     np.savez_compressed(dst, eigenvalues=e0, eigenvector=vec, eigenvalues_out=e0_out,
-                        ring=ring, ring_ring=ring_ring)
+                        ring=ring, ring_ring=ring_ring, partial_vol=partial_vol)
     manifest["files"]["eigs48_jpm0.02_B0.1.npz"] = {
         "from": [EIGS, EIGS_OUT],
         "produced_by": "test/test_correct.sh:8 (mfeig0, Jpm=0.02, B=(0.1,0.1,0.1), -k 20, "
