@@ -52,14 +52,17 @@ def _close(a, b):
     return abs(a - b) <= EIG_TOL * max(1.0, abs(b))
 
 
-def test_engine_vs_dense_and_eigsh(lattices, cuda):
+@pytest.mark.parametrize("ncv", [12, 24, 30, 40])
+def test_engine_vs_dense_and_eigsh(ncv, lattices, cuda):
+    """Every Krylov-basis width class of the engine's kernels (up to 16, 24,
+    32 and 64 vectors)."""
     jd = lattices[32]
     b = load_basis(32)
     basis = q.GpuBasis(b, 32)
     op = q.GpuOp(basis, serve.slotted_hamiltonian(jd))
     assert op.n_slots == serve.N_SLOTS
     nev = 3
-    eng = serve.Engine(op, max_cols=8, nev=nev, ncv=24, tol=1e-11)
+    eng = serve.Engine(op, max_cols=8, nev=nev, ncv=ncv, tol=1e-11)
     ids = {}
     for k, (g, B) in enumerate(POINTS):
         ids[eng.submit(serve.point_coeffs(g, _h_of(B)), seed=100 + k)] = k
@@ -77,7 +80,7 @@ def test_engine_vs_dense_and_eigsh(lattices, cuda):
             k = int(np.argmin(np.abs(w - f.evals[i])))
             assert _close(f.evals[i], w[k]), (ids[f.id], i, f.evals[i], w[k])
         assert np.all(np.diff(f.evals) >= 0)
-        ev, _, _, _ = q.GpuOp(basis, H).eigsh(nev=nev, ncv=24, tol=1e-11)
+        ev, _, _, _ = q.GpuOp(basis, H).eigsh(nev=nev, ncv=ncv, tol=1e-11)
         assert _close(f.evals[0], ev[0])
 
 
