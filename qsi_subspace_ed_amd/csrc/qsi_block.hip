@@ -357,7 +357,9 @@ __global__ __launch_bounds__(256) void k_cc_fill(BasisView b, const OffTerm* __r
 
 // The block apply over cached connectivity: same arithmetic, order and
 // results as k_blk_apply.  G lanes per row (one per column), PIPE entries'
-// gathers in flight.
+// gathers in flight.  G may be any width up to 64 when no partials are asked
+// for (a wavefront then serves floor(64 / G) rows per pass); with partials it
+// must be a power of two (the per-column sums cross row groups by butterfly).
 template <int NW, int PIPE>
 __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
     BlkArgs a, const double* __restrict__ X, double* __restrict__ Y,
@@ -365,18 +367,18 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int G = a.G, R = a.R;
-  const int lg = __builtin_ctz(G);
-  const int grp = lane >> lg, col = lane & (G - 1);
-  const bool col_ok = col < R;
+  const int grp = lane / G, col = lane % G;
+  const int rpp = 64 / G;
+  const bool col_ok = col < R && grp < rpp;
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * kBlkWaves + wv;
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * kBlkWaves;
-  const int rpp = 64 >> lg;
   double dot = 0.0;
   int flushed = 0;
   auto flush_to = [&](int upto, int cur) {
     for (; flushed < upto; ++flushed) {
+      if (!partials) continue;
       const double v = sum_over_groups(flushed == cur ? dot : 0.0, G);
-      if (partials && grp == 0 && col_ok) partials[(flushed * n_waves + wave) * R + col] = v;
+      if (grp == 0 && col_ok) partials[(flushed * n_waves + wave) * R + col] = v;
     }
   };
   int cur = -1;

@@ -870,7 +870,8 @@ int qsi_engine_step_begin(qsi_engine_t E, int* launched) {
   a.R = nact;
   int G = 64;
   while (G / 2 >= nact && G > 1) G >>= 1;
-  a.G = G;
+  // the cached kernel takes any row-group width: one lane per live column
+  a.G = blk_cached(a) ? nact : G;
   a.ldx = ldx;
   rc = launch_blk_apply(a, E->op->basis->nw, E->n_blocks, E->X, E->Wc, nullptr, s);
   if (rc != QSI_OK) return rc;
