@@ -490,7 +490,7 @@ def main():
                     help="64spin: the default line; 96spin: the HBM-scale apply")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
 
