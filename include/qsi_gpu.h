@@ -187,7 +187,9 @@ int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
  * bit-identical for all).  No counterpart in the reference. */
 int qsi_op_set_cached_pipe(qsi_op_t op, int pipe);
 int qsi_op_get_cached_pipe(qsi_op_t op);
-/* Block-apply configuration (bits 0-1: kernel 0-2, bit 2: half-size grid) of
+/* Block-apply configuration (bits 0-1: kernel 0-2, bit 2: half-size grid,
+ * bit 3: two columns per lane in the cached kernel, used when X's rows are
+ * 16-byte aligned and hold an even number of columns) of
  * qsi_op_apply_block, qsi_lanczos_block (the first sector's) and engines
  * created afterwards.  Chosen by qsi_op_autotune / qsi_block_autotune; set it
  * to pin one.  No counterpart in the reference. */

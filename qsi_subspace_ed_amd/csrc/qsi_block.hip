@@ -458,6 +458,134 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
   flush_to(a.n_sec, cur);
 }
 
+// Column pairs: each lane of a row group serves columns c0 = 2q and c0 + 1
+// with one 16-byte gather per entry, so a wavefront covers twice the rows per
+// pass of k_blk_capply for the same number of memory instructions (G here is
+// the number of lanes per row, ceil(R / 2), a power of two when partials are
+// asked for).  Needs 16-byte aligned rows of X that hold both columns of
+// every lane: even ldx >= R rounded up to even.  Per column the arithmetic
+// and its order are those of k_blk_capply, so Y is identical; partials sum
+// the row groups in another order.
+template <int NW, int PIPE>
+__global__ __launch_bounds__(kBlkThreads) void k_blk_capply2(
+    BlkArgs a, const double* __restrict__ X, double* __restrict__ Y,
+    double* __restrict__ partials) {
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int G = a.G, R = a.R;
+  const int grp = lane / G, c0 = 2 * (lane % G);
+  const int rpp = 64 / G;
+  const bool ok0 = c0 < R && grp < rpp;
+  const bool ok1 = ok0 && c0 + 1 < R;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * kBlkWaves + wv;
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * kBlkWaves;
+  double dot0 = 0.0, dot1 = 0.0;
+  int flushed = 0;
+  auto flush_to = [&](int upto, int cur) {
+    for (; flushed < upto; ++flushed) {
+      if (!partials) continue;
+      const double v0 = sum_over_groups(flushed == cur ? dot0 : 0.0, G);
+      const double v1 = sum_over_groups(flushed == cur ? dot1 : 0.0, G);
+      if (grp == 0 && ok0) partials[(flushed * n_waves + wave) * R + c0] = v0;
+      if (grp == 0 && ok1) partials[(flushed * n_waves + wave) * R + c0 + 1] = v1;
+    }
+  };
+  int cur = -1;
+  double cc0[kMaxCoefClasses], cc1[kMaxCoefClasses];
+  for (int64_t t = wave; t < a.n_tiles; t += n_waves) {
+    const int s = __builtin_amdgcn_readfirstlane(find_sector(a, t));
+    const BlkSector& S = a.sec[s];
+    if (s != cur) {
+      flush_to(s, cur);
+      cur = s;
+      dot0 = dot1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < kMaxCoefClasses; ++k) {
+        cc0[k] = cc1[k] = 0.0;
+        if (k < S.n_cls) {
+          const CoefClass& K = S.cls[k];
+          const bool slot = S.coeffs && K.slot >= 0;
+          if (ok0) cc0[k] = slot ? S.coeffs[K.slot * R + c0] : K.c;
+          if (ok1) cc1[k] = slot ? S.coeffs[K.slot * R + c0 + 1] : K.c;
+        }
+      }
+    }
+    const int64_t r_begin = (t - S.tile_begin) * kTileRows;
+    const int n_rows = static_cast<int>(min<int64_t>(kTileRows, S.b.dim - r_begin));
+    const int n_pass = (n_rows + rpp - 1) / rpp;
+    for (int p = 0; p < n_pass; ++p) {
+      const int r = p * rpp + grp;
+      if (r >= n_rows || !ok0) continue;
+      const int64_t i = r_begin + r;
+      const int64_t row = S.row_off + i;
+      const int64_t g0 = a.y_colmajor ? c0 * a.ldy + row : row * a.ldy + c0;
+      const int64_t g1 = a.y_colmajor ? g0 + a.ldy : g0 + 1;
+      double acc0 = a.accumulate ? Y[g0] : 0.0;
+      double acc1 = a.accumulate && ok1 ? Y[g1] : 0.0;
+      const uint64_t e0 = S.rowptr[i], e1 = S.rowptr[i + 1];
+      for (uint64_t e = e0; e < e1; e += PIPE) {
+        uint32_t J[PIPE];
+        uint8_t mt[PIPE];
+        double2 xj[PIPE];
+#pragma unroll
+        for (int u = 0; u < PIPE; ++u) {
+          const bool in = e + u < e1;
+          J[u] = in ? S.cj[e + u] : kNoRank;
+          mt[u] = in ? S.cm[e + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < PIPE; ++u)
+          xj[u] = J[u] != kNoRank
+                      ? *reinterpret_cast<const double2*>(
+                            X + (S.row_off + static_cast<int64_t>(J[u])) * a.ldx + c0)
+                      : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < PIPE; ++u) {
+          if (J[u] != kNoRank) {
+            const int k = mt[u] & 0x7F;
+            double ca = cc0[0], cb = cc1[0];
+#pragma unroll
+            for (int q = 1; q < kMaxCoefClasses; ++q)
+              if (k == q) {
+                ca = cc0[q];
+                cb = cc1[q];
+              }
+            if (mt[u] & 0x80) {
+              ca = -ca;
+              cb = -cb;
+            }
+            if (fabs(xj[u].x) > kApplyTol) acc0 += ca * xj[u].x;
+            if (fabs(xj[u].y) > kApplyTol) acc1 += cb * xj[u].y;
+          }
+        }
+      }
+      const State<NW> sr = load_state_cached<NW>(S.b.states, i);
+      const double2 xi = *reinterpret_cast<const double2*>(X + row * a.ldx + c0);
+      for (int k = 0; k < S.n_diag; ++k) {
+        const DiagTerm& d = S.dg[k];
+        double f;
+        if (d.kind == 0) {
+          f = diag_ok<NW>(d, sr) ? d.sgn * (1 - 2 * parity_notz<NW>(sr, d.Z)) : 0.0;
+        } else {
+          int nz = __popcll(d.Z[0]);
+          if constexpr (NW == 2) nz += __popcll(d.Z[1]);
+          f = d.sgn * static_cast<double>(2 * popc_and<NW>(sr, d.Z) - nz);
+        }
+        const bool slot = S.coeffs && d.slot >= 0;
+        const double ca = slot ? S.coeffs[d.slot * R + c0] : d.c;
+        const double cb = ok1 ? (slot ? S.coeffs[d.slot * R + c0 + 1] : d.c) : 0.0;
+        acc0 += ca * xi.x * f;
+        acc1 += cb * xi.y * f;
+      }
+      dot0 += xi.x * acc0;
+      dot1 += xi.y * acc1;
+      Y[g0] = acc0;
+      if (ok1) Y[g1] = acc1;
+    }
+  }
+  flush_to(a.n_sec, cur);
+}
+
 // C -= alpha B + beta A (beta, A skipped when A is null); |C_c|^2 partials.
 __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
     BlkArgs a, const double* __restrict__ A, const double* B,
@@ -573,9 +701,14 @@ namespace {
 using BlkKernel = void (*)(BlkArgs, const double*, double*, double*);
 
 // Block-apply kernel of a configuration: cached kernels by gathers in flight
-// (4, 8, 16), searching kernels by lookups in flight (4, 2).
-BlkKernel blk_kernel(int nw, bool cached, int cfg) {
+// (4, 8, 16), one or two columns per lane; searching kernels by lookups in
+// flight (4, 2).
+BlkKernel blk_kernel(int nw, bool cached, int cfg, bool pairs) {
   const int k = blk_cfg_kernel(cfg);
+  if (cached && pairs) {
+    if (nw == 1) return k == 0 ? k_blk_capply2<1, 4> : (k == 2 ? k_blk_capply2<1, 16> : k_blk_capply2<1, 8>);
+    return k == 0 ? k_blk_capply2<2, 4> : (k == 2 ? k_blk_capply2<2, 16> : k_blk_capply2<2, 8>);
+  }
   if (cached) {
     if (nw == 1) return k == 0 ? k_blk_capply<1, 4> : (k == 2 ? k_blk_capply<1, 16> : k_blk_capply<1, 8>);
     return k == 0 ? k_blk_capply<2, 4> : (k == 2 ? k_blk_capply<2, 16> : k_blk_capply<2, 8>);
@@ -586,11 +719,17 @@ BlkKernel blk_kernel(int nw, bool cached, int cfg) {
 
 }  // namespace
 
+bool blk_pairs(const BlkArgs& a, const double* X) {
+  return blk_cached(a) && blk_cfg_pairs(a.cfg) && a.R >= 2 && a.ldx % 2 == 0 &&
+         a.ldx >= a.R + (a.R & 1) && reinterpret_cast<uintptr_t>(X) % 16 == 0;
+}
+
 int blk_apply_blocks(int nw, int64_t n_tiles, bool cached, int cfg) {
-  // resident blocks per CU of each kernel: [cached][nw - 1][kernel]
-  static int per_cu[2][2][3] = {};
+  // resident blocks per CU of each kernel: [cached][nw - 1][kernel][pairs]
+  static int per_cu[2][2][3][2] = {};
   static int n_cu = 0;
-  int& occ = per_cu[cached ? 1 : 0][nw - 1][blk_cfg_kernel(cfg)];
+  const bool pairs = cached && blk_cfg_pairs(cfg);
+  int& occ = per_cu[cached ? 1 : 0][nw - 1][blk_cfg_kernel(cfg)][pairs ? 1 : 0];
   if (occ == 0) {
     int dev = 0, o = 0;
     (void)hipGetDevice(&dev);
@@ -598,7 +737,7 @@ int blk_apply_blocks(int nw, int64_t n_tiles, bool cached, int cfg) {
         n_cu <= 0)
       n_cu = 256;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &o, blk_kernel(nw, cached, cfg), kBlkThreads, 0);
+        &o, blk_kernel(nw, cached, cfg, pairs), kBlkThreads, 0);
     occ = (e == hipSuccess && o > 0) ? o : 4;
   }
   int64_t resident = static_cast<int64_t>(occ) * n_cu;
@@ -609,8 +748,24 @@ int blk_apply_blocks(int nw, int64_t n_tiles, bool cached, int cfg) {
 
 int launch_blk_apply(const BlkArgs& a, int nw, int n_blocks, const double* X, double* Y,
                      double* partials, hipStream_t s) {
-  hipLaunchKernelGGL(blk_kernel(nw, blk_cached(a), a.cfg), dim3(n_blocks), dim3(kBlkThreads), 0,
-                     s, a, X, Y, partials);
+  if (blk_pairs(a, X)) {
+    // ceil(R / 2) lanes per row; a power of two when the partials cross
+    // row groups by butterfly
+    BlkArgs b = a;
+    const int w = (a.R + 1) / 2;
+    int G = w;
+    if (partials) {
+      G = 1;
+      while (G < w) G <<= 1;
+    }
+    b.G = G;
+    hipLaunchKernelGGL(blk_kernel(nw, true, a.cfg, true), dim3(n_blocks), dim3(kBlkThreads), 0,
+                       s, b, X, Y, partials);
+    QSI_CHECK_LAUNCH();
+    return QSI_OK;
+  }
+  hipLaunchKernelGGL(blk_kernel(nw, blk_cached(a), a.cfg, false), dim3(n_blocks),
+                     dim3(kBlkThreads), 0, s, a, X, Y, partials);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }
@@ -915,10 +1070,12 @@ int tune_block(BlkArgs a, int nw, int reps, const double* X, double* Y, int* pic
 
   const bool cached = blk_cached(a);
   const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
+  const int n_pairs = cached && a.R >= 2 ? 2 : 1;
   *pick = -1;
   for (int k : kernels) {
-    for (int half = 0; half < 2; ++half) {
-      a.cfg = k | (half << 2);
+    for (int cf = 0; cf < 2 * n_pairs; ++cf) {
+      const int half = cf & 1, pairs = cf >> 1;
+      a.cfg = k | (half << 2) | (pairs << 3);
       const int nb = blk_apply_blocks(nw, a.n_tiles, cached, a.cfg);
       double ms = 0.0;
       const int rc = time_ms([&]() { return launch_blk_apply(a, nw, nb, X, Y, nullptr, nullptr); },
@@ -976,11 +1133,14 @@ extern "C" int qsi_block_autotune(const qsi_op_t* ops, int n_sectors, int R, int
   int64_t rows = 0;
   const bool cached = blk_cached(sector_args(ops, n_sectors, R, cp.data(), &rows));
   const std::vector<int> kernels = cached ? std::vector<int>{0, 1, 2} : std::vector<int>{1, 2};
+  const int n_pairs = cached && R >= 2 ? 2 : 1;
   int pick = -1;
   double pick_ms = 0.0;
   for (int k : kernels) {
-    for (int half = 0; half < 2; ++half) {
-      for (int s = 0; s < n_sectors; ++s) ops[s]->blk_cfg = k | (half << 2);
+    for (int cf = 0; cf < 2 * n_pairs; ++cf) {
+      const int half = cf & 1, pairs = cf >> 1;
+      const int cfg = k | (half << 2) | (pairs << 3);
+      for (int s = 0; s < n_sectors; ++s) ops[s]->blk_cfg = cfg;
       double ms = 0.0;
       for (int r = 0; r <= reps; ++r) {  // the first run warms up
         double t = 0.0;
@@ -994,7 +1154,7 @@ extern "C" int qsi_block_autotune(const qsi_op_t* ops, int n_sectors, int R, int
       }
       ms /= reps;
       if (pick < 0 || ms < pick_ms) {
-        pick = k | (half << 2);
+        pick = cfg;
         pick_ms = ms;
       }
     }

@@ -48,10 +48,12 @@ struct BlkArgs {
 
 // Block-apply configuration: bits 0-1 pick the kernel (cached connectivity:
 // 4 / 8 / 16 gathers in flight per lane; searching: 4 / 4 / 2 lookups in
-// flight), bit 2 halves the persistent grid.
+// flight), bit 2 halves the persistent grid, bit 3 lets the cached kernel
+// serve two columns per lane (16-byte gathers) where X's rows allow it.
 constexpr int kBlkCfgDefault = 1;
 inline int blk_cfg_kernel(int cfg) { return cfg & 3; }
 inline bool blk_cfg_half(int cfg) { return (cfg >> 2) & 1; }
+inline bool blk_cfg_pairs(int cfg) { return (cfg >> 3) & 1; }
 
 // True when every sector of `a` has its connectivity cached: the block
 // apply then gathers through the cached ranks instead of searching.
@@ -60,6 +62,11 @@ inline bool blk_cached(const BlkArgs& a) {
     if (!a.sec[k].rowptr) return false;
   return a.n_sec > 0;
 }
+
+// True when a cached block apply on X serves two columns per lane: the
+// configuration's pairs bit, R >= 2, an even ldx >= R rounded up to even and
+// a 16-byte aligned X.
+bool blk_pairs(const BlkArgs& a, const double* X);
 
 // Blocks of a persistent block-apply grid over n_tiles 64-row tiles: all
 // resident at once (occupancy x compute units), never more than the tiles

@@ -1584,8 +1584,9 @@ int qsi_op_get_cached_pipe(qsi_op_t op) { return op ? op->cached_pipe : -1; }
 
 int qsi_op_set_block_cfg(qsi_op_t op, int cfg) {
   QSI_REQUIRE(op != nullptr, "qsi_op_set_block_cfg: null operator");
-  QSI_REQUIRE(cfg >= 0 && cfg <= 7 && blk_cfg_kernel(cfg) <= 2,
-              "qsi_op_set_block_cfg: cfg must be kernel 0-2 plus 4 for a half grid");
+  QSI_REQUIRE(cfg >= 0 && cfg <= 15 && blk_cfg_kernel(cfg) <= 2,
+              "qsi_op_set_block_cfg: cfg must be kernel 0-2, plus 4 for a half grid, "
+              "plus 8 for column pairs");
   op->blk_cfg = cfg;
   return QSI_OK;
 }

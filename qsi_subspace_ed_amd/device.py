@@ -206,7 +206,8 @@ class GpuOp:
         return _lib.load().qsi_op_get_cached_pipe(self._h)
 
     def set_block_cfg(self, cfg):
-        """Pin the block-apply configuration (kernel 0-2, + 4 for a half grid)."""
+        """Pin the block-apply configuration (kernel 0-2, + 4 for a half grid,
+        + 8 for two columns per lane in the cached kernel)."""
         check(_lib.load().qsi_op_set_block_cfg(self._h, int(cfg)), "qsi_op_set_block_cfg")
 
     def block_cfg(self):

@@ -295,6 +295,40 @@ def test_cached_connectivity_block_apply_identical(R, lattices, cuda):
     assert op.cache_connectivity(False) == -1
 
 
+@pytest.mark.parametrize("R,ldx", [(2, 2), (3, 3), (3, 4), (5, 6), (6, 6), (6, 8), (8, 8),
+                                   (16, 16), (64, 64)])
+def test_cached_block_apply_column_pairs(R, ldx, lattices, cuda):
+    """The cached block apply with two columns per lane (configuration bit 3)
+    gives the bits of the one-column kernel at every gather depth, for X rows
+    padded or not (an odd unpadded R runs the one-column kernel)."""
+    torch = cuda
+    W = 48
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    Hs = q.lattice.add_ring_terms(q.SymbolicOpSum(), jd, [0.0] * 4, slots=True)
+    q.add_local_field(Hs, jd, [0.0, 0.0, 0.0], slots=True)
+    op = q.GpuOp(q.GpuBasis(b, W), Hs, n_slots=8)
+    op.cache_connectivity(True)
+    rng = np.random.default_rng(100 + R)
+    Xp = torch.from_numpy(rng.standard_normal((D, ldx))).cuda()
+    X = Xp[:, :R]
+    co = torch.from_numpy(rng.standard_normal((8, R))).cuda()
+    for k in range(3):
+        ys = []
+        for cfg in (k, k | 8, k | 8 | 4):
+            op.set_block_cfg(cfg)
+            assert op.block_cfg() == cfg
+            Y = torch.full((D, R), 7.0, dtype=torch.float64, device="cuda")
+            op.apply_block(X, Y, coeffs=co)
+            Ya = torch.ones((D, R), dtype=torch.float64, device="cuda")
+            op.apply_block(X, Ya, coeffs=co, accumulate=True)
+            ys.append((Y, Ya))
+        for Y, Ya in ys[1:]:
+            assert torch.equal(Y, ys[0][0]), k
+            assert torch.equal(Ya, ys[0][1]), k
+
+
 def test_autotune_keeps_results(lattices, cuda):
     """The autotuner picks a kernel by timing; every pick gives the same bits."""
     torch = cuda
@@ -322,7 +356,7 @@ def test_autotune_keeps_results(lattices, cuda):
     for cache in (False, True):
         op.cache_connectivity(cache)
         cfg, ms = op.autotune(R, reps=2)
-        assert 0 <= cfg < 8 and ms > 0
+        assert 0 <= cfg < 16 and ms > 0
         Y1 = torch.zeros_like(X)
         op.apply_block(X, Y1, coeffs=co)
         assert torch.equal(Y0, Y1)

@@ -117,6 +117,31 @@ def test_cached_connectivity_same_coefficients(lattices, cuda):
     assert np.max(np.abs(b0 - b1)) <= 1e-12 * sc
 
 
+@pytest.mark.parametrize("R", [6, 16])
+def test_cached_column_pairs_same_coefficients(R, lattices, cuda):
+    """Block Lanczos with the cached kernel's two columns per lane: the same
+    recurrence as one column per lane (the alpha partials cross the row
+    groups in another order, so 1e-12 relative, not bits)."""
+    torch = cuda
+    jd = lattices[48]
+    H = _ham(jd, [0.7, 0.3, 0.9, 0.5], [0.02, 0.01, -0.03])
+    bases = [q.enumerate_basis(jd, n, n_threads=8) for n in (0, 1)]
+    ops = [q.GpuOp(q.GpuBasis(bb, 48), H) for bb in bases]
+    M = 30
+    V0s = [torch.from_numpy(ftlm.random_sign_block(bb.shape[0], R, seed=9 + k)).cuda()
+           for k, bb in enumerate(bases)]
+    for op in ops:
+        op.cache_connectivity(True)
+        op.set_block_cfg(1)
+    a0, b0, _ = ftlm.lanczos_block(ops, V0s, M)
+    for op in ops:
+        op.set_block_cfg(9)
+    a1, b1, _ = ftlm.lanczos_block(ops, V0s, M)
+    sc = np.max(np.abs(a0))
+    assert np.max(np.abs(a0 - a1)) <= 1e-12 * sc
+    assert np.max(np.abs(b0 - b1)) <= 1e-12 * sc
+
+
 def test_block_autotune_multi_sector(lattices, cuda):
     """ftlm.autotune times the configurations on the two-sector launch and
     sets one that lanczos_block then uses; the recurrence stays the same
@@ -133,7 +158,7 @@ def test_block_autotune_multi_sector(lattices, cuda):
         op.cache_connectivity(True)
     a0, b0, _ = ftlm.lanczos_block(ops, V0s, M)
     cfg, ms = ftlm.autotune(ops, R, reps=2)
-    assert cfg in (0, 1, 2, 4, 5, 6) and ms > 0
+    assert cfg in (0, 1, 2, 4, 5, 6, 8, 9, 10, 12, 13, 14) and ms > 0
     a1, b1, _ = ftlm.lanczos_block(ops, V0s, M)
     sc = np.max(np.abs(a0))
     assert np.max(np.abs(a0 - a1)) <= 1e-12 * sc

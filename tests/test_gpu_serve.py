@@ -184,21 +184,27 @@ def test_engine_matches_eigsh_same_start(lattices, cuda):
 
 
 def test_engine_cached_connectivity_bit_identical(lattices, cuda):
-    """The engine gives the same bits with and without cached connectivity."""
+    """The engine gives the same bits with and without cached connectivity,
+    and with the cached kernel's one or two columns per lane (4, 3, 2 and 1
+    live columns as requests finish)."""
     jd = lattices[32]
     b = load_basis(32)
     basis = q.GpuBasis(b, 32)
     out = []
-    for cache in (False, True):
+    for cache, cfg in ((False, None), (True, 1), (True, 9), (True, 8)):
         op = q.GpuOp(basis, serve.slotted_hamiltonian(jd))
-        eng = serve.Engine(op, max_cols=4, nev=3, ncv=16, tol=1e-10, cache_connectivity=cache)
+        if cfg is not None:
+            op.set_block_cfg(cfg)
+        eng = serve.Engine(op, max_cols=4, nev=3, ncv=16, tol=1e-10, cache_connectivity=cache,
+                           autotune=cfg is None)
         for k, (g, B) in enumerate(POINTS[:4]):
             eng.submit(serve.point_coeffs(g, _h_of(B)), seed=70 + k)
         out.append({f.id: f.evals for f in eng.run_all()})
         eng.close()
-    assert out[0].keys() == out[1].keys()
-    for k in out[0]:
-        assert np.array_equal(out[0][k], out[1][k])
+    for o in out[1:]:
+        assert out[0].keys() == o.keys()
+        for k in out[0]:
+            assert np.array_equal(out[0][k], o[k])
 
 
 def test_server_concurrent_engines_same_bits(lattices, cuda):
