@@ -65,6 +65,24 @@ def build_library(verbose=False):
     return LIB
 
 
+CPP_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "wrapper_check.cpp")
+CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "wrapper_check")
+
+
+def build_cpp_check():
+    """Compile the C++ face's check program (tests/cpp/wrapper_check) against
+    include/qsi_gpu.hpp and libqsi_gpu.so (found through an $ORIGIN rpath)."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    tmp = CPP_CHECK + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I" + os.path.join(rocm, "include"), "-I" + os.path.join(ROOT, "include"),
+          CPP_CHECK_SRC, "-L" + PKG, "-lqsi_gpu", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN/../../qsi_subspace_ed_amd", "-Wl,-rpath," + os.path.join(rocm, "lib"),
+          "-o", tmp])
+    os.replace(tmp, CPP_CHECK)
+    return CPP_CHECK
+
+
 def build_oracle(with_reference=None):
     """Build oracle/liboracle.so; also oracle/_ref/libref.so when the reference
     tree is present (with_reference=None: build it if it exists)."""
@@ -79,5 +97,6 @@ def build_oracle(with_reference=None):
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_cpp_check()
     build_oracle()
     print("built", LIB, file=sys.stderr)

@@ -42,3 +42,21 @@ def test_version_and_validation():
     assert rc == _lib.QSI_E_INVALID
     rc = lib.qsi_basis_create(None, 1, 0, 0, C.byref(h))
     assert rc == _lib.QSI_E_INVALID
+
+
+def test_cpp_face_compiles():
+    """include/qsi_gpu.hpp (the C++ face over the ABI) and its check program
+    compile with the host compiler (no GPU needed)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        import pytest
+        pytest.skip("no g++")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(rocm, "include"),
+                        "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "wrapper_check.cpp")],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
