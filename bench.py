@@ -65,6 +65,20 @@ def build_workload(seed=1234, n_threads=16):
     return jd, states, H, g, B, t_enum
 
 
+def host_cpu():
+    """The host's CPU model and logical core count (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(states, H, x, budget_s=12.0, threads=None):
     """Oracle push apply (the reference's OpenMP algorithm) on this host."""
     from oracle import pyoracle as orc
@@ -88,7 +102,7 @@ def cpu_baseline(states, H, x, budget_s=12.0, threads=None):
     t = float(np.median(times))
     return {"value": D / t, "unit": "basis-states/s", "cores": int(nthr), "kind": "port",
             "sample": f"{n} full applies (after 1 warm-up) of the same 64-spin workload, "
-                      f"median {t * 1e3:.1f} ms per apply"}, y
+                      f"median {t * 1e3:.1f} ms per apply", **host_cpu()}, y
 
 
 def bench_ftlm(R=64, M=100, seed=77, plain_samples=4):
@@ -463,7 +477,7 @@ def bench_96(args):
         xh = x.cpu().numpy()
         rate, nthr, sample = cpu_sample_rate(states, H, xh)
         out["cpu_baseline"] = {"value": rate, "unit": "basis-states/s", "cores": int(nthr),
-                               "kind": "port", "sample": sample}
+                               "kind": "port", "sample": sample, **host_cpu()}
         out["speedup_vs_cpu"] = out["value"] / rate
     print(json.dumps(out), flush=True)
 

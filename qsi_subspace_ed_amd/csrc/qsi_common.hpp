@@ -161,6 +161,16 @@ __device__ __forceinline__ bool st_lt(const State<NW>& a, const State<NW>& b) {
   }
 }
 
+// Bucket k's ranks [lo, lo + n) from table[k], table[k + 1] with one 8-byte
+// load (the pair is 4-byte aligned, which global loads allow).
+__device__ __forceinline__ void bucket_bounds(const uint32_t* __restrict__ table, uint32_t k,
+                                              uint32_t& lo, uint32_t& n) {
+  uint2 p;
+  __builtin_memcpy(&p, table + k, sizeof(p));
+  lo = p.x;
+  n = p.y - p.x;
+}
+
 // Rank of `key` in the basis, or -1.  The bucket [table[k], table[k+1]) is
 // searched with a branch-free lower-bound bisection.  A basis state's key
 // selects the bucket that holds it; any other state is reported absent.

@@ -186,8 +186,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
           n[g] = 0;
           if (act[g]) {
             const uint32_t k = bucket_key<NW>(b, key[g]);
-            lo[g] = b.table[k];
-            n[g] = b.table[k + 1] - lo[g];
+            bucket_bounds(b.table, k, lo[g], n[g]);
           }
         }
         // interleaved bisection down to one candidate per key
