@@ -65,6 +65,23 @@ def build_workload(seed=1234, n_threads=16):
     return jd, states, H, g, B, t_enum
 
 
+EIGS64_REF = os.path.join(ROOT, "tests", "golden", "eigs64_bench.json")
+
+
+def cpu_eigs_reference(g, B, D):
+    """The committed CPU eigenvalues of this workload (tests/golden/
+    make_eigs64.py: the oracle's sparse H and ARPACK), when the file is there
+    and was made for the same g, B and basis size; else None."""
+    try:
+        with open(EIGS64_REF) as f:
+            ref = json.load(f)
+    except OSError:
+        return None
+    same = (ref.get("dim") == int(D) and np.array_equal(ref.get("g"), [float(v) for v in g])
+            and np.array_equal(ref.get("B"), [float(v) for v in B]))
+    return ref if same else None
+
+
 def host_cpu():
     """The host's CPU model and logical core count (for the CPU baseline)."""
     model = "unknown"
@@ -593,6 +610,12 @@ def main():
         te = time.perf_counter() - te0
         eigsh = {"nev": 5, "ncv": 15, "tol": 1e-10, "seconds": te, "nconv": nconv,
                  "applies": napp, "evals": [float(v) for v in evals]}
+        ref = cpu_eigs_reference(g, B, D)
+        if ref is not None:
+            w = np.asarray(ref["evals"][:5])
+            eigsh["cpu_reference"] = {
+                "method": ref["method"], "evals": [float(v) for v in w],
+                "max_rel_diff": float(np.max(np.abs(np.asarray(evals) - w) / np.abs(w)))}
 
     # the same workload with the connectivity cached on the device
     cached = None

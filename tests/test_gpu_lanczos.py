@@ -160,6 +160,26 @@ def test_eigsh_48_vs_sparse(lattices, cuda):
     _check_against_dense(evals, w, 5)
 
 
+def test_eigsh_64_bench_vs_cpu(cuda):
+    """The 64-spin bench workload (diag(2,2,4), Q=0, seeded g and B): the five
+    lowest eigenvalues of the GPU solve (nev 5, ncv 15, tol 1e-10) against the
+    CPU reference in tests/golden/eigs64_bench.json (the oracle's sparse H and
+    ARPACK, make_eigs64.py), 1e-9 relative."""
+    import json
+    import bench
+    with open(os.path.join(GOLDEN, "eigs64_bench.json")) as f:
+        ref = json.load(f)
+    jd, states, H, g, B, _ = bench.build_workload(1234)
+    assert states.shape[0] == ref["dim"]
+    assert np.array_equal([float(v) for v in g], ref["g"])
+    assert np.array_equal([float(v) for v in B], ref["B"])
+    print(f"[eigs64] D={states.shape[0]}", flush=True)
+    op = q.GpuOp(q.GpuBasis(states, 64), H)
+    evals, _, nconv, _ = op.eigsh(nev=5, ncv=15, tol=1e-10, seed=0)
+    assert nconv == 5
+    _check_against_dense(evals, np.asarray(ref["evals"]), 5)
+
+
 def test_lanczos_coeffs_match_oracle(lattices, cuda):
     """qsi_lanczos_coeffs: the 3-term recurrence from a given start vector
     against the oracle's (first 25 steps, 1e-10 relative; later steps drift
