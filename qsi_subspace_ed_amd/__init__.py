@@ -6,9 +6,10 @@ Public API:
   load_lattice, build_hamiltonian_g/_jpm,
   add_local_field, greedy_spin_ordering       lattice + Hamiltonian builders
   GpuBasis, GpuOp, Settings, enumerate_basis  GPU objects and solvers
-  ftlm, serve, observables, h5io              batched FTLM, solve engine/server,
-                                              ring observables, HDF5 files
-                                              (submodules)
+  ftlm, serve, observables, h5io, diag        batched FTLM, solve engine/server,
+                                              ring observables, HDF5 files,
+                                              diagonalise_real's algorithm
+                                              dispatch (submodules)
 """
 from .operators import SymbolicOpSum, SymbolicPMROperator  # noqa: F401
 from .lattice import (add_local_field, build_hamiltonian_g,  # noqa: F401

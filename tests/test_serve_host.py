@@ -63,3 +63,13 @@ def test_engine_create_validation():
     assert lib.qsi_engine_create(None, 4, 1, 10, 1e-10, 10, C.byref(h)) == _lib.QSI_E_INVALID
     assert lib.qsi_engine_poll(None, None, None, None, None) == 0
     assert lib.qsi_engine_step(None, None) == _lib.QSI_E_INVALID
+
+
+def test_default_algorithm_by_size():
+    """diagonalise_real's default (include/matrix_diag_bits.hpp:126-136)."""
+    from qsi_subspace_ed_amd import diag
+    assert diag.choose_algorithm(18) == "dense"
+    assert diag.choose_algorithm(99) == "dense"
+    assert diag.choose_algorithm(100) == "sparse"
+    assert diag.choose_algorithm(9_999_999) == "sparse"
+    assert diag.choose_algorithm(10_000_000) == "mfsparse"
