@@ -1,0 +1,78 @@
+"""Edge cases at the boundary: empty and one-state bases, inputs below the
+apply tolerance (APPLY_TOL, components/operator/include/operator_matrix.hpp:
+113), and the reference's errors for an empty vector
+(components/linalg_routines/src/lanczos.cpp: "v called with size 0")."""
+import numpy as np
+import pytest
+
+import qsi_subspace_ed_amd as q
+from conftest import load_basis
+from oracle import pyoracle as orc
+from qsi_subspace_ed_amd._lib import QsiError
+
+pytestmark = pytest.mark.gpu
+
+
+def _ham(jd):
+    H = q.build_hamiltonian_g(jd, [-1.0, -0.7, -0.4, -0.9])
+    q.add_local_field(H, jd, [0.05, -0.03, 0.08])
+    return H
+
+
+def test_empty_basis(lattices, cuda):
+    torch = cuda
+    basis = q.GpuBasis(np.zeros((0, 2), dtype=np.uint64), 8)
+    assert basis.dim() == 0
+    op = q.GpuOp(basis, _ham(lattices[8]))
+    x = torch.zeros(0, dtype=torch.float64, device="cuda")
+    y = torch.zeros(0, dtype=torch.float64, device="cuda")
+    op.evaluate(x, y)
+    op.evaluate_add(x, y)
+    with pytest.raises(QsiError, match="empty basis"):
+        op.eigsh(nev=1, ncv=2)
+    with pytest.raises(QsiError, match="size 0"):
+        op.lanczos_eigval0(q.Settings())
+
+
+def test_one_state_basis(lattices, cuda):
+    """Every flip leaves the basis, so H is the state's diagonal entry."""
+    torch = cuda
+    jd = lattices[8]
+    one = load_basis(8)[5:6]
+    H = _ham(jd)
+    op = q.GpuOp(q.GpuBasis(one, 8), H)
+    x = torch.tensor([0.7], dtype=torch.float64, device="cuda")
+    y = torch.zeros(1, dtype=torch.float64, device="cuda")
+    op.evaluate(x, y)
+    ref = orc.evaluate_add(orc.Basis(one), H, np.array([0.7]), np.zeros(1))
+    assert abs(y.item() - ref[0]) <= 1e-13 * max(1.0, abs(ref[0]))
+    evals, _, nconv, _ = op.eigsh(nev=1, ncv=15)
+    assert nconv == 1
+    assert abs(evals[0] - ref[0] / 0.7) <= 1e-12 * max(1.0, abs(ref[0]))
+
+
+@pytest.mark.parametrize("variant", ["lane4", "window", "cached"])
+def test_inputs_below_apply_tolerance(variant, lattices, cuda):
+    """x at 1e-16 except 50 entries: contributions from |x_j| <= 1e-15 are
+    skipped, so most rows are their diagonal term alone; entry by entry the
+    GPU matches the oracle's push apply (which skips the same sources) to
+    1e-12 relative plus 1e-15 |x_i| + 1e-28 (the rounding left when a row's
+    diagonal terms cancel) — a kept contribution would add at least
+    1e-16 x 0.4."""
+    torch = cuda
+    jd = lattices[32]
+    b = load_basis(32)
+    D = b.shape[0]
+    H = _ham(jd)
+    op = q.GpuOp(q.GpuBasis(b, 32), H)
+    if variant == "cached":
+        op.cache_connectivity(True)
+    op.set_variant(variant)
+    rng = np.random.default_rng(17)
+    x = 1e-16 * np.where(rng.random(D) < 0.5, -1.0, 1.0)
+    x[rng.choice(D, 50, replace=False)] = rng.standard_normal(50)
+    y = torch.zeros(D, dtype=torch.float64, device="cuda")
+    op.evaluate(torch.from_numpy(x).cuda(), y)
+    y = y.cpu().numpy()
+    ref = orc.evaluate_add(orc.Basis(b), H, x, np.zeros(D))
+    assert np.all(np.abs(y - ref) <= 1e-12 * np.abs(ref) + 1e-15 * np.abs(x) + 1e-28)

@@ -42,6 +42,10 @@ def test_version_and_validation():
     assert rc == _lib.QSI_E_INVALID
     rc = lib.qsi_basis_create(None, 1, 0, 0, C.byref(h))
     assert rc == _lib.QSI_E_INVALID
+    # ranks are 32-bit: 2^32 - 1 states or more are refused before any read
+    rc = lib.qsi_basis_create(big.ctypes.data_as(C.c_void_p), (1 << 32) - 1, 8, 0, C.byref(h))
+    assert rc == _lib.QSI_E_INVALID
+    assert b"dim must be" in lib.qsi_last_error()
 
 
 def test_cpp_face_compiles():
