@@ -301,16 +301,18 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_apply(
 // (coefficient class | negative total sign << 7).  A block apply is then a
 // pure gather: no flip tests, no searches.
 
-// counts[i] = number of connected off-diagonal terms of row i.
+// counts[i] = number of connected off-diagonal terms of row i, rounded up to
+// a multiple of `pad` (1 or 4: rows padded to whole groups of four entries).
 template <int NW>
 __global__ __launch_bounds__(256) void k_cc_count(BasisView b, const OffTerm* __restrict__ off,
-                                                  int n_off, uint64_t* __restrict__ counts) {
+                                                  int n_off, int pad,
+                                                  uint64_t* __restrict__ counts) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= b.dim) return;
   const State<NW> st = load_state_cached<NW>(b.states, i);
   uint64_t n = 0;
   for (int t = 0; t < n_off; ++t) n += term_ok<NW>(off[t], st) ? 1u : 0u;
-  counts[i] = n;
+  counts[i] = (n + pad - 1) / pad * pad;
 }
 
 // Entries of row i at rowptr[i]...: ranks looked up kBlkBatch at a time.
@@ -352,6 +354,11 @@ __global__ __launch_bounds__(256) void k_cc_fill(BasisView b, const OffTerm* __r
         }
       }
     }
+  }
+  // padding entries (padded layout): absent, so every apply skips them
+  for (const uint64_t end = rowptr[i + 1]; pos < end; ++pos) {
+    cj[pos] = kNoRank;
+    cm[pos] = 0;
   }
 }
 
@@ -795,27 +802,54 @@ int op_cache_connectivity(qsi_op_s* op, bool enable) {
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   hipError_t e = hipMalloc(&op->d_rowptr, static_cast<size_t>(D + 1) * sizeof(uint64_t));
-  if (e == hipSuccess) e = hipMemset(op->d_rowptr, 0, static_cast<size_t>(D + 1) * sizeof(uint64_t));
-  if (e == hipSuccess && D > 0) {
-    if (op->basis->nw == 1)
-      hipLaunchKernelGGL(k_cc_count<1>, dim3(grid), dim3(256), 0, 0, view, op->d_off, op->n_off,
-                         op->d_rowptr);
-    else
-      hipLaunchKernelGGL(k_cc_count<2>, dim3(grid), dim3(256), 0, 0, view, op->d_off, op->n_off,
-                         op->d_rowptr);
-    e = hipGetLastError();
+  // row offsets: counts (rounded up to multiples of pad), then an in-place
+  // exclusive scan of the D + 1 values (the last count is 0); returns the total
+  auto offsets = [&](int pad, uint64_t* total) -> hipError_t {
+    hipError_t r = hipMemset(op->d_rowptr, 0, static_cast<size_t>(D + 1) * sizeof(uint64_t));
+    if (r == hipSuccess && D > 0) {
+      if (op->basis->nw == 1)
+        hipLaunchKernelGGL(k_cc_count<1>, dim3(grid), dim3(256), 0, 0, view, op->d_off, op->n_off,
+                           pad, op->d_rowptr);
+      else
+        hipLaunchKernelGGL(k_cc_count<2>, dim3(grid), dim3(256), 0, 0, view, op->d_off, op->n_off,
+                           pad, op->d_rowptr);
+      r = hipGetLastError();
+    }
+    if (r == hipSuccess && !tmp) {
+      r = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, op->d_rowptr, op->d_rowptr, D + 1);
+      if (r == hipSuccess) r = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
+    }
+    if (r == hipSuccess)
+      r = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, op->d_rowptr, op->d_rowptr, D + 1);
+    if (r == hipSuccess)
+      r = hipMemcpy(total, op->d_rowptr + D, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    return r;
+  };
+  uint64_t nnz = 0, slots = 0;
+  if (e == hipSuccess) e = offsets(1, &nnz);
+  // Rows padded to whole groups of four entries let the cached apply read
+  // every group with one aligned load (about 15 % faster at 64 spins); the
+  // padding (at most 3 entries, 15 bytes, per row) is spent only when the
+  // padded arrays leave a quarter of the free memory (at least 1 GiB) unused.
+  if (e == hipSuccess) {
+    size_t free_b = 0, total_b = 0;
+    const size_t padded = (nnz + 3 * static_cast<size_t>(D)) * 5;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+        free_b > padded + std::max<size_t>(free_b / 4, size_t{1} << 30)) {
+      e = offsets(4, &slots);
+      op->conn_pad = 4;
+    } else {
+      slots = nnz;
+      op->conn_pad = 1;
+    }
   }
-  // in-place exclusive scan of the D + 1 counts (the last one is 0)
-  if (e == hipSuccess)
-    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, op->d_rowptr, op->d_rowptr, D + 1);
-  if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1));
-  if (e == hipSuccess)
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, op->d_rowptr, op->d_rowptr, D + 1);
-  uint64_t nnz = 0;
-  if (e == hipSuccess) e = hipMemcpy(&nnz, op->d_rowptr + D, sizeof(uint64_t), hipMemcpyDeviceToHost);
   if (tmp) (void)hipFree(tmp);
-  if (e == hipSuccess) e = hipMalloc(&op->d_cj, std::max<size_t>(nnz, 1) * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&op->d_cm, std::max<size_t>(nnz, 1));
+  // rounded up to whole 16-byte groups of columns (the cached apply reads the
+  // columns four at a time from 16-byte aligned addresses, and the class
+  // bytes four at a time)
+  const size_t cap = std::max<size_t>((slots + 3) & ~uint64_t{3}, 4);
+  if (e == hipSuccess) e = hipMalloc(&op->d_cj, cap * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&op->d_cm, cap);
   if (e == hipSuccess && D > 0) {
     if (op->basis->nw == 1)
       hipLaunchKernelGGL(k_cc_fill<1>, dim3(grid), dim3(256), 0, 0, view, op->d_off, op->n_off,

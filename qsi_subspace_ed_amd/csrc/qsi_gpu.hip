@@ -246,7 +246,10 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
 // connectivity (qsi_op_cache_connectivity), so an apply is a gather with no
 // flip tests or searches.  Same terms, order and arithmetic as the lane
 // variant, so y is bit-identical.
-template <int NW, bool DOT, int PIPE>
+//
+// ALIGNED: the rows are padded to whole groups of four (conn_pad 4), so every
+// group the loop loads belongs to the row and no entry needs masking.
+template <int NW, bool DOT, int PIPE, bool ALIGNED>
 __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
     BasisView b, const CoefClass* __restrict__ cls, int n_cls,
     const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ cj,
@@ -267,15 +270,28 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
     // latency overlaps theirs
     const State<NW> si = load_state_cached<NW>(b.states, i);
     const uint64_t e0 = rowptr[i], e1 = rowptr[i + 1];
-    for (uint64_t e = e0; e < e1; e += PIPE) {
+    // columns and class bytes come four at a time from 16- and 4-byte aligned
+    // groups (the arrays are allocated in whole groups); without padding,
+    // entries of the neighbouring rows in the first and last group are masked
+    // off
+    for (uint64_t e = ALIGNED ? e0 : (e0 & ~uint64_t{3}); e < e1; e += PIPE) {
       uint32_t J[PIPE];
       uint8_t mt[PIPE];
       double xj[PIPE];
 #pragma unroll
-      for (int u = 0; u < PIPE; ++u) {
-        const bool in = e + u < e1;
-        J[u] = in ? cj[e + u] : 0xFFFFFFFFu;
-        mt[u] = in ? cm[e + u] : 0;
+      for (int u = 0; u < PIPE; u += 4) {
+        const uint64_t g = e + u;
+        const bool in = g < e1;
+        const uint4 j4 = in ? reinterpret_cast<const uint4*>(cj)[g >> 2]
+                           : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        const uint32_t m4 = in ? reinterpret_cast<const uint32_t*>(cm)[g >> 2] : 0u;
+        const uint32_t jv[4] = {j4.x, j4.y, j4.z, j4.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const bool mine = ALIGNED || (g + k >= e0 && g + k < e1);
+          J[u + k] = mine ? jv[k] : 0xFFFFFFFFu;
+          mt[u + k] = static_cast<uint8_t>(m4 >> (8 * k));
+        }
       }
 #pragma unroll
       for (int u = 0; u < PIPE; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] : 0.0;
@@ -821,6 +837,18 @@ int64_t apply_num_blocks(const qsi_op_s* op) {
   return (op->basis->dim + kApplyBlock - 1) / kApplyBlock;
 }
 
+template <int NW, bool DOT, bool ALIGNED>
+static void launch_cached_t(const qsi_op_s* op, dim3 grid, dim3 block, const BasisView& v,
+                            const double* x, double* y, const double* y_in, double yscale,
+                            const double* yscale_dev, double* partials, hipStream_t s) {
+  auto k = op->cached_pipe == 32 ? k_apply_cached<NW, DOT, 32, ALIGNED>
+         : op->cached_pipe == 4  ? k_apply_cached<NW, DOT, 4, ALIGNED>
+         : op->cached_pipe == 16 ? k_apply_cached<NW, DOT, 16, ALIGNED>
+                                 : k_apply_cached<NW, DOT, 8, ALIGNED>;
+  hipLaunchKernelGGL(k, grid, block, 0, s, v, op->d_cls, op->n_cls, op->d_rowptr, op->d_cj,
+                     op->d_cm, op->d_diag, op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+}
+
 template <int NW, bool HAS_Z, bool DOT>
 static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
                            const double* y_in, double yscale,
@@ -835,22 +863,12 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   switch (variant) {
     case QSI_APPLY_CACHED:
       // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)
-      if (op->cached_pipe == 32)
-        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 32>), grid, block, 0, s, v, op->d_cls,
-                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
-                           x, y, y_in, yscale, yscale_dev, partials);
-      else if (op->cached_pipe == 4)
-        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 4>), grid, block, 0, s, v, op->d_cls,
-                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
-                           x, y, y_in, yscale, yscale_dev, partials);
-      else if (op->cached_pipe == 16)
-        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 16>), grid, block, 0, s, v, op->d_cls,
-                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
-                           x, y, y_in, yscale, yscale_dev, partials);
+      if (op->conn_pad == 4)
+        launch_cached_t<NW, DOT, true>(op, grid, block, v, x, y, y_in, yscale, yscale_dev,
+                                       partials, s);
       else
-        hipLaunchKernelGGL((k_apply_cached<NW, DOT, 8>), grid, block, 0, s, v, op->d_cls,
-                           op->n_cls, op->d_rowptr, op->d_cj, op->d_cm, op->d_diag, op->n_diag,
-                           x, y, y_in, yscale, yscale_dev, partials);
+        launch_cached_t<NW, DOT, false>(op, grid, block, v, x, y, y_in, yscale, yscale_dev,
+                                        partials, s);
       break;
     case QSI_APPLY_LANE:
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,

@@ -111,7 +111,8 @@ struct qsi_op_s {
   uint64_t* d_rowptr = nullptr;
   uint32_t* d_cj = nullptr;
   uint8_t* d_cm = nullptr;
-  int64_t conn_nnz = -1;  // -1: not cached
+  int64_t conn_nnz = -1;  // -1: not cached (connected terms; absent ones included)
+  int conn_pad = 1;       // rows padded to multiples of conn_pad entries (1 or 4)
   int blk_cfg = 1;        // block-apply configuration (qsi_op_autotune)
   int cached_pipe = 8;    // gathers in flight per lane of the cached apply (4-32)
 };
