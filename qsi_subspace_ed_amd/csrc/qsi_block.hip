@@ -362,6 +362,32 @@ __global__ __launch_bounds__(256) void k_cc_fill(BasisView b, const OffTerm* __r
   }
 }
 
+// PIPE entries of a row's cached connectivity from entry e (a multiple of 4):
+// columns four at a time with one 16-byte load, class bytes with one 4-byte
+// load (the arrays are allocated in whole groups); entries outside the row
+// [e0, e1) read as absent.
+template <int PIPE>
+__device__ __forceinline__ void load_entries(const BlkSector& S, uint64_t e, uint64_t e0,
+                                             uint64_t e1, uint32_t (&J)[PIPE],
+                                             uint8_t (&mt)[PIPE]) {
+  static_assert(PIPE % 4 == 0, "groups of four");
+#pragma unroll
+  for (int u = 0; u < PIPE; u += 4) {
+    const uint64_t g = e + u;
+    const bool in = g < e1;
+    const uint4 j4 = in ? reinterpret_cast<const uint4*>(S.cj)[g >> 2]
+                       : make_uint4(kNoRank, kNoRank, kNoRank, kNoRank);
+    const uint32_t m4 = in ? reinterpret_cast<const uint32_t*>(S.cm)[g >> 2] : 0u;
+    const uint32_t jv[4] = {j4.x, j4.y, j4.z, j4.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool mine = g + k >= e0 && g + k < e1;
+      J[u + k] = mine ? jv[k] : kNoRank;
+      mt[u + k] = static_cast<uint8_t>(m4 >> (8 * k));
+    }
+  }
+}
+
 // The block apply over cached connectivity: same arithmetic, order and
 // results as k_blk_apply.  G lanes per row (one per column), PIPE entries'
 // gathers in flight.  G may be any width up to 64 when no partials are asked
@@ -417,16 +443,11 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply(
       const int64_t gi = a.y_colmajor ? col * a.ldy + row : row * a.ldy + col;
       double acc = a.accumulate ? Y[gi] : 0.0;
       const uint64_t e0 = S.rowptr[i], e1 = S.rowptr[i + 1];
-      for (uint64_t e = e0; e < e1; e += PIPE) {
+      for (uint64_t e = e0 & ~uint64_t{3}; e < e1; e += PIPE) {
         uint32_t J[PIPE];
         uint8_t mt[PIPE];
         double xj[PIPE];
-#pragma unroll
-        for (int u = 0; u < PIPE; ++u) {
-          const bool in = e + u < e1;
-          J[u] = in ? S.cj[e + u] : kNoRank;
-          mt[u] = in ? S.cm[e + u] : 0;
-        }
+        load_entries<PIPE>(S, e, e0, e1, J, mt);
 #pragma unroll
         for (int u = 0; u < PIPE; ++u)
           xj[u] = J[u] != kNoRank ? X[(S.row_off + static_cast<int64_t>(J[u])) * a.ldx + col] : 0.0;
@@ -530,16 +551,11 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply2(
       double acc0 = a.accumulate ? Y[g0] : 0.0;
       double acc1 = a.accumulate && ok1 ? Y[g1] : 0.0;
       const uint64_t e0 = S.rowptr[i], e1 = S.rowptr[i + 1];
-      for (uint64_t e = e0; e < e1; e += PIPE) {
+      for (uint64_t e = e0 & ~uint64_t{3}; e < e1; e += PIPE) {
         uint32_t J[PIPE];
         uint8_t mt[PIPE];
         double2 xj[PIPE];
-#pragma unroll
-        for (int u = 0; u < PIPE; ++u) {
-          const bool in = e + u < e1;
-          J[u] = in ? S.cj[e + u] : kNoRank;
-          mt[u] = in ? S.cm[e + u] : 0;
-        }
+        load_entries<PIPE>(S, e, e0, e1, J, mt);
 #pragma unroll
         for (int u = 0; u < PIPE; ++u)
           xj[u] = J[u] != kNoRank
