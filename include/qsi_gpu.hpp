@@ -357,6 +357,8 @@ class FtlmComputer {
     return t;
   }
   int64_t n_samples() const { return n_samples_; }
+  int krylov_dim() const { return krylov_dim_; }
+  double tolerance() const { return tol_; }
   double energy_shift() const { return e_ref_; }
   // Z(beta) exp(beta e_ref): (D / n) sum_r Z_r
   std::vector<double> partition_function_shifted() const {
