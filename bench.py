@@ -43,6 +43,33 @@ def measured_traffic(path=PMC_SUMMARY):
         return None
 
 
+def stream_copy_ceiling(gib=2, reps=10):
+    """Measured HBM ceiling for a plain streaming kernel on this device:
+    torch's device copy of a `gib` GiB FP64 buffer, read + write bytes per
+    second (HIP events), beside the 8 TB/s published peak."""
+    import torch
+    n = (gib << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    gbps = 2 * n * 8 / sec / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GBps": gbps, "frac_of_peak": gbps / HBM_PEAK_GBS,
+            "method": f"torch copy_ of a {gib} GiB FP64 buffer, read + write bytes, "
+                      f"mean of {reps}"}
+
+
 def cached_kernel_bytes(D, nnz, S):
     """Bytes the cached-connectivity apply itself moves: per row the state
     (diagonal), x_i, two row offsets (one shared) and y_i; per nonzero the
@@ -488,6 +515,7 @@ def bench_96(args):
         "cached_connectivity": cached,
         "algorithmic_bytes_per_apply": int(bytes_apply),
         "lanczos": lz,
+        "stream_copy_ceiling": stream_copy_ceiling(),
     }
     if not args.no_cpu:
         states = basis.states()
@@ -711,6 +739,7 @@ def main():
         "cached_connectivity": cached,
         "ftlm_batched": ftlm_res,
         "served": serve_res,
+        "stream_copy_ceiling": stream_copy_ceiling(),
     }
     if not args.no_cpu:
         cb, y_cpu = cpu_baseline(states, H, xh)
