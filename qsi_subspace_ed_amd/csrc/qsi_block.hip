@@ -632,13 +632,31 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
     const double al = col_ok && alpha ? alpha[s * R + col] : 0.0;
     const double be = col_ok && A ? beta[s * R + col] : 0.0;
     double part = 0.0;
-    for (int64_t i = r_begin + grp; i < r_end; i += rows_per_pass) {
+    // four rows per round: their loads are issued together; the norm is
+    // still accumulated row by row in order
+    constexpr int kU = 4;
+    for (int64_t i0 = r_begin + grp; i0 < r_end; i0 += kU * rows_per_pass) {
       if (!col_ok) continue;
-      const int64_t gi = (S.row_off + i) * R + col;
-      double w = C[gi] - al * B[gi];
-      if (A) w -= be * A[gi];
-      C[gi] = w;
-      part += w * w;
+      double cv[kU], bv[kU], av[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = i0 + u * rows_per_pass;
+        const int64_t gi = (S.row_off + i) * R + col;
+        const bool in = i < r_end;
+        cv[u] = in ? C[gi] : 0.0;
+        bv[u] = in ? B[gi] : 0.0;
+        av[u] = in && A ? A[gi] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = i0 + u * rows_per_pass;
+        if (i >= r_end) continue;
+        const int64_t gi = (S.row_off + i) * R + col;
+        double w = cv[u] - al * bv[u];
+        if (A) w -= be * av[u];
+        C[gi] = w;
+        part += w * w;
+      }
     }
     for (int k = 0; k < kMaxSectors; ++k)
       if (k == s) nrm[k] += part;
@@ -692,8 +710,20 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_scale(
     const int64_t r_end = min<int64_t>(r_begin + kTileRows, S.b.dim);
     if (!col_ok) continue;
     const double f = invb[s * R + col];
-    for (int64_t i = r_begin + grp; i < r_end; i += rows_per_pass)
-      C[(S.row_off + i) * R + col] *= f;
+    constexpr int kU = 4;
+    for (int64_t i0 = r_begin + grp; i0 < r_end; i0 += kU * rows_per_pass) {
+      double cv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = i0 + u * rows_per_pass;
+        cv[u] = i < r_end ? C[(S.row_off + i) * R + col] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = i0 + u * rows_per_pass;
+        if (i < r_end) C[(S.row_off + i) * R + col] = cv[u] * f;
+      }
+    }
   }
   if (step && blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
 }
