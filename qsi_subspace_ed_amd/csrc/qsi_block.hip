@@ -632,9 +632,9 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
     const double al = col_ok && alpha ? alpha[s * R + col] : 0.0;
     const double be = col_ok && A ? beta[s * R + col] : 0.0;
     double part = 0.0;
-    // four rows per round: their loads are issued together; the norm is
+    // eight rows per round: their loads are issued together; the norm is
     // still accumulated row by row in order
-    constexpr int kU = 4;
+    constexpr int kU = 8;
     for (int64_t i0 = r_begin + grp; i0 < r_end; i0 += kU * rows_per_pass) {
       if (!col_ok) continue;
       double cv[kU], bv[kU], av[kU];
@@ -710,7 +710,7 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_scale(
     const int64_t r_end = min<int64_t>(r_begin + kTileRows, S.b.dim);
     if (!col_ok) continue;
     const double f = invb[s * R + col];
-    constexpr int kU = 4;
+    constexpr int kU = 8;
     for (int64_t i0 = r_begin + grp; i0 < r_end; i0 += kU * rows_per_pass) {
       double cv[kU];
 #pragma unroll
