@@ -645,13 +645,29 @@ inline int vec_blocks(int64_t n) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nb, 8192)));
 }
 
+// The grid-stride dot, axpy-norm and device-scalar scale kernels take
+// kVecUnroll of a thread's elements per round, all loads first, and
+// accumulate in element order (the sums are the ones a one-element loop
+// gives).
+constexpr int kVecUnroll = 8;
+
 __global__ __launch_bounds__(kVecBlock) void k_dot_partial(
     const double* __restrict__ a, const double* __restrict__ b, int64_t n,
     double* __restrict__ partials) {
   double v = 0.0;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
-       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
-    v += a[i] * b[i];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBlock;
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+  for (; i + (kVecUnroll - 1) * stride < n; i += kVecUnroll * stride) {
+    double av[kVecUnroll], bv[kVecUnroll];
+#pragma unroll
+    for (int k = 0; k < kVecUnroll; ++k) {
+      av[k] = a[i + k * stride];
+      bv[k] = b[i + k * stride];
+    }
+#pragma unroll
+    for (int k = 0; k < kVecUnroll; ++k) v += av[k] * bv[k];
+  }
+  for (; i < n; i += stride) v += a[i] * b[i];
   v = block_sum<kVecBlock>(v);
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
@@ -679,8 +695,23 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
     double* __restrict__ u, int64_t n, double* __restrict__ partials) {
   const double a = alpha[0];
   double acc = 0.0;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
-       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBlock;
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+  for (; i + (kVecUnroll - 1) * stride < n; i += kVecUnroll * stride) {
+    double uv[kVecUnroll], vv[kVecUnroll];
+#pragma unroll
+    for (int k = 0; k < kVecUnroll; ++k) {
+      uv[k] = u[i + k * stride];
+      vv[k] = v[i + k * stride];
+    }
+#pragma unroll
+    for (int k = 0; k < kVecUnroll; ++k) {
+      const double w = uv[k] - a * vv[k];
+      u[i + k * stride] = w;
+      acc += w * w;
+    }
+  }
+  for (; i < n; i += stride) {
     const double w = u[i] - a * v[i];
     u[i] = w;
     acc += w * w;
@@ -693,9 +724,16 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
 __global__ __launch_bounds__(kVecBlock) void k_scal_dev(
     const double* __restrict__ s, double* __restrict__ x, int64_t n) {
   const double a = s[0];
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
-       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
-    x[i] *= a;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBlock;
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+  for (; i + (kVecUnroll - 1) * stride < n; i += kVecUnroll * stride) {
+    double xv[kVecUnroll];
+#pragma unroll
+    for (int k = 0; k < kVecUnroll; ++k) xv[k] = x[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < kVecUnroll; ++k) x[i + k * stride] = xv[k] * a;
+  }
+  for (; i < n; i += stride) x[i] *= a;
 }
 
 // ---- tall-skinny kernels for full reorthogonalisation ----------------------
