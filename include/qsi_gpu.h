@@ -172,11 +172,15 @@ int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
 
 /* Keep (enable = 1) or drop (0) the operator's connectivity on the device:
  * for every row, in term order, the rank of each connected state and the
- * term's coefficient class and sign (5 bytes per nonzero plus 8 per row).
- * Block applies (qsi_op_apply_block, qsi_lanczos_block, qsi_engine_*) then
- * gather through it instead of testing and searching, with identical
- * results; worth it when the same operator is applied many times.  *nnz
- * (optional) receives the cached nonzero count (-1 when not cached).
+ * term's coefficient class and sign (5 bytes per nonzero plus 8 per row;
+ * when the padded arrays leave a quarter of the free device memory unused,
+ * rows are padded to multiples of four entries, up to 15 more bytes per
+ * row, which the applies read one aligned group at a time).  The
+ * single-vector apply (QSI_APPLY_CACHED) and the block applies
+ * (qsi_op_apply_block, qsi_lanczos_block, qsi_engine_*) then gather through
+ * it instead of testing and searching, with identical results; worth it
+ * when the same operator is applied many times.  *nnz (optional) receives
+ * the cached nonzero count, padding excluded (-1 when not cached).
  * Requires at most 8 distinct (slot, coeff) pairs among the off-diagonal
  * terms.  Synchronises.  The counterpart of the reference's `sparse`
  * algorithm, which materialises H (operator_matrix.hpp:79-105) instead. */
