@@ -76,9 +76,10 @@ def dense_matrix(op: GpuOp):
     check(lib.qsi_op_connectivity(op.handle, C.byref(nnz), r.data_ptr(), c.data_ptr(),
                                   v.data_ptr(), d.data_ptr()), "qsi_op_connectivity")
     k = nnz.value
-    M = torch.diag(d[:D].clone())
-    M.index_put_((r[:k], c[:k]), v[:k], accumulate=True)
-    return M
+    # duplicate (row, col) pairs (two terms reaching the same state) are
+    # summed by coalescing (sorted, a fixed order), not by atomic adds
+    off = torch.sparse_coo_tensor(torch.stack((r[:k], c[:k])), v[:k], (D, D)).coalesce()
+    return off.to_dense() + torch.diag(d[:D])
 
 
 def diagonalise_real(op: GpuOp, opts: Options = Options()):
