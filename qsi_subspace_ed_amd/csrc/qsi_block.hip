@@ -9,14 +9,15 @@
 // and one persistent grid of wavefronts walks the concatenated tile list, so
 // each basis state's flip tests and rank lookups serve all R columns.
 //
-// Per step (three rotating D x R blocks A = v_{j-1}, B = v_j, C):
+// Per step (three rotating D x R blocks holding v_{j-1}, v_j unnormalised,
+// A = w_{j-1}, B = w_j with v = scale * column, and C):
 //   k_blk_apply      C = H B, per-wave partials of <B_c, C_c>
-//   k_blk_alpha      alpha[s][c]
-//   k_blk_update     C -= alpha B + beta A, per-wave partials of |C_c|^2
-//   k_blk_beta       beta[s][c], 1/beta (0 when beta underflows)
-//   k_blk_scale      C *= 1/beta;  then (A, B, C) <- (B, C, A)
-// The five launches of a step are captured once per rotation phase as a HIP
-// graph and replayed.
+//   k_blk_finalize   alpha[s][c] = scale_B^2 <B_c, C_c>
+//   k_blk_update     C = scale_B (C - alpha B) - beta scale_A A, partials of |C_c|^2
+//   k_blk_finalize   beta[s][c] = |C_c|, scale_C = 1/beta (0 when beta underflows)
+//   k_blk_step       history index += 1;  then (A, B, C) <- (B, C, A)
+// No pass rewrites a block to normalise it.  The five launches of a step are
+// captured once per rotation phase as a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -609,11 +610,14 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_capply2(
   flush_to(a.n_sec, cur);
 }
 
-// C -= alpha B + beta A (beta, A skipped when A is null); |C_c|^2 partials.
+// C = fb C - alpha fb B - beta fa A per column (fb, fa: the scales sB, sA of
+// the unnormalised blocks B and A, 1 when null; beta, A skipped when A is
+// null); |C_c|^2 partials.
 __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
     BlkArgs a, const double* __restrict__ A, const double* B,
     double* C, const double* __restrict__ alpha,
-    const double* __restrict__ beta, double* __restrict__ partials) {
+    const double* __restrict__ beta, double* __restrict__ partials,
+    const double* __restrict__ sA, const double* __restrict__ sB) {
   const int lane = threadIdx.x & 63;
   const int G = a.G, R = a.R;
   const int grp = lane / G, col = lane % G;
@@ -631,6 +635,11 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
     const int64_t r_end = min<int64_t>(r_begin + kTileRows, S.b.dim);
     const double al = col_ok && alpha ? alpha[s * R + col] : 0.0;
     const double be = col_ok && A ? beta[s * R + col] : 0.0;
+    // the blocks hold unnormalised vectors: v_B = fb B, v_A = fa A, and C
+    // holds H B, so H v_B = fb C
+    const double fb = col_ok && sB ? sB[s * R + col] : 1.0;
+    const double fa = col_ok && sA ? sA[s * R + col] : 1.0;
+    const double alb = al * fb, bea = be * fa;
     double part = 0.0;
     // eight rows per round: their loads are issued together; the norm is
     // still accumulated row by row in order
@@ -652,8 +661,9 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
         const int64_t i = i0 + u * rows_per_pass;
         if (i >= r_end) continue;
         const int64_t gi = (S.row_off + i) * R + col;
-        double w = cv[u] - al * bv[u];
-        if (A) w -= be * av[u];
+        double w = fb * cv[u] - alb * bv[u];
+        if (A) w -= bea * av[u];
+        if (fb == 0.0) w = 0.0;  // a column whose beta underflowed stays zero
         C[gi] = w;
         part += w * w;
       }
@@ -668,13 +678,13 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_update(
 }
 
 // One workgroup per (sector, column): fixed-order sum of the wave partials.
-// mode 0 (alpha): out[s*R+c] = sum.  mode 1 (beta): out[s*R+c] = sqrt(sum),
+// mode 0 (alpha): out[s*R+c] = sum, times sq[s*R+c]^2 when sq is given.  mode 1 (beta): out[s*R+c] = sqrt(sum),
 // out2[s*R+c] = 1/beta or 0.  hist (optional) receives the value at
 // hist[(s*R + c)*M + step[0]].
 __global__ __launch_bounds__(256) void k_blk_finalize(
     const double* __restrict__ partials, int64_t n_waves, int R, int mode,
     double* __restrict__ out, double* __restrict__ out2, double* __restrict__ hist,
-    const int* __restrict__ step, int M) {
+    const int* __restrict__ step, int M, const double* __restrict__ sq) {
   const int sc = blockIdx.x;  // s * R + c
   const int s = sc / R, c = sc % R;
   double v = 0.0;
@@ -683,6 +693,7 @@ __global__ __launch_bounds__(256) void k_blk_finalize(
   v = block_sum<256>(v);
   if (threadIdx.x == 0) {
     double r = v;
+    if (mode == 0 && sq) r = r * sq[sc] * sq[sc];
     if (mode == 1) {
       r = sqrt(v);
       out2[sc] = r > 1e-300 ? 1.0 / r : 0.0;
@@ -690,6 +701,11 @@ __global__ __launch_bounds__(256) void k_blk_finalize(
     out[sc] = r;
     if (hist) hist[static_cast<int64_t>(sc) * M + step[0]] = r;
   }
+}
+
+// Advances the step counter the coefficient history is indexed by.
+__global__ void k_blk_step(int* __restrict__ step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
 }
 
 // C *= 1/beta per (sector, column); optionally advances the step counter.
@@ -1020,35 +1036,56 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
   // normalise the start columns: B = B / |B_c|
   hipLaunchKernelGGL(k_blk_update, dim3(n_blocks), dim3(kBlkThreads), 0, st, args,
                      static_cast<const double*>(nullptr), B, B, static_cast<const double*>(nullptr),
-                     static_cast<const double*>(nullptr), part);
+                     static_cast<const double*>(nullptr), part, static_cast<const double*>(nullptr),
+                     static_cast<const double*>(nullptr));
   QSI_CHECK_LAUNCH();
   // (C = B - 0*B: the kernel writes back B unchanged and produces |B_c|^2)
   hipLaunchKernelGGL(k_blk_finalize, dim3(SR), dim3(256), 0, st, part, n_waves, R, 1,
-                     beta, invb, static_cast<double*>(nullptr), static_cast<const int*>(nullptr), M);
+                     beta, invb, static_cast<double*>(nullptr), static_cast<const int*>(nullptr), M,
+                     static_cast<const double*>(nullptr));
   QSI_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_blk_scale, dim3(n_blocks), dim3(kBlkThreads), 0, st, args, B, invb,
                      static_cast<int*>(nullptr));
   QSI_CHECK_LAUNCH();
   QSI_CHECK_HIP(hipMemsetAsync(beta, 0, SR * sizeof(double), st));
 
-  auto enqueue_step = [&](double* a, double* b, double* c) -> int {
+  // The blocks hold the Lanczos vectors unnormalised, each column with its
+  // scale (v = scale * column) in scl[k] beside bufs[k]: the step applies H to
+  // the unnormalised B, folds B's scale into alpha and the update, and the
+  // new column's scale is 1/beta — no pass rewrites a block to normalise it.
+  // B starts normalised (scale 1); A is zero.
+  double* bufs[3] = {A, B, C};
+  double* scl[3] = {nullptr, nullptr, nullptr};
+  for (int k = 0; k < 3; ++k) {
+    QSI_CHECK_HIP(mem.alloc(&scl[k], SR));
+    std::vector<double> ones(static_cast<size_t>(SR), 1.0);
+    QSI_CHECK_HIP(hipMemcpyAsync(scl[k], ones.data(), SR * sizeof(double),
+                                 hipMemcpyHostToDevice, st));
+    QSI_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  auto enqueue_step = [&](int p) -> int {
+    double* a = bufs[p % 3];
+    double* b = bufs[(p + 1) % 3];
+    double* c = bufs[(p + 2) % 3];
+    const double* sa = scl[p % 3];
+    const double* sb = scl[(p + 1) % 3];
+    double* sc = scl[(p + 2) % 3];
     const int rc = launch_blk_apply(args, nw, n_blocks, b, c, part, st);
     if (rc != QSI_OK) return rc;
     hipLaunchKernelGGL(k_blk_finalize, dim3(SR), dim3(256), 0, st, part, n_waves, R, 0,
-                       alpha, static_cast<double*>(nullptr), ha, step, M);
+                       alpha, static_cast<double*>(nullptr), ha, step, M, sb);
     QSI_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_blk_update, dim3(n_blocks), dim3(kBlkThreads), 0, st, args, a, b, c,
-                       alpha, beta, part);
+                       alpha, beta, part, sa, sb);
     QSI_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_blk_finalize, dim3(SR), dim3(256), 0, st, part, n_waves, R, 1,
-                       beta, invb, hb, step, M);
+                       beta, sc, hb, step, M, static_cast<const double*>(nullptr));
     QSI_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_blk_scale, dim3(n_blocks), dim3(kBlkThreads), 0, st, args, c, invb, step);
+    hipLaunchKernelGGL(k_blk_step, dim3(1), dim3(64), 0, st, step);
     QSI_CHECK_LAUNCH();
     return QSI_OK;
   };
 
-  double* bufs[3] = {A, B, C};
   hipEvent_t e0, e1;
   QSI_CHECK_HIP(hipEventCreate(&e0));
   QSI_CHECK_HIP(hipEventCreate(&e1));
@@ -1072,7 +1109,7 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
     for (int p = 0; p < 3 && p < M; ++p) {
       hipGraph_t g = nullptr;
       QSI_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_step(bufs[p % 3], bufs[(p + 1) % 3], bufs[(p + 2) % 3]);
+      const int rc = enqueue_step(p);
       hipGraph_t g2 = nullptr;
       const hipError_t ec = hipStreamEndCapture(st, &g2);
       g = g2;
@@ -1093,7 +1130,7 @@ extern "C" int qsi_lanczos_block(const qsi_op_t* ops, int n_sectors, int R,
     QSI_CHECK_HIP(hipEventRecord(e0, st));
     for (int j = 0; j < M; ++j) {
       const int p = j % 3;
-      const int rc = enqueue_step(bufs[p], bufs[(p + 1) % 3], bufs[(p + 2) % 3]);
+      const int rc = enqueue_step(p);
       if (rc != QSI_OK) return rc;
     }
     QSI_CHECK_HIP(hipEventRecord(e1, st));
