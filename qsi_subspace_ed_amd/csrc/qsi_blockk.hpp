@@ -39,7 +39,9 @@ struct BlkArgs {
   int n_sec;
   int64_t n_tiles;
   int R;
-  int G;            // lanes per row group (power of two >= R)
+  int G;            // lanes per row group: a power of two >= R, or any width
+                    // >= R for the cached kernel without partials (the
+                    // column-pairs kernel sets its own, ceil(R / 2))
   int64_t ldx, ldy; // row strides of the X and Y blocks (>= R)
   int accumulate;   // Y += H X instead of Y = H X (apply kernel)
   int y_colmajor;   // Y stored column-major: Y[col * ldy + row] (apply kernel)
