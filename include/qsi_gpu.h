@@ -128,7 +128,7 @@ int qsi_op_n_diag(qsi_op_t op);
  * connected rows search a 64-state window staged in LDS.  QSI_APPLY_CACHED:
  * gathers through the cached connectivity (qsi_op_cache_connectivity; setting
  * it requires the cache, dropping the cache resets it to AUTO).  AUTO =
- * CACHED when the connectivity is cached, else LANE8 (the fastest search
+ * CACHED when the connectivity is cached, else LANE4 (the fastest search
  * measured on MI355X: the sorted-basis bisection keeps the lookups of
  * neighbouring rows in the same cache lines, which the hash index cannot).
  * All give bit-identical y. */

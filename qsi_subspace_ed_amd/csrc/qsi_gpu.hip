@@ -897,7 +897,7 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   const auto v = op->basis->view();
   int variant = op->variant;
   if (variant == QSI_APPLY_AUTO)
-    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE8;
+    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE4;
   switch (variant) {
     case QSI_APPLY_CACHED:
       // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)

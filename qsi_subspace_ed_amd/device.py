@@ -190,10 +190,11 @@ class GpuOp:
         return self._h
 
     def set_variant(self, variant):
-        """Choose the apply kernel: 'auto' (= 'lane8'), 'hash', 'hash4'
-        (hash-index lookups; the basis needs hash_index=True),
-        'lane', 'lane2', 'lane4', 'lane8' (bucket-table lookups, 1-8 in flight
-        per lane) or 'window' (LDS-staged bucket windows per wavefront)."""
+        """Choose the apply kernel: 'auto' ('cached' when the connectivity is
+        cached, else 'lane4'), 'cached', 'hash', 'hash4' (hash-index
+        lookups; the basis needs hash_index=True), 'lane', 'lane2', 'lane4',
+        'lane8' (bucket-table lookups, 1-8 in flight per lane) or 'window'
+        (LDS-staged bucket windows per wavefront)."""
         v = _lib.VARIANTS[variant]
         check(_lib.load().qsi_op_set_variant(self._h, v), "qsi_op_set_variant")
 
