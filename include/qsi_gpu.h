@@ -181,10 +181,14 @@ int qsi_op_apply_block(qsi_op_t op, const double *d_X, int64_t ldx,
  * it instead of testing and searching, with identical results; worth it
  * when the same operator is applied many times.  *nnz (optional) receives
  * the cached nonzero count, padding excluded (-1 when not cached).
+ * enable = 2 keeps it with rows unpadded whatever the free memory.  An
+ * existing cache keeps its layout (drop it first to rebuild);
+ * qsi_op_connectivity_padding reports it: 4 (padded), 1, or -1 (none).
  * Requires at most 8 distinct (slot, coeff) pairs among the off-diagonal
  * terms.  Synchronises.  The counterpart of the reference's `sparse`
  * algorithm, which materialises H (operator_matrix.hpp:79-105) instead. */
 int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t *nnz);
+int qsi_op_connectivity_padding(qsi_op_t op);
 
 /* Gathers of x in flight per lane of the cached apply (4, 8, 16 or 32).
  * Chosen by qsi_op_autotune; set it to pin one depth (results are

@@ -109,6 +109,7 @@ SIGNATURES = {
     "qsi_op_apply_host": (C.c_int, [_P, _P, _P, C.c_int]),
     "qsi_op_apply_block": (C.c_int, [_P, _P, _I64, _P, _I64, C.c_int, _P, C.c_int, _P]),
     "qsi_op_cache_connectivity": (C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
+    "qsi_op_connectivity_padding": (C.c_int, [_P]),
     "qsi_op_autotune": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.POINTER(C.c_int), _DP]),
     "qsi_block_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _DP]),
     "qsi_op_connectivity": (C.c_int, [_P, C.POINTER(_I64), _P, _P, _P, _P]),

@@ -839,7 +839,7 @@ int launch_blk_apply(const BlkArgs& a, int nw, int n_blocks, const double* X, do
   return QSI_OK;
 }
 
-int op_cache_connectivity(qsi_op_s* op, bool enable) {
+int op_cache_connectivity(qsi_op_s* op, bool enable, bool allow_pad) {
   auto release = [&]() {
     if (op->d_rowptr) (void)hipFree(op->d_rowptr);
     if (op->d_cj) (void)hipFree(op->d_cj);
@@ -896,7 +896,7 @@ int op_cache_connectivity(qsi_op_s* op, bool enable) {
   if (e == hipSuccess) {
     size_t free_b = 0, total_b = 0;
     const size_t padded = (nnz + 3 * static_cast<size_t>(D)) * 5;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+    if (allow_pad && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
         free_b > padded + std::max<size_t>(free_b / 4, size_t{1} << 30)) {
       e = offsets(4, &slots);
       op->conn_pad = 4;
@@ -1359,9 +1359,14 @@ extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double
 
 extern "C" int qsi_op_cache_connectivity(qsi_op_t op, int enable, int64_t* nnz) {
   QSI_REQUIRE(op != nullptr, "qsi_op_cache_connectivity: null operator");
-  const int rc = op_cache_connectivity(op, enable != 0);
+  QSI_REQUIRE(enable >= 0 && enable <= 2, "qsi_op_cache_connectivity: enable must be 0, 1 or 2");
+  const int rc = op_cache_connectivity(op, enable != 0, enable == 1);
   if (nnz) *nnz = op->conn_nnz;
   return rc;
+}
+
+extern "C" int qsi_op_connectivity_padding(qsi_op_t op) {
+  return op && op->conn_nnz >= 0 ? op->conn_pad : -1;
 }
 
 extern "C" int qsi_tridiag_eig(const double* d, const double* e, int n,

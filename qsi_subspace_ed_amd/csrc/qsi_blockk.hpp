@@ -81,6 +81,6 @@ int launch_blk_apply(const BlkArgs& a, int nw, int n_blocks, const double* X, do
                      double* partials, hipStream_t s);
 
 // Builds (enable != 0) or frees the operator's cached connectivity.
-int op_cache_connectivity(qsi_op_s* op, bool enable);
+int op_cache_connectivity(qsi_op_s* op, bool enable, bool allow_pad = true);
 
 }  // namespace qsi

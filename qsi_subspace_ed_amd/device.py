@@ -287,14 +287,21 @@ class GpuOp:
             return names.get(best.value, best.value), ms.value
         return best.value, ms.value
 
-    def cache_connectivity(self, enable=True):
-        """Keep the connected ranks on the device so block applies gather
-        instead of searching (qsi_op_cache_connectivity).  Returns the cached
-        nonzero count (-1 when dropped)."""
+    def cache_connectivity(self, enable=True, padded=None):
+        """Keep the connected ranks on the device so applies gather instead
+        of searching (qsi_op_cache_connectivity).  padded=None lets the
+        library pad rows to groups of four when memory allows; False keeps
+        them unpadded.  Returns the cached nonzero count (-1 when dropped)."""
         nnz = C.c_int64(-1)
-        check(_lib.load().qsi_op_cache_connectivity(self._h, int(bool(enable)), C.byref(nnz)),
+        mode = 0 if not enable else (2 if padded is False else 1)
+        check(_lib.load().qsi_op_cache_connectivity(self._h, mode, C.byref(nnz)),
               "qsi_op_cache_connectivity")
         return nnz.value
+
+    def connectivity_padding(self):
+        """Rows of the cached connectivity padded to multiples of this many
+        entries (4 or 1), or -1 when nothing is cached."""
+        return _lib.load().qsi_op_connectivity_padding(self._h)
 
     def connectivity(self):
         """(rows, cols, vals, diag) numpy arrays: off-diagonal H[row, col] in

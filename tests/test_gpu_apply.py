@@ -329,6 +329,53 @@ def test_cached_block_apply_column_pairs(R, ldx, lattices, cuda):
             assert torch.equal(Ya, ys[0][1]), k
 
 
+def test_cached_layouts_identical(lattices, cuda):
+    """The cached connectivity padded to groups of four (the default when
+    memory allows) and unpadded (first and last group of a row masked) give
+    the searching kernel's bits, single-vector at every gather depth and in
+    block applies of one and two columns per lane."""
+    torch = cuda
+    W = 48
+    jd = lattices[W]
+    b = load_basis(W)
+    D = b.shape[0]
+    Hs = q.lattice.add_ring_terms(q.SymbolicOpSum(), jd, [0.0] * 4, slots=True)
+    q.add_local_field(Hs, jd, [0.0, 0.0, 0.0], slots=True)
+    H = q.build_hamiltonian_g(jd, [-0.9, -0.35, -0.6, -0.45])
+    q.add_local_field(H, jd, [0.05, -0.02, 0.04])
+    op = q.GpuOp(q.GpuBasis(b, W), H)
+    ops = q.GpuOp(q.GpuBasis(b, W), Hs, n_slots=8)
+    rng = np.random.default_rng(31)
+    x = torch.from_numpy(rng.standard_normal(D)).cuda()
+    R = 6
+    X = torch.from_numpy(rng.standard_normal((D, R))).cuda()
+    co = torch.from_numpy(rng.standard_normal((8, R))).cuda()
+    op.set_variant("lane4")
+    y_ref = torch.zeros(D, dtype=torch.float64, device="cuda")
+    op.evaluate(x, y_ref)
+    Y_ref = torch.zeros((D, R), dtype=torch.float64, device="cuda")
+    ops.apply_block(X, Y_ref, coeffs=co)
+    for padded, pad in ((False, 1), (None, 4)):
+        assert op.cache_connectivity(True, padded=padded) == 703680
+        assert op.connectivity_padding() == pad
+        op.set_variant("cached")
+        for pipe in (4, 8, 16, 32):
+            op.set_cached_pipe(pipe)
+            y = torch.zeros(D, dtype=torch.float64, device="cuda")
+            op.evaluate(x, y)
+            assert torch.equal(y, y_ref), (pad, pipe)
+        op.cache_connectivity(False)
+        assert op.connectivity_padding() == -1
+        ops.cache_connectivity(True, padded=padded)
+        assert ops.connectivity_padding() == pad
+        for cfg in (0, 1, 2, 8, 9, 10):
+            ops.set_block_cfg(cfg)
+            Y = torch.zeros((D, R), dtype=torch.float64, device="cuda")
+            ops.apply_block(X, Y, coeffs=co)
+            assert torch.equal(Y, Y_ref), (pad, cfg)
+        ops.cache_connectivity(False)
+
+
 def test_autotune_keeps_results(lattices, cuda):
     """The autotuner picks a kernel by timing; every pick gives the same bits."""
     torch = cuda
