@@ -258,13 +258,14 @@ struct EigshResult {
 };
 
 // Lowest nev eigenpairs; sizing as compute_spectrum_iterative
-// (include/matrix_diag_bits.hpp:64-66: nev <= ncv - 1, ncv <= rows).  d_v0:
+// (include/matrix_diag_bits.hpp:64-66: nev <= ncv - 1, ncv <= rows), with at
+// least one eigenvalue for a one-state basis.  d_v0:
 // device start vector, or null for qsi_vec_randn(seed).
 inline EigshResult eigsh(const GpuOp& op, int nev, int ncv, double tol, int max_restarts = 1000,
                          int n_evecs = 0, const double* d_v0 = nullptr, uint64_t seed = 0) {
   const int64_t D = op.rows();
   ncv = static_cast<int>(std::min<int64_t>(ncv, D));
-  nev = std::min(nev, ncv - 1);
+  nev = std::min(nev, std::max(ncv - 1, 1));
   EigshResult r;
   r.evals.resize(static_cast<std::size_t>(std::max(nev, 0)));
   if (n_evecs > 0) r.evecs = DeviceVector(static_cast<std::size_t>(D) * n_evecs);
