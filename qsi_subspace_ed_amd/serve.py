@@ -199,12 +199,14 @@ class Space:
         """Device bytes when resident: basis words + bucket table (2^K + 1
         entries, K at most ceil(log2 D) + 7, 30 and n_spins) + cached
         connectivity (5 bytes per nonzero, at most n_spins / 4 nonzeros per row
-        on pyrochlore ring terms, + 8 per row) + terms + engine buffers."""
+        on pyrochlore ring terms, rows padded to a multiple of 4, + 8 per row)
+        + terms + engine buffers."""
         words = 1 if self.n_spins <= 64 else 2
         lg = max(0, int(self.dim - 1).bit_length())
         kbits = max(1, min(lg + 7, 30, self.n_spins))
         basis = self.dim * 8 * words + 4 * ((1 << kbits) + 1)
-        conn = self.dim * (8 + 5 * max(1, self.n_spins // 4))
+        row_cap = (max(1, self.n_spins // 4) + 3) // 4 * 4
+        conn = self.dim * (8 + 5 * row_cap)
         terms = 1 << 20  # term tables are a few KB; a MiB covers them
         return basis + conn + terms + Engine.footprint(self.dim, max_cols, ncv)
 

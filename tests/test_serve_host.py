@@ -1,9 +1,11 @@
 """Host side of the serving layer (no GPU): the slotted Hamiltonian has the
 same terms as the plain one, slot coefficients reproduce the plain
-coefficients, and the engine footprint agrees with its buffer list."""
+coefficients, the engine footprint agrees with its buffer list, and the
+server's admission keeps resident spaces within the HBM budget."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 import qsi_subspace_ed_amd as q
 from qsi_subspace_ed_amd import _lib, serve
@@ -73,3 +75,75 @@ def test_default_algorithm_by_size():
     assert diag.choose_algorithm(100) == "sparse"
     assert diag.choose_algorithm(9_999_999) == "sparse"
     assert diag.choose_algorithm(10_000_000) == "mfsparse"
+
+
+class _Closable:
+    def close(self):
+        pass
+
+
+class _FakeEngine:
+    """Engine stand-in: every request finishes after three finished steps."""
+
+    def __init__(self):
+        self.jobs, self.next, self.inflight = {}, 0, 0
+
+    def submit(self, coeffs, seed):
+        self.jobs[self.next] = 3
+        self.next += 1
+        return self.next - 1
+
+    active = property(lambda self: len(self.jobs))
+    queued = property(lambda self: 0)
+
+    def busy(self):
+        return self.inflight > 0 or bool(self.jobs)
+
+    def ready(self):
+        return True
+
+    def step_begin(self):
+        if not self.jobs:
+            return False
+        self.inflight += 1
+        return True
+
+    def step_end(self):
+        self.inflight -= 1
+        for k in self.jobs:
+            self.jobs[k] -= 1
+
+    def poll(self):
+        done = [k for k, v in self.jobs.items() if v <= 0]
+        for k in done:
+            del self.jobs[k]
+        return [serve.Finished(k, np.zeros(1), 1, 3) for k in done]
+
+    def close(self):
+        assert not self.busy(), "a busy engine was evicted"
+
+
+def test_server_admission_under_budget(monkeypatch):
+    """Three spaces of 100 bytes under a 250-byte budget: at most two are
+    resident, a space is uploaded only once an idle one is evicted, and every
+    request is answered."""
+    monkeypatch.setattr(serve.Space, "footprint", lambda self, m, n: self.dim)
+    srv = serve.SolveServer(250)
+
+    def upload(sp):
+        sp.engine, sp.op, sp.basis = _FakeEngine(), _Closable(), _Closable()
+        srv.stats.uploads += 1
+        assert srv.resident_bytes() <= srv.budget
+
+    monkeypatch.setattr(srv, "_upload", upload)
+    for key in "abc":
+        srv.add_space(key, {}, np.zeros((100, 2), np.uint64), 16)
+    with pytest.raises(ValueError):
+        srv.add_space("big", {}, np.zeros((300, 2), np.uint64), 16)
+    reqs = [serve.Request(space=k, coeffs=np.zeros(8), seed=i) for i, k in enumerate("abcab")]
+    out = srv.run(reqs)
+    assert sorted(r.id for r in out) == list(range(5))
+    assert all(r.result is not None and r.result.nconv == 1 for r in out)
+    assert srv.stats.uploads >= 3 and srv.stats.evictions >= 1
+    assert sum(s.engine is not None for s in srv.spaces.values()) <= 2
+    assert not srv.busy()
