@@ -20,6 +20,8 @@
  *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
  *     device work is stream ordered; an entry point returns once its work is
  *     enqueued unless its comment says it synchronises.
+ *   - A handle (basis, operator, engine) is used from one host thread at a
+ *     time; different handles may be used from different threads at once.
  */
 #ifndef QSI_GPU_H
 #define QSI_GPU_H
