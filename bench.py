@@ -1,17 +1,23 @@
-"""Benchmark: single-vector matrix-free H·v on the 64-spin pyrochlore ice basis.
+"""Benchmark: single-vector matrix-free H·v on the 96-spin pyrochlore ice basis.
 
-Workload (one GPU): pyrochlore supercell diag(2,2,4) on the fcc primitive
-vectors (64 spins), Q=0 sector (all 2-in-2-out states, enumerated here),
-H = -sum_hex g_s (O_hex + O_hex^dagger) - sum_i (B.z_sl(i)) S^z_i with
-g_s ~ U[0.2, 1.0] per hexagon sublattice and |B| = 0.1 uniform on the sphere
-(seeded), FP64 vectors.  A step is one y = H x over the whole basis with x, y
-and the basis resident in HBM.
+Default workload (one GPU): pyrochlore supercell diag(2,4,3) on the fcc
+primitive vectors (96 spins), Q=0 sector (all 2-in-2-out states, 2.0e9 of them,
+enumerated and sorted on the device), H = -sum_hex g_s (O_hex + O_hex^dagger)
+- sum_i (B.z_sl(i)) S^z_i with g_s ~ U[0.2, 1.0] per hexagon sublattice and
+|B| = 0.05 uniform on the sphere (seeded), FP64 vectors.  A step is one
+y = H x over the whole basis with x, y and the basis resident in HBM.
 
 Prints one JSON line: basis states/s (value), the apply kernel's roofline
-figures from HIP events, Lanczos and eigensolver rates, and the CPU baseline
-(the oracle's OpenMP push apply on this host, same workload).
+figures from HIP events, a parity check of the same apply against the oracle
+(the reference's push algorithm) over a bounded sample of source rows, 200
+Lanczos steps with Cullum-Willoughby-selected Ritz values and residuals, and
+the CPU baseline (the oracle's OpenMP push apply on this host over the same
+sample of rows).  The 64-spin diag(2,2,4) apply, its Lanczos/eigensolver
+rates, batched FTLM (48 spins, two sectors) and served solves follow under
+"spin64" (skip them with --no-extras).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--workload 96spin|64spin] [--no-extras]
 """
 import argparse
 import json
@@ -342,21 +348,19 @@ def bench_serve(states64, n_req=40, max_cols=16, nev=5, ncv=15, tol=1e-10, seed=
     }
 
 
-def cpu_sample_rate(states, H, x, budget_s=15.0, threads=None):
+def cpu_sample_rate(ob, D, H, x, budget_s=15.0, threads=None):
     """Oracle push apply (the reference's OpenMP algorithm) over a bounded
-    sample of rows of a basis too large to apply whole on the CPU in the
-    bench's time: rows [0, n) with n grown until one run takes about a
-    tenth of the budget, then the median of a few runs.  Returns the rate in
-    rows/s and a description of the sample."""
+    sample of source rows of a basis too large to apply whole on the CPU in
+    the bench's time: rows [0, n) with n grown (by 4x from 2^20) until one run
+    takes about a tenth of the budget, then the median of 3 runs.  Returns
+    the rate in rows/s, the thread count, a description of the sample, n,
+    and y of one run from zero (the pushes of rows [0, n) only, for the
+    parity check)."""
     from oracle import pyoracle as orc
     lib = orc.load()
     if threads:
         lib.oracle_set_threads(int(threads))
     nthr = lib.oracle_num_threads()
-    t0 = time.perf_counter()
-    ob = orc.Basis(states)
-    t_build = time.perf_counter() - t0
-    D = states.shape[0]
     y = np.zeros(D)
     n = 1 << 20
     while True:
@@ -367,20 +371,46 @@ def cpu_sample_rate(states, H, x, budget_s=15.0, threads=None):
             break
         n = min(D, n * 4)
     times = []
-    for _ in range(3):
+    for k in range(3):
+        if k == 0:
+            y[:] = 0.0
         t0 = time.perf_counter()
         orc.evaluate_add_rows(ob, H, x, y, 0, n)
         times.append(time.perf_counter() - t0)
+        if k == 0:
+            y_one = y.copy()
     t = float(np.median(times))
     return n / t, nthr, (f"rows [0, {n}) of the {D}-state basis (its first {100 * n / D:.2f}%), "
-                         f"median of 3 runs, {t:.2f} s each; oracle basis built in {t_build:.0f} s")
+                         f"median of 3 runs, {t:.2f} s each"), n, y_one
+
+
+def lanczos_lowest(op, M, nev, seed):
+    """M steps of the 3-term recurrence (no reorthogonalisation) from a
+    seeded Gaussian vector and the nev lowest Cullum-Willoughby-selected Ritz
+    values with their residual estimates."""
+    import qsi_subspace_ed_amd as q
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    al, be = op.lanczos_coeffs(M, seed=seed)
+    t_lz = time.perf_counter() - t0
+    vals, res, mult = q.lanczos_ritz(al, be)
+    T = np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+    plain = np.linalg.eigvalsh(T)
+    return {"steps": int(len(al)), "seconds": t_lz, "ms_per_step": t_lz / len(al) * 1e3,
+            "lowest": [float(v) for v in vals[:nev]],
+            "residuals": [float(r) for r in res[:nev]],
+            "multiplicity_in_T": [int(m) for m in mult[:nev]],
+            "selection": "Cullum-Willoughby (copies merged, spurious values dropped)",
+            "plain_lowest_ritz_values": [float(v) for v in plain[:nev]]}
 
 
 def bench_96(args):
     """The HBM-scale single-vector apply: 96-spin diag(2,4,3), Q=0, the basis
     enumerated and sorted on the device, g_s ~ U[0.2, 1] and |B| = 0.05 in a
-    random direction (seeded), FP64; the kernel picked by the autotuner; 200
-    Lanczos steps for the lowest Ritz values."""
+    random direction (seeded), FP64; the kernel picked by the autotuner;
+    parity against the oracle over a sample of source rows; 200 Lanczos
+    steps for the 5 lowest eigenvalues."""
     import torch
     import qsi_subspace_ed_amd as q
     from qsi_subspace_ed_amd import _lib
@@ -388,7 +418,7 @@ def bench_96(args):
     torch.cuda.set_device(0)
     jd = q.load_lattice(q.lattice_path("pyro_2,0,0_0,4,0_0,0,3.json"))
     t0 = time.perf_counter()
-    basis = q.enumerate_basis_device(jd, 0)
+    basis = q.enumerate_basis_device(jd, 0, bucket_bits=args.bucket_bits)
     t_enum = time.perf_counter() - t0
     D = basis.dim()
     print(f"[bench] 96-spin Q=0 basis D={D} enumerated on the device in {t_enum:.1f} s",
@@ -411,10 +441,11 @@ def bench_96(args):
     if tuned is None:
         op.set_variant(args.variant)
     print(f"[bench] autotuned variant {tuned}", file=sys.stderr, flush=True)
-    steps = max(1, min(args.steps, 10))
-    for _ in range(max(1, min(args.warmup, 2))):
+    steps = max(1, min(args.steps, 50))
+    for _ in range(max(1, min(args.warmup, 3))):
         op.evaluate(x, y)
     torch.cuda.synchronize()
+    # HIP events on the stream the library launches on (torch's current one)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -425,68 +456,11 @@ def bench_96(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_s = e0.elapsed_time(e1) / 1e3 / steps
+    print(f"[bench] {steps} applies: {wall * 1e3 / steps:.1f} ms/step wall, "
+          f"{kern_s * 1e3:.1f} ms/apply (events)", file=sys.stderr, flush=True)
     S = 16
     bytes_apply = D * (S + 8 + 8) + 8 * nnz
     achieved = bytes_apply / kern_s / 1e9
-    if args.calib_copy:
-        a = torch.full((CALIB_BYTES // 8,), 1.0, dtype=torch.float64, device="cuda")
-        c = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=c)
-        torch.cuda.synchronize()
-        del a, c
-    # the same apply through the cached connectivity, when it fits in HBM
-    cached = None
-    free, _ = torch.cuda.mem_get_info()
-    need = 5 * nnz + 8 * (D + 1)
-    if not args.no_cached and free > 1.05 * need:
-        torch.cuda.synchronize()
-        tc0 = time.perf_counter()
-        op.cache_connectivity(True)
-        t_build = time.perf_counter() - tc0
-        yc = torch.zeros_like(y)
-        # the cached kernel's gather depth (and whether it wins) by the autotuner
-        tuned_c = op.autotune(0, reps=1, x=x, y=yc)
-        op.evaluate(x, yc)
-        torch.cuda.synchronize()
-        ec0 = torch.cuda.Event(enable_timing=True)
-        ec1 = torch.cuda.Event(enable_timing=True)
-        tw0 = time.perf_counter()
-        ec0.record()
-        for _ in range(steps):
-            op.evaluate(x, yc)
-        ec1.record()
-        torch.cuda.synchronize()
-        wall_c = time.perf_counter() - tw0
-        kc = ec0.elapsed_time(ec1) / 1e3 / steps
-        cached = {"states_per_s": D * steps / wall_c, "apply_kernel_us": kc * 1e6,
-                  "build_s": t_build, "device_bytes": int(need), "autotuned": tuned_c[0],
-                  "cached_pipe": op.cached_pipe(),
-                  "algorithmic_GBps": bytes_apply / kc / 1e9,
-                  "kernel_bytes_per_apply": cached_kernel_bytes(D, nnz, S),
-                  "kernel_GBps": cached_kernel_bytes(D, nnz, S) / kc / 1e9,
-                  "traffic": measured_traffic(PMC_SUMMARY_96_CACHED),
-                  "bit_identical_to_search": bool(torch.equal(y, yc))}
-        print(f"[bench] cached connectivity: {kc * 1e3:.1f} ms/apply (built in {t_build:.1f} s)",
-              file=sys.stderr, flush=True)
-        del yc
-        torch.cuda.empty_cache()  # the Lanczos vectors below are allocated by the library
-    elif not args.no_cached:
-        cached = {"skipped": f"needs {need / 1e9:.0f} GB, {free / 1e9:.0f} GB free"}
-    lz = None
-    if args.lanczos_steps > 0:
-        M = args.lanczos_steps
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        al, be = op.lanczos_coeffs(M, seed=args.seed)
-        t_lz = time.perf_counter() - t0
-        T = np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
-        ritz = np.linalg.eigvalsh(T)
-        lz = {"steps": int(len(al)), "seconds": t_lz, "ms_per_step": t_lz / len(al) * 1e3,
-              "apply": "cached" if cached and "skipped" not in cached else "search",
-              "lowest_ritz_values": [float(v) for v in ritz[:5]]}
-        print(f"[bench] {M} Lanczos steps in {t_lz:.1f} s, E0 ~ {ritz[0]:.12f}",
-              file=sys.stderr, flush=True)
     out = {
         "metric": "H.v basis-states/s (96-spin Q=0 single-vector matrix-free apply)",
         "value": D * steps / wall,
@@ -508,55 +482,112 @@ def bench_96(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PMC_SUMMARY_96)},
-        "apply_kernel_us": kern_s * 1e6,
+        "apply_kernel_ms": kern_s * 1e3,
         "apply_variant": tuned[0] if tuned else args.variant,
         "autotuned": tuned is not None,
+        "bucket_bits": basis.bucket_bits(),
         "enumeration_s": t_enum,
-        "cached_connectivity": cached,
         "algorithmic_bytes_per_apply": int(bytes_apply),
-        "lanczos": lz,
-        "stream_copy_ceiling": stream_copy_ceiling(),
     }
+    if args.calib_copy:
+        a = torch.full((CALIB_BYTES // 8,), 1.0, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=c)
+        torch.cuda.synchronize()
+        del a, c
+    # parity and CPU baseline on a bounded sample of source rows [0, n): the
+    # oracle pushes those rows; the device pulls with x zero outside them
     if not args.no_cpu:
+        from oracle import pyoracle as orc
         states = basis.states()
+        tb = time.perf_counter()
+        ob = orc.Basis(states)
+        t_ob = time.perf_counter() - tb
+        basis._states_host = None
+        del states
         xh = x.cpu().numpy()
-        rate, nthr, sample = cpu_sample_rate(states, H, xh)
+        rate, nthr, sample, n_rows, y_cpu = cpu_sample_rate(ob, D, H, xh)
+        del ob
+        xm = x.clone()
+        xm[n_rows:] = 0.0
+        ym = torch.zeros_like(x)
+        op.evaluate(xm, ym)
+        yc = torch.from_numpy(y_cpu).cuda()
+        del y_cpu
+        err = float(torch.linalg.norm(ym - yc) / torch.linalg.norm(yc))
+        out["parity"] = {"rows": int(n_rows), "rel_err_vs_oracle": err,
+                         "max_abs_diff": float((ym - yc).abs().max()),
+                         "method": "oracle push of source rows [0, n) vs device apply of x "
+                                   "zeroed outside them, whole output vector"}
+        del xm, ym, yc
+        print(f"[bench] parity over {n_rows} source rows: rel err {err:.2e}", file=sys.stderr,
+              flush=True)
         out["cpu_baseline"] = {"value": rate, "unit": "basis-states/s", "cores": int(nthr),
-                               "kind": "port", "sample": sample, **host_cpu()}
+                               "kind": "port", "sample": sample + f"; oracle basis built in "
+                               f"{t_ob:.0f} s", **host_cpu()}
         out["speedup_vs_cpu"] = out["value"] / rate
-    print(json.dumps(out), flush=True)
+        torch.cuda.empty_cache()
+    # the same apply through the cached connectivity, when it fits in HBM
+    cached = None
+    free, _ = torch.cuda.mem_get_info()
+    need = 5 * nnz + 8 * (D + 1)
+    if not args.no_cached and free > 1.05 * need + 2 * 8 * D:
+        torch.cuda.synchronize()
+        tc0 = time.perf_counter()
+        op.cache_connectivity(True)
+        t_build = time.perf_counter() - tc0
+        yc = torch.zeros_like(y)
+        # the cached kernel's gather depth (and whether it wins) by the autotuner
+        tuned_c = op.autotune(0, reps=1, x=x, y=yc)
+        op.evaluate(x, yc)
+        torch.cuda.synchronize()
+        ec0 = torch.cuda.Event(enable_timing=True)
+        ec1 = torch.cuda.Event(enable_timing=True)
+        tw0 = time.perf_counter()
+        ec0.record()
+        for _ in range(steps):
+            op.evaluate(x, yc)
+        ec1.record()
+        torch.cuda.synchronize()
+        wall_c = time.perf_counter() - tw0
+        kc = ec0.elapsed_time(ec1) / 1e3 / steps
+        cached = {"states_per_s": D * steps / wall_c, "apply_kernel_ms": kc * 1e3,
+                  "build_s": t_build, "device_bytes": int(need), "autotuned": tuned_c[0],
+                  "cached_pipe": op.cached_pipe(),
+                  "algorithmic_GBps": bytes_apply / kc / 1e9,
+                  "kernel_bytes_per_apply": cached_kernel_bytes(D, nnz, S),
+                  "kernel_GBps": cached_kernel_bytes(D, nnz, S) / kc / 1e9,
+                  "traffic": measured_traffic(PMC_SUMMARY_96_CACHED),
+                  "bit_identical_to_search": bool(torch.equal(y, yc))}
+        print(f"[bench] cached connectivity: {kc * 1e3:.1f} ms/apply (built in {t_build:.1f} s)",
+              file=sys.stderr, flush=True)
+        del yc
+    elif not args.no_cached:
+        cached = {"skipped": f"needs {need / 1e9:.0f} GB, {free / 1e9:.0f} GB free"}
+    out["cached_connectivity"] = cached
+    del x, y
+    torch.cuda.empty_cache()  # the Lanczos vectors are allocated by the library
+    if args.lanczos_steps > 0:
+        lz = lanczos_lowest(op, args.lanczos_steps, 5, args.seed)
+        lz["apply"] = "cached" if cached and "skipped" not in cached else "search"
+        out["lanczos"] = lz
+        print(f"[bench] {lz['steps']} Lanczos steps in {lz['seconds']:.1f} s, lowest "
+              f"{lz['lowest']}", file=sys.stderr, flush=True)
+    op.close()
+    basis.close()
+    torch.cuda.empty_cache()
+    out["stream_copy_ceiling"] = stream_copy_ceiling()
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--lanczos-steps", type=int, default=200)
-    ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="also run a 64 MiB device copy (read+write) for counter calibration")
-    ap.add_argument("--no-solvers", action="store_true", help="skip the Lanczos/eigsh extras")
-    ap.add_argument("--variant", default="auto",
-                    help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")
-    ap.add_argument("--bucket-bits", type=int, default=0, help="bucket key width K in bits (0 = automatic)")
-    ap.add_argument("--no-ftlm", action="store_true", help="skip the batched FTLM comparison")
-    ap.add_argument("--no-serve", action="store_true", help="skip the served-solves comparison")
-    ap.add_argument("--no-cached", action="store_true",
-                    help="skip the cached-connectivity apply figures")
-    ap.add_argument("--workload", default="64spin", choices=["64spin", "96spin"],
-                    help="64spin: the default line; 96spin: the HBM-scale apply")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is design-only)"}))
-        sys.exit(2)
-
-    if args.workload == "96spin":
-        bench_96(args)
-        return
-
+def bench_64(args):
+    """The 64-spin diag(2,2,4) Q=0 apply (search and cached connectivity),
+    Lanczos and eigensolver rates, the CPU baseline on full applies, batched
+    FTLM (48 spins) and served solves; returns the JSON record.  An apply
+    takes ~0.35 ms here, so at least 200 are timed after 10 warm-ups."""
+    steps = max(args.steps, 200)
+    warmup = max(args.warmup, 10)
     import torch
     import qsi_subspace_ed_amd as q
     from qsi_subspace_ed_amd import _lib
@@ -584,21 +615,21 @@ def main():
     x = torch.from_numpy(xh).cuda()
     y = torch.zeros(D, dtype=torch.float64, device="cuda")
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         op.evaluate(x, y)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         op.evaluate(x, y)
     ev1.record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
-    kern_s = ev0.elapsed_time(ev1) / 1e3 / args.steps
-    print(f"[bench] {args.steps} applies: {wall * 1e3 / args.steps:.3f} ms/step wall, "
+    kern_s = ev0.elapsed_time(ev1) / 1e3 / steps
+    print(f"[bench] {steps} applies: {wall * 1e3 / steps:.3f} ms/step wall, "
           f"{kern_s * 1e6:.1f} us/apply (events)", file=sys.stderr, flush=True)
 
     S = 8  # bytes per state (<= 64 spins)
@@ -657,20 +688,20 @@ def main():
         if tuned_c[0] != "cached":
             op.set_variant("cached")
         yc = torch.zeros_like(y)
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             op.evaluate(x, yc)
         torch.cuda.synchronize()
         ec0 = torch.cuda.Event(enable_timing=True)
         ec1 = torch.cuda.Event(enable_timing=True)
         tw0 = time.perf_counter()
         ec0.record()
-        for _ in range(args.steps):
+        for _ in range(steps):
             op.evaluate(x, yc)
         ec1.record()
         torch.cuda.synchronize()
         wall_c = time.perf_counter() - tw0
-        kc = ec0.elapsed_time(ec1) / 1e3 / args.steps
-        cached = {"states_per_s": D * args.steps / wall_c, "apply_kernel_us": kc * 1e6,
+        kc = ec0.elapsed_time(ec1) / 1e3 / steps
+        cached = {"states_per_s": D * steps / wall_c, "apply_kernel_us": kc * 1e6,
                   "build_s": t_build, "nnz": int(cnnz), "autotuned": tuned_c[0],
                   "cached_pipe": op.cached_pipe(),
                   "device_bytes": int(cnnz) * 5 + 8 * (D + 1),
@@ -711,12 +742,12 @@ def main():
 
     out = {
         "metric": "H.v basis-states/s (64-spin Q=0 single-vector matrix-free apply)",
-        "value": D * args.steps / wall,
+        "value": D * steps / wall,
         "unit": "basis-states/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -725,7 +756,7 @@ def main():
         "config": {"workload": "pyro diag(2,2,4) 64 spins, Q=0, single-vector H.v",
                    "n_spins": 64, "basis_dim": int(D), "nnz_offdiag": int(nnz),
                    "g": [float(v) for v in g], "B": [float(v) for v in B]},
-        "hv_per_s": args.steps / wall,
+        "hv_per_s": steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic()},
@@ -747,6 +778,44 @@ def main():
         yg = y.cpu().numpy()
         out["parity_rel_err_vs_cpu"] = float(np.linalg.norm(yg - y_cpu) / np.linalg.norm(y_cpu))
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanczos-steps", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 64 MiB device copy (read+write) for counter calibration")
+    ap.add_argument("--no-solvers", action="store_true", help="skip the Lanczos/eigsh extras")
+    ap.add_argument("--variant", default="auto",
+                    help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")
+    ap.add_argument("--bucket-bits", type=int, default=0, help="bucket key width K in bits (0 = automatic)")
+    ap.add_argument("--no-ftlm", action="store_true", help="skip the batched FTLM comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-solves comparison")
+    ap.add_argument("--no-cached", action="store_true",
+                    help="skip the cached-connectivity apply figures")
+    ap.add_argument("--workload", default="96spin", choices=["64spin", "96spin"],
+                    help="96spin: the default line (the HBM-scale apply); 64spin: the 64-spin "
+                         "apply and its extras alone")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="96spin: skip the 64-spin, FTLM and served sections")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+
+    if args.workload == "96spin":
+        out = bench_96(args)
+        if not args.no_extras:
+            out["spin64"] = bench_64(args)
+    else:
+        out = bench_64(args)
     print(json.dumps(out), flush=True)
 
 

@@ -313,6 +313,23 @@ int qsi_lanczos_block(const qsi_op_t *ops, int n_sectors, int R,
 int qsi_tridiag_eig(const double *d, const double *e, int n, double *evals,
                     double *first_row);
 
+/* Ritz values of an m-step run of the 3-term recurrence without
+ * reorthogonalisation (h_alpha, h_beta as qsi_lanczos_coeffs returns them;
+ * h_beta[m-1] is the final residual norm), with the Cullum-Willoughby test:
+ * numerically multiple eigenvalues of T_m (ghost copies of a converged one,
+ * agreeing within tol) are reported once with their multiplicity, and simple
+ * eigenvalues of T_m that are also eigenvalues of T_m without its first row
+ * and column (spurious) are dropped.  resid[k] = |beta_m z_m| of the
+ * cluster's best member (z_m = last component of its unit eigenvector of
+ * T_m), a residual-norm estimate of the Ritz pair.  tol <= 0 picks
+ * 10 m eps ||T||_inf.  At most max_out values, ascending; *n_out receives the
+ * count written.  Host only.  The reference checks convergence of the lowest
+ * Ritz value only (components/linalg_routines/src/lanczos.cpp:14-53); this
+ * serves runs asked for several low eigenvalues with no room for a
+ * reorthogonalised Krylov basis. */
+int qsi_lanczos_ritz(const double *h_alpha, const double *h_beta, int m, double tol,
+                     int max_out, double *vals, double *resid, int *mult, int *n_out);
+
 /* ---- observables ----------------------------------------------------------
  * compute_all_expectations (src/expectation_eval.cpp:28-53):
  *   h_out[(l*n_vec + i)*n_vec + j] = psi_j . (O_l psi_i)

@@ -264,8 +264,10 @@ struct EigshResult {
 inline EigshResult eigsh(const GpuOp& op, int nev, int ncv, double tol, int max_restarts = 1000,
                          int n_evecs = 0, const double* d_v0 = nullptr, uint64_t seed = 0) {
   const int64_t D = op.rows();
-  ncv = static_cast<int>(std::min<int64_t>(ncv, D));
-  nev = std::min(nev, std::max(ncv - 1, 1));
+  // qsi_eigsh clamps ncv to the basis size itself; it needs ncv >= 2 on
+  // entry, so the clamp here keeps at least 2
+  ncv = static_cast<int>(std::max<int64_t>(2, std::min<int64_t>(ncv, D)));
+  nev = std::min(nev, std::max(static_cast<int>(std::min<int64_t>(ncv, D)) - 1, 1));
   EigshResult r;
   r.evals.resize(static_cast<std::size_t>(std::max(nev, 0)));
   if (n_evecs > 0) r.evecs = DeviceVector(static_cast<std::size_t>(D) * n_evecs);

@@ -16,6 +16,6 @@ from .lattice import (add_local_field, build_hamiltonian_g,  # noqa: F401
                       build_hamiltonian_jpm, g_vals, get_loc_z, get_ring_ops,
                       greedy_spin_ordering, lattice_path, load_lattice)
 from .device import (GpuBasis, GpuOp, Settings, as_lohi, enumerate_basis,  # noqa: F401
-                     enumerate_basis_device)
+                     enumerate_basis_device, lanczos_ritz)
 
 __version__ = "0.1.0"

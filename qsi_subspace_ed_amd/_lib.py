@@ -122,6 +122,8 @@ SIGNATURES = {
     "qsi_lanczos_block": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P),
                                     C.POINTER(_P), C.c_int, _DP, _DP, C.c_int, _DP]),
     "qsi_tridiag_eig": (C.c_int, [_DP, _DP, C.c_int, _DP, _DP]),
+    "qsi_lanczos_ritz": (C.c_int, [_DP, _DP, C.c_int, _D, C.c_int, _DP, _DP,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsi_engine_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, C.c_int, C.POINTER(_P)]),
     "qsi_engine_destroy": (C.c_int, [_P]),
     "qsi_engine_footprint": (_I64, [_I64, C.c_int, C.c_int]),

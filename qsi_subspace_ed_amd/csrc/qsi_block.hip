@@ -796,25 +796,29 @@ bool blk_pairs(const BlkArgs& a, const double* X) {
 
 int blk_apply_blocks(int nw, int64_t n_tiles, bool cached, int cfg) {
   // resident blocks per CU of each kernel: [cached][nw - 1][kernel][pairs];
-  // atomics, as host threads may launch concurrently (they store equal values)
+  // atomics, as host threads may launch concurrently (they store equal
+  // values).  The CU count is a function-local static, whose initialisation
+  // is thread-safe, so no thread can see an occupancy without it.
   static std::atomic<int> per_cu[2][2][3][2];
-  static std::atomic<int> n_cu{0};
-  const bool pairs = cached && blk_cfg_pairs(cfg);
-  std::atomic<int>& occ = per_cu[cached ? 1 : 0][nw - 1][blk_cfg_kernel(cfg)][pairs ? 1 : 0];
-  int o_cu = occ.load(std::memory_order_relaxed);
-  if (o_cu == 0) {
-    int dev = 0, o = 0, cu = 0;
+  static const int n_cu = [] {
+    int dev = 0, cu = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cu <= 0)
       cu = 256;
-    n_cu.store(cu, std::memory_order_relaxed);
+    return cu;
+  }();
+  const bool pairs = cached && blk_cfg_pairs(cfg);
+  std::atomic<int>& occ = per_cu[cached ? 1 : 0][nw - 1][blk_cfg_kernel(cfg)][pairs ? 1 : 0];
+  int o_cu = occ.load(std::memory_order_relaxed);
+  if (o_cu == 0) {
+    int o = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &o, blk_kernel(nw, cached, cfg, pairs), kBlkThreads, 0);
     o_cu = (e == hipSuccess && o > 0) ? o : 4;
     occ.store(o_cu, std::memory_order_relaxed);
   }
-  int64_t resident = static_cast<int64_t>(o_cu) * std::max(1, n_cu.load(std::memory_order_relaxed));
+  int64_t resident = static_cast<int64_t>(o_cu) * n_cu;
   if (blk_cfg_half(cfg)) resident = std::max<int64_t>(1, resident / 2);
   const int64_t need = (n_tiles + kBlkWaves - 1) / kBlkWaves;
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(need, resident)));

@@ -296,6 +296,82 @@ int qsi_lanczos_coeffs(qsi_op_t op, const double* d_v0, uint64_t seed, int M,
   return QSI_OK;
 }
 
+// Cullum-Willoughby selection of the Ritz values of an M-step Lanczos run
+// without reorthogonalisation (J. Cullum, R. Willoughby, "Lanczos Algorithms
+// for Large Symmetric Eigenvalue Computations", vol. 1, ch. 4).  Lost
+// orthogonality makes T_M repeat converged eigenvalues (numerically multiple
+// copies: kept, reported once) and adds spurious ones; a spurious eigenvalue
+// is simple in T_M and also an eigenvalue of T_M with its first row and
+// column removed, because its eigenvector has a vanishing first component.
+// The residual of a Ritz pair is |beta_M z_M| (z_M the last component of the
+// unit eigenvector of T_M); a cluster reports the smallest of its members'.
+int qsi_lanczos_ritz(const double* h_alpha, const double* h_beta, int m, double tol,
+                     int max_out, double* vals, double* resid, int* mult, int* n_out) {
+  QSI_REQUIRE(h_alpha && h_beta && m >= 1 && max_out >= 0 && n_out,
+              "qsi_lanczos_ritz: bad arguments");
+  QSI_REQUIRE(max_out == 0 || (vals && resid && mult), "qsi_lanczos_ritz: null output");
+  // a beta that underflowed ends the recurrence: T is then exact on an
+  // invariant subspace (no copies, no spurious values, zero residuals)
+  int n = m;
+  for (int j = 0; j < m; ++j) {
+    if (!(h_beta[j] > 0.0)) {
+      n = j + 1;
+      break;
+    }
+  }
+  const double beta_m = n == m ? h_beta[m - 1] : 0.0;
+  std::vector<double> d(h_alpha, h_alpha + n), e(h_beta, h_beta + std::max(n - 1, 0));
+  double tnorm = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double r = std::fabs(d[j]) + (j > 0 ? std::fabs(e[j - 1]) : 0.0) +
+                     (j + 1 < n ? std::fabs(e[j]) : 0.0);
+    tnorm = std::max(tnorm, r);
+  }
+  if (!(tol > 0.0)) tol = 10.0 * n * std::numeric_limits<double>::epsilon() * std::max(tnorm, 1e-300);
+  std::vector<double> theta, Z;
+  if (!hostla::tridiag_eig(d, e, theta, &Z)) {
+    set_error("qsi_lanczos_ritz: no convergence of the tridiagonal eigensolver");
+    return QSI_E_NOCONV;
+  }
+  std::vector<double> that;  // eigenvalues of T_M without its first row and column
+  if (n > 1 && n == m) {
+    std::vector<double> d2(d.begin() + 1, d.end()), e2(e.begin() + 1, e.end()), w2;
+    if (!hostla::tridiag_eig(d2, e2, that, nullptr)) {
+      set_error("qsi_lanczos_ritz: no convergence of the tridiagonal eigensolver");
+      return QSI_E_NOCONV;
+    }
+  }
+  int k = 0;
+  for (int a = 0; a < n;) {
+    int b = a + 1;
+    while (b < n && theta[b] - theta[b - 1] <= tol) ++b;  // cluster [a, b)
+    double best_r = std::numeric_limits<double>::infinity(), best_v = theta[a];
+    for (int i = a; i < b; ++i) {
+      const double r = std::fabs(beta_m * Z[static_cast<size_t>(i) * n + (n - 1)]);
+      if (r < best_r) {
+        best_r = r;
+        best_v = theta[i];
+      }
+    }
+    bool spurious = false;
+    if (b - a == 1 && !that.empty()) {
+      auto it = std::lower_bound(that.begin(), that.end(), theta[a] - tol);
+      spurious = it != that.end() && *it <= theta[a] + tol;
+    }
+    if (!spurious) {
+      if (k < max_out) {
+        vals[k] = best_v;
+        resid[k] = best_r;
+        mult[k] = b - a;
+      }
+      ++k;
+    }
+    a = b;
+  }
+  *n_out = std::min(k, max_out);
+  return QSI_OK;
+}
+
 int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
               const double* d_v0, uint64_t seed, double* h_evals,
               double* d_evecs, int n_evecs, int* nconv_out, int* n_applies) {

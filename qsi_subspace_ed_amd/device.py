@@ -402,6 +402,30 @@ def randn(n, seed):
     return v
 
 
+def lanczos_ritz(alpha, beta, tol=0.0, max_out=None):
+    """Ritz values of a run of the 3-term recurrence without
+    reorthogonalisation (the alphas and betas of GpuOp.lanczos_coeffs) after
+    the Cullum-Willoughby test (qsi_lanczos_ritz): ghost copies of converged
+    eigenvalues merged, spurious eigenvalues dropped.  Returns ascending
+    (values, residual estimates |beta_M z_M|, multiplicities in T_M)."""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    if a.ndim != 1 or a.shape != b.shape or a.size == 0:
+        raise ValueError("lanczos_ritz: alpha and beta must be non-empty 1-d arrays of one length")
+    m = a.size
+    k = m if max_out is None else int(max_out)
+    vals = np.zeros(max(k, 1))
+    res = np.zeros(max(k, 1))
+    mult = np.zeros(max(k, 1), dtype=np.int32)
+    n = C.c_int(0)
+    P = C.POINTER(C.c_double)
+    check(_lib.load().qsi_lanczos_ritz(a.ctypes.data_as(P), b.ctypes.data_as(P), m, float(tol), k,
+                                       vals.ctypes.data_as(P), res.ctypes.data_as(P),
+                                       mult.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n)),
+          "qsi_lanczos_ritz")
+    return vals[: n.value], res[: n.value], mult[: n.value]
+
+
 def _lattice_arrays(jdata, order):
     from .lattice import greedy_spin_ordering
     W = len(jdata["atoms"])

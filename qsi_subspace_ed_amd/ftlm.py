@@ -81,10 +81,15 @@ def tridiag_eig(alpha, beta):
 
 def quadrature(alpha, beta):
     """Nodes and weights of every sample: lists of (e_k, w_k) arrays for the
-    (R, M) coefficient arrays of one sector."""
+    (R, M) coefficient arrays of one sector.  A sample whose recurrence ended
+    (beta_j == 0: its Krylov space is invariant, later coefficients are 0)
+    keeps only its first j + 1 steps, as the C++ FtlmComputer does."""
     out = []
     for r in range(alpha.shape[0]):
-        out.append(tridiag_eig(alpha[r], beta[r]))
+        b = np.asarray(beta[r])
+        z = np.flatnonzero(b == 0.0)
+        m = int(z[0]) + 1 if z.size else len(alpha[r])
+        out.append(tridiag_eig(alpha[r][:m], b[:m]))
     return out
 
 

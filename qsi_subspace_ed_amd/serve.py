@@ -190,25 +190,44 @@ class Space:
     op: object = None
     engine: object = None
     last_used: float = 0.0
+    _padded_nnz: int = -1
 
     @property
     def dim(self):
         return int(self.states.shape[0])
 
+    def padded_nnz(self):
+        """Connectivity entries of the space's operator with every row padded
+        to a multiple of 4 (what the cache allocates at most): each state's
+        count of connecting off-diagonal terms, counted on the host once."""
+        if self._padded_nnz < 0:
+            H = slotted_hamiltonian(self.jdata)
+            lo = np.ascontiguousarray(self.states[:, 0])
+            hi = np.ascontiguousarray(self.states[:, 1])
+            f = np.zeros(self.dim, dtype=np.int32)
+            M = (1 << 64) - 1
+            for (_, o, _) in H.off_diag_terms:
+                m = o.up | o.down
+                v = o.up ^ (o.X & m)
+                f += ((lo & np.uint64(m & M)) == np.uint64(v & M)) & \
+                     ((hi & np.uint64(m >> 64)) == np.uint64(v >> 64))
+            self._padded_nnz = int(((f + 3) // 4 * 4).sum())
+        return self._padded_nnz
+
     def footprint(self, max_cols, ncv):
         """Device bytes when resident: basis words + bucket table (2^K + 1
         entries, K at most ceil(log2 D) + 7, 30 and n_spins) + cached
-        connectivity (5 bytes per nonzero, at most n_spins / 4 nonzeros per row
-        on pyrochlore ring terms, rows padded to a multiple of 4, + 8 per row)
-        + terms + engine buffers."""
+        connectivity (5 bytes per entry, rows padded to a multiple of 4, + 8
+        per row) + terms + engine buffers + the block autotuner's two D x
+        max_cols scratch blocks (held while the engine is created)."""
         words = 1 if self.n_spins <= 64 else 2
         lg = max(0, int(self.dim - 1).bit_length())
         kbits = max(1, min(lg + 7, 30, self.n_spins))
         basis = self.dim * 8 * words + 4 * ((1 << kbits) + 1)
-        row_cap = (max(1, self.n_spins // 4) + 3) // 4 * 4
-        conn = self.dim * (8 + 5 * row_cap)
+        conn = 8 * (self.dim + 1) + 5 * self.padded_nnz() + 64
         terms = 1 << 20  # term tables are a few KB; a MiB covers them
-        return basis + conn + terms + Engine.footprint(self.dim, max_cols, ncv)
+        tune = 2 * 8 * self.dim * max_cols
+        return basis + conn + terms + tune + Engine.footprint(self.dim, max_cols, ncv)
 
 
 @dataclass

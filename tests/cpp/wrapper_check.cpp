@@ -12,7 +12,10 @@
 //                        the complete unit-vector start set, krylov_dim 80
 //        ftlm_rng.bin    [Z shifted, E] of 4 samples from mt19937(7) one at a
 //                        time, then of the same 4 as one block
-//   stdout: "error ok" when an unsorted basis is refused with qsi::Error
+//   stdout: "one-state ok" when eigsh on the basis of the first state alone
+//           gives that state's diagonal element;
+//           "error ok" when an unsorted basis is refused with qsi::Error
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -108,6 +111,20 @@ int main(int argc, char** argv) {
       out.insert(out.end(), e.begin(), e.end());
     }
     write_all(dir + "/ftlm_rng.bin", out);
+  }
+
+  // a one-state basis: eigsh gives its one eigenvalue, the diagonal element
+  {
+    std::vector<uint64_t> one(lohi.begin(), lohi.begin() + 2);
+    qsi::GpuBasis b1(one, n_spins);
+    qsi::GpuOp op1(b1, terms);
+    std::vector<double> e0 = {1.0}, h00 = {0.0};
+    op1.perform_op(e0.data(), h00.data());
+    const auto r1 = qsi::eigsh(op1, 5, 15, 1e-10);
+    const bool ok = r1.evals.size() == 1 &&
+                    std::fabs(r1.evals[0] - h00[0]) <= 1e-12 * (1.0 + std::fabs(h00[0]));
+    std::printf("one-state %s %.17g %.17g\n", ok ? "ok" : "bad",
+                r1.evals.empty() ? 0.0 : r1.evals[0], h00[0]);
   }
 
   // the error path: an unsorted basis is refused

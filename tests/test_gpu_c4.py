@@ -1,0 +1,103 @@
+"""The HBM-scale workload: 96-spin diag(2,4,3), Q=0 (2.0e9 states enumerated
+and sorted on the device), g_s ~ U[0.2, 1] and |B| = 0.05 (seeded as the
+benchmark does).
+
+* device ranks of a random sample of basis states equal their indices, and
+  states one spin flip away from a basis state (each breaks the ice rule on
+  two tetrahedra) are reported absent;
+* H·v against the oracle (the reference's push algorithm,
+  components/operator/src/operator_matrix.cpp:28-82): the oracle pushes
+  source rows [0, 2^24) of the full basis; the device apply of x zeroed
+  outside those rows must equal it over the whole output vector, to 1e-13
+  relative 2-norm (the push over a set of sources is the pull of the masked
+  vector, targets anywhere in the basis).
+
+The three tests share one enumeration (module fixture, ~1.5 min).
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+import qsi_subspace_ed_amd as q
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+HV_TOL = 1e-13
+N_ROWS = 1 << 24
+
+
+@pytest.fixture(scope="module")
+def c4(cuda):
+    torch = cuda
+    free, _ = torch.cuda.mem_get_info()
+    if free < 120e9:
+        pytest.skip("the 96-spin basis needs ~120 GB of device memory")
+    jd = q.load_lattice(q.lattice_path("pyro_2,0,0_0,4,0_0,0,3.json"))
+    basis = q.enumerate_basis_device(jd, 0)
+    rng = np.random.default_rng(1234)
+    g = rng.uniform(0.2, 1.0, size=4)
+    b = rng.normal(size=3)
+    B = 0.05 * b / np.linalg.norm(b)
+    H = q.build_hamiltonian_g(jd, [-v for v in g])
+    q.add_local_field(H, jd, B)
+    op = q.GpuOp(basis, H)
+    states = basis.states()
+    yield {"jd": jd, "basis": basis, "op": op, "H": H, "states": states}
+    op.close()
+    basis.close()
+    torch.cuda.empty_cache()
+
+
+def test_c4_basis(c4):
+    st = c4["states"]
+    D = st.shape[0]
+    assert D > 1_000_000_000
+    # strictly increasing as 128-bit numbers, in chunks to bound memory
+    step = 1 << 28
+    for a in range(0, D - 1, step):
+        lo = st[a:a + step + 1, 0]
+        hi = st[a:a + step + 1, 1]
+        inc = (hi[1:] > hi[:-1]) | ((hi[1:] == hi[:-1]) & (lo[1:] > lo[:-1]))
+        assert inc.all(), a
+    assert int(st[-1, 1]) >> 32 == 0  # 96 spins: nothing above bit 95
+
+
+def test_c4_ranks(c4):
+    st = c4["states"]
+    D = st.shape[0]
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.integers(0, D, size=1 << 20))
+    got = c4["basis"].search(st[idx])
+    assert np.array_equal(got, idx)
+    # one spin flipped: that spin's two tetrahedra no longer hold 2 up spins
+    qs = st[idx[:1 << 16]].copy()
+    spin = rng.integers(0, 96, size=qs.shape[0])
+    w = spin // 64
+    qs[np.arange(qs.shape[0]), w] ^= (np.uint64(1) << (spin % 64).astype(np.uint64))
+    assert np.all(c4["basis"].search(qs) == -1)
+
+
+def test_c4_apply_parity(c4, cuda):
+    torch = cuda
+    st = c4["states"]
+    D = st.shape[0]
+    rng = np.random.default_rng(11)
+    x = np.zeros(D)
+    x[:N_ROWS] = rng.standard_normal(N_ROWS)
+    ob = orc.Basis(st)
+    y_ref = np.zeros(D)
+    orc.evaluate_add_rows(ob, c4["H"], x, y_ref, 0, N_ROWS)
+    del ob
+    xd = torch.from_numpy(x).cuda()
+    del x
+    yd = torch.full((D,), 3.0, dtype=torch.float64, device="cuda")
+    c4["op"].evaluate(xd, yd)
+    del xd
+    yr = torch.from_numpy(y_ref).cuda()
+    del y_ref
+    nz = int((yr != 0).sum())
+    assert nz > N_ROWS  # the pushes land far outside the source rows
+    err = float(torch.linalg.norm(yd - yr) / torch.linalg.norm(yr))
+    assert err <= HV_TOL, err
+    # rows no source connects to stay exactly zero
+    assert bool(((yr == 0) == (yd == 0)).all())

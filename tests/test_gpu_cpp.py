@@ -45,6 +45,7 @@ def test_cpp_face(lattices, cuda, tmp_path):
                        timeout=120)
     assert r.returncode == 0, r.stdout
     assert "error ok" in r.stdout, r.stdout
+    assert "one-state ok" in r.stdout, r.stdout
 
     op = q.GpuOp(q.GpuBasis(b, 32), H)
     xd = torch.from_numpy(x).cuda()

@@ -147,3 +147,20 @@ def test_server_admission_under_budget(monkeypatch):
     assert srv.stats.uploads >= 3 and srv.stats.evictions >= 1
     assert sum(s.engine is not None for s in srv.spaces.values()) <= 2
     assert not srv.busy()
+
+
+@pytest.mark.parametrize("W", [32, 48])
+def test_padded_nnz_counts_oracle_rows(W):
+    # the footprint's connectivity count: per row, the oracle's off-diagonal
+    # entries (the reference's toSparseMatrix rows), padded to groups of 4
+    from conftest import load_basis
+    from oracle import pyoracle as orc
+    jd = _lat(W)
+    b = load_basis(W)
+    sp = serve.Space(f"{W}", jd, q.as_lohi(b), W)
+    H = q.build_hamiltonian_g(jd, [-1.0, -1.0, -1.0, -1.0])
+    r, c, _ = orc.sparse_triplets(orc.Basis(b), H)
+    off = r != c
+    f = np.bincount(r[off], minlength=b.shape[0])
+    assert sp.padded_nnz() == int(((f + 3) // 4 * 4).sum())
+    assert f.max() > W // 4 or W == 32  # rows above n_spins / 4 exist at 48 spins
