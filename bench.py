@@ -384,6 +384,25 @@ def cpu_sample_rate(ob, D, H, x, budget_s=15.0, threads=None):
                          f"median of 3 runs, {t:.2f} s each"), n, y_one
 
 
+def time_variants(op, x, y, names, reps=2):
+    """HIP-event time (ms) of one apply with each named kernel variant."""
+    import torch
+    out = {}
+    for v in names:
+        op.set_variant(v)
+        op.evaluate(x, y)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op.evaluate(x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        out[v] = e0.elapsed_time(e1) / reps
+    return out
+
+
 def lanczos_lowest(op, M, nev, seed):
     """M steps of the 3-term recurrence (no reorthogonalisation) from a
     seeded Gaussian vector and the nev lowest Cullum-Willoughby-selected Ritz
@@ -429,6 +448,11 @@ def bench_96(args):
     B = 0.05 * b / np.linalg.norm(b)
     H = q.build_hamiltonian_g(jd, [-v for v in g])
     q.add_local_field(H, jd, B)
+    if not args.no_slots:
+        ts = time.perf_counter()
+        basis.build_slots()
+        print(f"[bench] slot index built in {time.perf_counter() - ts:.1f} s", file=sys.stderr,
+              flush=True)
     op = q.GpuOp(basis, H)
     nnz = C.c_int64(0)
     _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
@@ -437,10 +461,13 @@ def bench_96(args):
         device="cuda").manual_seed(args.seed))
     x /= torch.linalg.norm(x)
     y = torch.zeros_like(x)
+    variants_ms = time_variants(op, x, y, ["lane4"] + ([] if args.no_slots else
+                                                       ["slot2", "slot4", "slot8"]))
     tuned = op.autotune(0, reps=1, x=x, y=y) if args.variant == "auto" else None
     if tuned is None:
         op.set_variant(args.variant)
-    print(f"[bench] autotuned variant {tuned}", file=sys.stderr, flush=True)
+    print(f"[bench] autotuned variant {tuned}; per variant (ms) {variants_ms}", file=sys.stderr,
+          flush=True)
     steps = max(1, min(args.steps, 50))
     for _ in range(max(1, min(args.warmup, 3))):
         op.evaluate(x, y)
@@ -485,7 +512,9 @@ def bench_96(args):
         "apply_kernel_ms": kern_s * 1e3,
         "apply_variant": tuned[0] if tuned else args.variant,
         "autotuned": tuned is not None,
+        "variants_ms": variants_ms,
         "bucket_bits": basis.bucket_bits(),
+        "slot_index": basis.has_slots(),
         "enumeration_s": t_enum,
         "algorithmic_bytes_per_apply": int(bytes_apply),
     }
@@ -529,7 +558,11 @@ def bench_96(args):
         out["speedup_vs_cpu"] = out["value"] / rate
         torch.cuda.empty_cache()
     # the same apply through the cached connectivity, when it fits in HBM
+    # (the slot index is dropped first: the cached kernel does not search)
     cached = None
+    if not args.no_cached:
+        basis.build_slots(False)
+        torch.cuda.empty_cache()
     free, _ = torch.cuda.mem_get_info()
     need = 5 * nnz + 8 * (D + 1)
     if not args.no_cached and free > 1.05 * need + 2 * 8 * D:
@@ -600,6 +633,8 @@ def bench_64(args):
           flush=True)
     basis = q.GpuBasis(states, 64, bucket_bits=args.bucket_bits,
                        hash_index=args.variant.startswith("hash"))
+    if not args.no_slots:
+        basis.build_slots()
     op = q.GpuOp(basis, H)
     tuned = None
     if args.variant == "auto":
@@ -804,6 +839,8 @@ def main():
                          "apply and its extras alone")
     ap.add_argument("--no-extras", action="store_true",
                     help="96spin: skip the 64-spin, FTLM and served sections")
+    ap.add_argument("--no-slots", action="store_true",
+                    help="do not build the slot index (search by bucket bisection only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "

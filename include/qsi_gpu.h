@@ -92,6 +92,19 @@ int qsi_basis_create(const uint64_t *h_lohi, int64_t dim, int n_spins,
 int qsi_basis_create_ex(const uint64_t *h_lohi, int64_t dim, int n_spins,
                         int bucket_bits, int with_hash, qsi_basis_t *out);
 int qsi_basis_has_hash(qsi_basis_t b);
+/* Builds (enable = 1) or drops (enable = 0) the basis's slot index: a second
+ * level under the bucket table for bases whose buckets stay large.  Bucket
+ * [lo, lo + n) owns slots [2 lo, 2 lo + 2n) of one uint32 array (8 bytes per
+ * state); each state sits in one slot as (16-bit hash fingerprint, rank - lo),
+ * placed by linear probing from a hash of the state.  A lookup reads the
+ * bucket bounds, then slots until an empty one or a fingerprint match, whose
+ * state is compared: three dependent loads where a bisection of an n-state
+ * bucket needs log2(n) + 2.  Used by the QSI_APPLY_SLOT* kernels and by
+ * qsi_basis_search.  Fails (QSI_E_INVALID) if a bucket holds more than 65535
+ * states.  Dropping synchronises the device.  No counterpart in the
+ * reference (its buckets are searched by bisection, operator.cpp:38-70). */
+int qsi_basis_build_slots(qsi_basis_t b, int enable);
+int qsi_basis_has_slots(qsi_basis_t b);
 int qsi_basis_destroy(qsi_basis_t b);
 int64_t qsi_basis_dim(qsi_basis_t b);
 int qsi_basis_n_spins(qsi_basis_t b);
@@ -106,7 +119,8 @@ int qsi_basis_bucket_bits(qsi_basis_t b);
 int qsi_basis_key_fields(qsi_basis_t b, int *n_fields, int *lo, int *width);
 /* Device-side state -> rank lookup of n states (test hook and building block;
  * replaces ZBasisInterp::search, operator.cpp:38-70).  d_lohi: n rows of
- * [lo,hi] in device memory; d_rank receives the rank or -1 if absent. */
+ * [lo,hi] in device memory; d_rank receives the rank or -1 if absent.  Goes
+ * through the slot index when the basis has one, else bisects the bucket. */
 int qsi_basis_search(qsi_basis_t b, const uint64_t *d_lohi, int64_t n,
                      int64_t *d_rank, qsi_stream_t stream);
 
@@ -129,11 +143,12 @@ int qsi_op_n_diag(qsi_op_t op);
  * QSI_APPLY_WINDOW: per wavefront, one bucket lookup per term, then the
  * connected rows search a 64-state window staged in LDS.  QSI_APPLY_CACHED:
  * gathers through the cached connectivity (qsi_op_cache_connectivity; setting
- * it requires the cache, dropping the cache resets it to AUTO).  AUTO =
- * CACHED when the connectivity is cached, else LANE4 (the fastest search
- * measured on MI355X: the sorted-basis bisection keeps the lookups of
- * neighbouring rows in the same cache lines, which the hash index cannot).
- * All give bit-identical y. */
+ * it requires the cache, dropping the cache resets it to AUTO).
+ * QSI_APPLY_SLOT{2,4,8}: as LANE*, through the slot index under each bucket
+ * (qsi_basis_build_slots; setting one requires the index; if the index is
+ * dropped later the kernel falls back to LANE of the same depth).  AUTO =
+ * CACHED when the connectivity is cached, else SLOT4 when the basis has a slot
+ * index, else LANE4.  All give bit-identical y. */
 enum {
   QSI_APPLY_AUTO = 0,
   QSI_APPLY_WINDOW = 1,
@@ -143,7 +158,10 @@ enum {
   QSI_APPLY_LANE8 = 5,
   QSI_APPLY_HASH = 6,
   QSI_APPLY_HASH4 = 7,
-  QSI_APPLY_CACHED = 8
+  QSI_APPLY_CACHED = 8,
+  QSI_APPLY_SLOT2 = 9,
+  QSI_APPLY_SLOT4 = 10,
+  QSI_APPLY_SLOT8 = 11
 };
 int qsi_op_set_variant(qsi_op_t op, int variant);
 int qsi_op_get_variant(qsi_op_t op);

@@ -26,9 +26,13 @@ QSI_APPLY_LANE8 = 5
 QSI_APPLY_HASH = 6
 QSI_APPLY_HASH4 = 7
 QSI_APPLY_CACHED = 8
+QSI_APPLY_SLOT2 = 9
+QSI_APPLY_SLOT4 = 10
+QSI_APPLY_SLOT8 = 11
 VARIANTS = {"auto": QSI_APPLY_AUTO, "window": QSI_APPLY_WINDOW, "lane": QSI_APPLY_LANE,
             "lane2": QSI_APPLY_LANE2, "lane4": QSI_APPLY_LANE4, "lane8": QSI_APPLY_LANE8,
-            "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4, "cached": QSI_APPLY_CACHED}
+            "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4, "cached": QSI_APPLY_CACHED,
+            "slot2": QSI_APPLY_SLOT2, "slot4": QSI_APPLY_SLOT4, "slot8": QSI_APPLY_SLOT8}
 
 
 class QsiError(RuntimeError):
@@ -87,6 +91,8 @@ SIGNATURES = {
     "qsi_basis_create": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.POINTER(_P)]),
     "qsi_basis_create_ex": (C.c_int, [_P, _I64, C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
     "qsi_basis_has_hash": (C.c_int, [_P]),
+    "qsi_basis_build_slots": (C.c_int, [_P, C.c_int]),
+    "qsi_basis_has_slots": (C.c_int, [_P]),
     "qsi_basis_destroy": (C.c_int, [_P]),
     "qsi_basis_dim": (_I64, [_P]),
     "qsi_basis_n_spins": (C.c_int, [_P]),

@@ -1323,6 +1323,11 @@ extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double
       cand.push_back(QSI_APPLY_HASH);
       cand.push_back(QSI_APPLY_HASH4);
     }
+    if (op->basis->d_slots) {
+      cand.push_back(QSI_APPLY_SLOT2);
+      cand.push_back(QSI_APPLY_SLOT4);
+      cand.push_back(QSI_APPLY_SLOT8);
+    }
     // the cached kernel once per gather depth (4, 8, 16, 32)
     std::vector<int> pipes(cand.size(), 8);
     if (op->conn_nnz >= 0) {

@@ -106,10 +106,11 @@ __device__ __forceinline__ bool st_eq(const State<NW>& a, const State<NW>& b) {
 // positions the key is the top K bits.
 constexpr int kKeySegs = 4;
 
-// Device view of a basis: states + bucket table.
+// Device view of a basis: states + bucket table (+ slot index).
 struct BasisView {
   const uint64_t* states;  // D * NW words
   const uint32_t* table;   // 2^K + 1 entries: table[k] = #states with key < k
+  const uint32_t* slots;   // slot index (2 D entries) or null
   int64_t dim;
   int kbits;               // K = sum of the field widths
   int n_spins;
@@ -189,6 +190,66 @@ __device__ __forceinline__ int64_t basis_lookup(const BasisView& b,
   }
   const State<NW> f = load_state_cached<NW>(b.states, lo);
   return st_eq<NW>(f, key) ? static_cast<int64_t>(lo) : -1;
+}
+
+// ---- slot index ------------------------------------------------------------
+// A second level under the bucket table for bases whose buckets stay large
+// (the 96-spin ice basis spreads its states over 49 informative positions, so
+// a 30-bit key leaves ~230 states in the bucket of an average lookup).  The
+// bucket of ranks [lo, lo + n) owns the 2n slots [2 lo, 2 lo + 2n) of one
+// uint32 array (so no offsets are stored: the slots of all buckets tile
+// [0, 2 D)); each of its states sits in one slot, placed by linear probing
+// from a hash of the state:
+//   slot = (fingerprint << 16) | (rank - lo),  fingerprint = hash bits 48..63
+// and kSlotEmpty marks a free slot (n <= kSlotMaxBucket keeps rank - lo below
+// 0xFFFF, so no entry equals it).  A lookup reads the bucket bounds, then
+// slots from the state's home until it meets an empty one (absent) or a
+// fingerprint match, whose state is compared (and x gathered) at once; load
+// factor 1/2 keeps the expected probe count near 1.5, and the slots of one
+// bucket are contiguous, so neighbouring rows' lookups share their lines.
+constexpr uint32_t kSlotEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kSlotMaxBucket = 0xFFFFu;
+
+template <int NW>
+__host__ __device__ __forceinline__ uint64_t slot_hash(const State<NW>& s) {
+  uint64_t h = s.w[0];
+  if constexpr (NW == 2) h ^= s.w[1] * 0x9E3779B97F4A7C15ull;
+  // splitmix64 finaliser (Steele, Lea & Flood 2014)
+  h ^= h >> 30;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 27;
+  h *= 0x94D049BB133111EBull;
+  return h ^ (h >> 31);
+}
+
+// Home slot (0 <= home < m) of a hash in a bucket region of m slots.
+__host__ __device__ __forceinline__ uint32_t slot_home(uint64_t h, uint32_t m) {
+  return static_cast<uint32_t>((static_cast<uint64_t>(static_cast<uint32_t>(h)) * m) >> 32);
+}
+
+__host__ __device__ __forceinline__ uint32_t slot_fp(uint64_t h) {
+  return static_cast<uint32_t>(h >> 48);
+}
+
+// Rank of key or -1 through the slot index (b.slots non-null).
+template <int NW>
+__device__ __forceinline__ int64_t slot_lookup(const BasisView& b, const State<NW>& key) {
+  uint32_t lo, n;
+  bucket_bounds(b.table, bucket_key<NW>(b, key), lo, n);
+  if (n == 0) return -1;
+  const uint64_t h = slot_hash<NW>(key);
+  const uint32_t m = 2 * n, fp = slot_fp(h);
+  const uint32_t* reg = b.slots + 2 * static_cast<uint64_t>(lo);
+  uint32_t p = slot_home(h, m);
+  for (;;) {
+    const uint32_t e = reg[p];
+    if (e == kSlotEmpty) return -1;
+    if ((e >> 16) == fp) {
+      const uint32_t J = lo + (e & 0xFFFFu);
+      if (st_eq<NW>(load_state_cached<NW>(b.states, J), key)) return J;
+    }
+    p = p + 1 == m ? 0 : p + 1;
+  }
 }
 
 // ---- hash index ------------------------------------------------------------

@@ -121,6 +121,30 @@ __global__ void k_build_hash(const uint64_t* __restrict__ states, int64_t dim,
   }
 }
 
+// Place every basis state in its bucket's slot region (slots pre-filled with
+// kSlotEmpty): linear probing from the state's home, a slot claimed by a
+// compare-and-swap.  A bucket above kSlotMaxBucket states sets *too_big and
+// places nothing.
+template <int NW>
+__global__ void k_build_slots(BasisView b, uint32_t* __restrict__ slots,
+                              unsigned int* __restrict__ too_big) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.dim) return;
+  const State<NW> s = load_state_cached<NW>(b.states, i);
+  uint32_t lo, n;
+  bucket_bounds(b.table, bucket_key<NW>(b, s), lo, n);
+  if (n > kSlotMaxBucket) {
+    atomicOr(too_big, 1u);
+    return;
+  }
+  const uint64_t h = slot_hash<NW>(s);
+  const uint32_t m = 2 * n;
+  const uint32_t e = (slot_fp(h) << 16) | static_cast<uint32_t>(i - lo);
+  uint32_t* reg = slots + 2 * static_cast<uint64_t>(lo);
+  uint32_t p = slot_home(h, m);
+  while (atomicCAS(reg + p, kSlotEmpty, e) != kSlotEmpty) p = p + 1 == m ? 0 : p + 1;
+}
+
 template <int NW>
 __global__ void k_search(BasisView b, const uint64_t* __restrict__ q, int64_t n,
                          int64_t* __restrict__ rank) {
@@ -132,7 +156,7 @@ __global__ void k_search(BasisView b, const uint64_t* __restrict__ q, int64_t n,
   if constexpr (NW == 2) s.w[1] = q[2 * i + 1];
   bool ok = true;
   if constexpr (NW == 1) ok = (q[2 * i + 1] == 0);
-  rank[i] = ok ? basis_lookup<NW>(b, s) : -1;
+  rank[i] = !ok ? -1 : b.slots ? slot_lookup<NW>(b, s) : basis_lookup<NW>(b, s);
 }
 
 // ============================================================================
@@ -222,6 +246,128 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if (hit[g] && fabs(xj[g]) > kApplyTol) {
+            const OffTerm& T = off[tt[g]];
+            double c = T.c * T.sgn;
+            if constexpr (HAS_Z) {
+              if (parity_notz<NW>(s, T.Z)) c = -c;
+            }
+            acc += c * xj[g];
+          }
+        }
+      }
+    }
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
+}
+
+// Slot variant: as the lane variant, with each connected state found through
+// the slot index under its bucket (bucket bounds, then slots from the
+// state's home until an empty slot or a fingerprint match, whose state and x
+// are loaded together) instead of a bisection of the bucket.  G lookups per
+// lane are carried through the stages together; contributions are added in
+// term order, so y is bit-identical to the other variants.
+template <int NW, bool HAS_Z, bool DOT, int G>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  double prod = 0.0;
+  if (i < b.dim) {
+    const State<NW> s = load_state_cached<NW>(b.states, i);
+    const double xi = x[i];
+    double acc = y_in ? yscale * y_in[i] : 0.0;
+    for (int t0 = 0; t0 < n_off; t0 += 64) {
+      const int tn = min(64, n_off - t0);
+      uint64_t mask = 0;
+      for (int t = 0; t < tn; ++t) {
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
+      }
+      while (mask) {
+        // per lookup: 0 absent, 1 probing, 2 candidate to check, 3 found
+        int tt[G], st[G];
+        State<NW> key[G];
+        uint32_t lo[G], m[G], p[G], fp[G], J[G];
+        double xj[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const bool act = mask != 0;
+          tt[g] = act ? t0 + __builtin_ctzll(mask) : t0;
+          mask &= mask - 1;
+          key[g] = xor_state<NW>(s, off[tt[g]].X);
+          st[g] = 0;
+          lo[g] = 0;
+          m[g] = 0;
+          xj[g] = 0.0;
+          J[g] = 0;
+          if (act) {
+            uint32_t n;
+            bucket_bounds(b.table, bucket_key<NW>(b, key[g]), lo[g], n);
+            m[g] = 2 * n;
+            st[g] = n ? 1 : 0;
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const uint64_t h = slot_hash<NW>(key[g]);
+          p[g] = slot_home(h, m[g]);
+          fp[g] = slot_fp(h);
+        }
+        bool more = true;
+        while (more) {
+          more = false;
+          uint32_t e[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            e[g] = st[g] == 1 ? b.slots[2 * static_cast<uint64_t>(lo[g]) + p[g]] : kSlotEmpty;
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (st[g] == 1) {
+              if (e[g] == kSlotEmpty) {
+                st[g] = 0;
+              } else if ((e[g] >> 16) == fp[g]) {
+                J[g] = lo[g] + (e[g] & 0xFFFFu);
+                st[g] = 2;
+              } else {
+                p[g] = p[g] + 1 == m[g] ? 0 : p[g] + 1;
+                more = true;
+              }
+            }
+          }
+          State<NW> f[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (st[g] == 2) {
+              f[g] = load_state_cached<NW>(b.states, J[g]);
+              xj[g] = x[J[g]];
+            }
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (st[g] == 2) {
+              if (st_eq<NW>(f[g], key[g])) {
+                st[g] = 3;
+              } else {
+                st[g] = 1;
+                p[g] = p[g] + 1 == m[g] ? 0 : p[g] + 1;
+                more = true;
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (st[g] == 3 && fabs(xj[g]) > kApplyTol) {
             const OffTerm& T = off[tt[g]];
             double c = T.c * T.sgn;
             if constexpr (HAS_Z) {
@@ -867,6 +1013,7 @@ void basis_release(qsi_basis_s* b) {
     if (b->d_states) (void)hipFree(b->d_states);
     if (b->d_table) (void)hipFree(b->d_table);
     if (b->d_hash) (void)hipFree(b->d_hash);
+    if (b->d_slots) (void)hipFree(b->d_slots);
     delete b;
   }
 }
@@ -897,7 +1044,13 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   const auto v = op->basis->view();
   int variant = op->variant;
   if (variant == QSI_APPLY_AUTO)
-    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE4;
+    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED
+            : op->basis->d_slots ? QSI_APPLY_SLOT4 : QSI_APPLY_LANE4;
+  // a slot variant chosen while the basis had its slot index, which has been
+  // dropped since, falls back to the bisection of the same depth
+  if (!op->basis->d_slots && variant >= QSI_APPLY_SLOT2 && variant <= QSI_APPLY_SLOT8)
+    variant = variant == QSI_APPLY_SLOT2 ? QSI_APPLY_LANE2
+            : variant == QSI_APPLY_SLOT8 ? QSI_APPLY_LANE8 : QSI_APPLY_LANE4;
   switch (variant) {
     case QSI_APPLY_CACHED:
       // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)
@@ -937,6 +1090,21 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
       hipLaunchKernelGGL((k_apply_hash<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
                          op->basis->hview(), op->d_off, op->n_off, op->d_diag,
                          op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_SLOT2:
+      hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 2>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_SLOT4:
+      hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
+      break;
+    case QSI_APPLY_SLOT8:
+      hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 8>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials);
       break;
     default:  // QSI_APPLY_LANE4
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
@@ -1325,6 +1493,57 @@ int qsi_basis_create(const uint64_t* h_lohi, int64_t dim, int n_spins,
 
 int qsi_basis_has_hash(qsi_basis_t b) { return b && b->d_hash ? 1 : 0; }
 
+int qsi_basis_build_slots(qsi_basis_t b, int enable) {
+  QSI_REQUIRE(b != nullptr, "qsi_basis_build_slots: null basis");
+  QSI_REQUIRE(enable == 0 || enable == 1, "qsi_basis_build_slots: enable must be 0 or 1");
+  if (!enable || b->d_slots) {
+    if (!enable && b->d_slots) {
+      QSI_CHECK_HIP(hipDeviceSynchronize());  // no apply may still read it
+      QSI_CHECK_HIP(hipFree(b->d_slots));
+      b->d_slots = nullptr;
+    }
+    return QSI_OK;
+  }
+  if (b->dim == 0) return QSI_OK;
+  const size_t n = 2 * static_cast<size_t>(b->dim);
+  uint32_t* slots = nullptr;
+  unsigned int* flag = nullptr;
+  hipError_t e = hipMalloc(&slots, n * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_basis_build_slots: hipMalloc: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
+  }
+  e = hipMalloc(&flag, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemset(flag, 0, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemset(slots, 0xFF, n * sizeof(uint32_t));
+  if (e == hipSuccess) {
+    qsi::BasisView v = b->view();
+    const uint32_t grid = static_cast<uint32_t>((b->dim + 255) / 256);
+    if (b->nw == 1)
+      hipLaunchKernelGGL(qsi::k_build_slots<1>, dim3(grid), dim3(256), 0, 0, v, slots, flag);
+    else
+      hipLaunchKernelGGL(qsi::k_build_slots<2>, dim3(grid), dim3(256), 0, 0, v, slots, flag);
+    e = hipGetLastError();
+  }
+  unsigned int too_big = 0;
+  if (e == hipSuccess) e = hipMemcpy(&too_big, flag, sizeof(too_big), hipMemcpyDeviceToHost);
+  if (flag) (void)hipFree(flag);
+  if (e != hipSuccess || too_big) {
+    (void)hipFree(slots);
+    if (e != hipSuccess) {
+      set_error(std::string("qsi_basis_build_slots: ") + hipGetErrorString(e));
+      return QSI_E_HIP;
+    }
+    set_error("qsi_basis_build_slots: a bucket holds more than 65535 states "
+              "(a wider bucket key is needed)");
+    return QSI_E_INVALID;
+  }
+  b->d_slots = slots;
+  return QSI_OK;
+}
+
+int qsi_basis_has_slots(qsi_basis_t b) { return b && b->d_slots ? 1 : 0; }
+
 }  // extern "C"
 
 namespace qsi {
@@ -1617,8 +1836,10 @@ int64_t qsi_op_dim(qsi_op_t op) { return op ? op->basis->dim : -1; }
 
 int qsi_op_set_variant(qsi_op_t op, int variant) {
   QSI_REQUIRE(op != nullptr, "qsi_op_set_variant: null operator");
-  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_CACHED,
+  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_SLOT8,
               "qsi_op_set_variant: unknown variant");
+  QSI_REQUIRE(op->basis->d_slots || variant < QSI_APPLY_SLOT2,
+              "qsi_op_set_variant: the basis has no slot index (qsi_basis_build_slots)");
   QSI_REQUIRE(variant != QSI_APPLY_CACHED || op->conn_nnz >= 0,
               "qsi_op_set_variant: no cached connectivity (qsi_op_cache_connectivity)");
   QSI_REQUIRE(op->basis->d_hash || (variant != QSI_APPLY_HASH && variant != QSI_APPLY_HASH4),

@@ -63,6 +63,7 @@ struct qsi_basis_s {
   uint64_t* d_states = nullptr;  // dim * nw words
   uint32_t* d_table = nullptr;   // 2^kbits + 1
   ulonglong2* d_hash = nullptr;  // hash index (null if not built)
+  uint32_t* d_slots = nullptr;   // slot index, 2 * dim entries (null if not built)
   int hbits = 0;
   std::atomic<int> refs{1};
 
@@ -78,6 +79,7 @@ struct qsi_basis_s {
     qsi::BasisView v;
     v.states = d_states;
     v.table = d_table;
+    v.slots = d_slots;
     v.dim = dim;
     v.kbits = kbits;
     v.n_spins = n_spins;

@@ -94,6 +94,16 @@ class GpuBasis:
     def bucket_bits(self):
         return int(_lib.load().qsi_basis_bucket_bits(self._h))
 
+    def build_slots(self, enable=True):
+        """Build (or drop) the slot index under the bucket table
+        (qsi_basis_build_slots): lookups by hashed slots per bucket instead of
+        bisection; the QSI_APPLY_SLOT* kernels and search() use it."""
+        check(_lib.load().qsi_basis_build_slots(self._h, int(bool(enable))),
+              "qsi_basis_build_slots")
+
+    def has_slots(self):
+        return bool(_lib.load().qsi_basis_has_slots(self._h))
+
     def key_fields(self):
         """Bucket key fields [(lo, width), ...], highest first."""
         n = C.c_int(0)

@@ -52,9 +52,18 @@ def test_search_ranks(W, kbits, cuda):
     assert np.all(basis.search(qs) == -1)
     qs[:, 1] = 1  # hi word set: outside a <= 64-spin basis
     assert np.all(basis.search(qs[:10]) == -1)
+    # the same through the slot index
+    basis.build_slots()
+    assert basis.has_slots()
+    assert np.array_equal(basis.search(b), np.arange(b.shape[0]))
+    assert np.all(basis.search(qs) == -1)
+    qs[:, 1] = 0
+    assert np.all(basis.search(qs) == -1)
+    basis.build_slots(False)
+    assert not basis.has_slots()
 
 
-@pytest.mark.parametrize("variant", ["auto", "window", "lane", "lane8", "hash", "cached"])
+@pytest.mark.parametrize("variant", ["auto", "window", "lane", "lane8", "hash", "slot4", "cached"])
 @pytest.mark.parametrize("W,kind", [(8, "ring"), (32, "ring"), (32, "ring+field"),
                                     (48, "ring+field"), (48, "jpm")])
 def test_apply_parity(W, kind, variant, lattices, cuda):
@@ -63,7 +72,10 @@ def test_apply_parity(W, kind, variant, lattices, cuda):
     b = load_basis(W)
     D = b.shape[0]
     H = _ham(jd, W, kind)
-    op = q.GpuOp(q.GpuBasis(b, W, hash_index=True), H)
+    gb = q.GpuBasis(b, W, hash_index=True)
+    if variant == "slot4":
+        gb.build_slots()
+    op = q.GpuOp(gb, H)
     if variant == "cached":
         op.cache_connectivity(True)
     op.set_variant(variant)
@@ -118,10 +130,13 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
     jd = lattices[W]
     b = load_basis(W)
     H = _ham(jd, W, "jpm" if W == 48 else "ring+field")
-    op = q.GpuOp(q.GpuBasis(b, W, bucket_bits=kbits, hash_index=True), H)
+    basis = q.GpuBasis(b, W, bucket_bits=kbits, hash_index=True)
+    basis.build_slots()
+    op = q.GpuOp(basis, H)
     x = torch.from_numpy(_rand_x(b.shape[0], 9)).cuda()
     ys = []
-    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "cached"):
+    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
+              "slot8", "cached"):
         if v == "cached":
             op.cache_connectivity(True)
         op.set_variant(v)
@@ -156,8 +171,11 @@ def test_apply_partial_basis(lattices, cuda):
     jd = lattices[32]
     b = load_basis(32)[::3].copy()
     H = _ham(jd, 32, "ring+field")
-    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4"):
-        op = q.GpuOp(q.GpuBasis(b, 32, hash_index=True), H)
+    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
+              "slot8"):
+        basis = q.GpuBasis(b, 32, hash_index=True)
+        basis.build_slots()
+        op = q.GpuOp(basis, H)
         op.set_variant(v)
         x = _rand_x(b.shape[0], 4, tiny=False)
         y = torch.zeros(b.shape[0], dtype=torch.float64, device="cuda")
@@ -205,16 +223,27 @@ def test_apply_parity_two_words(lattices, cuda):
     for i in range(48):  # a field on the real spins only
         H.add_term(0.013 * ((i % 4) - 1.5), q.SymbolicPMROperator.from_ops("z", [f(i)]))
     H.add_term(0.021, q.SymbolicPMROperator.from_ops("zz", [f(3), f(40)]))
-    op = q.GpuOp(q.GpuBasis(lohi, W2, hash_index=True), H)
+    gb = q.GpuBasis(lohi, W2, hash_index=True)
+    gb.build_slots()
+    op = q.GpuOp(gb, H)
     x = _rand_x(len(states), 5)
     ref = orc.evaluate_add(orc.Basis(lohi), H, x, np.zeros(len(states)))
-    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4"):
+    for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
+              "slot8"):
         op.set_variant(v)
         yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
         op.evaluate(torch.from_numpy(x).cuda(), yd)
         assert np.linalg.norm(yd.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref)
     basis = q.GpuBasis(lohi, W2)
     assert np.array_equal(basis.search(lohi), np.arange(len(states)))
+    basis.build_slots()
+    assert np.array_equal(basis.search(lohi), np.arange(len(states)))
+    # a slot variant falls back to the bisection once the index is dropped
+    op.set_variant("slot4")
+    gb.build_slots(False)
+    yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
+    op.evaluate(torch.from_numpy(x).cuda(), yd)
+    assert np.linalg.norm(yd.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref)
 
 
 @pytest.mark.parametrize("R", [1, 3, 16, 64])

@@ -40,6 +40,7 @@ def c4(cuda):
     B = 0.05 * b / np.linalg.norm(b)
     H = q.build_hamiltonian_g(jd, [-v for v in g])
     q.add_local_field(H, jd, B)
+    basis.build_slots()
     op = q.GpuOp(basis, H)
     states = basis.states()
     yield {"jd": jd, "basis": basis, "op": op, "H": H, "states": states}
@@ -90,14 +91,20 @@ def test_c4_apply_parity(c4, cuda):
     del ob
     xd = torch.from_numpy(x).cuda()
     del x
-    yd = torch.full((D,), 3.0, dtype=torch.float64, device="cuda")
-    c4["op"].evaluate(xd, yd)
-    del xd
     yr = torch.from_numpy(y_ref).cuda()
     del y_ref
     nz = int((yr != 0).sum())
     assert nz > N_ROWS  # the pushes land far outside the source rows
-    err = float(torch.linalg.norm(yd - yr) / torch.linalg.norm(yr))
-    assert err <= HV_TOL, err
-    # rows no source connects to stay exactly zero
-    assert bool(((yr == 0) == (yd == 0)).all())
+    yd = torch.empty(D, dtype=torch.float64, device="cuda")
+    prev = None
+    for v in ("slot4", "lane4"):  # slot index and bucket bisection
+        c4["op"].set_variant(v)
+        yd.fill_(3.0)
+        c4["op"].evaluate(xd, yd)
+        err = float(torch.linalg.norm(yd - yr) / torch.linalg.norm(yr))
+        assert err <= HV_TOL, (v, err)
+        # rows no source connects to stay exactly zero
+        assert bool(((yr == 0) == (yd == 0)).all()), v
+        if prev is not None:
+            assert torch.equal(prev, yd)
+        prev = yd.clone()

@@ -163,3 +163,48 @@ def test_block_autotune_multi_sector(lattices, cuda):
     sc = np.max(np.abs(a0))
     assert np.max(np.abs(a0 - a1)) <= 1e-12 * sc
     assert np.max(np.abs(b0 - b1)) <= 1e-12 * sc
+
+
+def test_c3_two_sectors_against_oracle(lattices, cuda):
+    """The batched FTLM workload itself: the 48-spin sectors n = 0 and n = 1
+    (uneven sizes) in one launch, R = 64 random-sign start vectors each,
+    M = 100 steps, g_s ~ U[0.2, 1] (seeded), B = 0, with the autotuned block
+    configuration and graph capture.
+      * 4 columns per sector: (alpha, beta) of the first 25 steps against
+        the oracle's single-vector recurrence from the same start vector
+        (lanczos.cpp:111-146), 1e-10 relative;
+      * every column at full M: the Gauss quadrature of T_M reproduces the
+        sample's moments <r|H^p|r> for p = 0..3 (exact for p < 2M),
+        computed with the oracle's H (the reference's push apply), 1e-10
+        relative."""
+    torch = cuda
+    jd = lattices[48]
+    rng = np.random.default_rng(77)
+    g = rng.uniform(0.2, 1.0, size=4)
+    H = _ham(jd, g)
+    bases = [q.enumerate_basis(jd, n, n_threads=8) for n in (0, 1)]
+    assert bases[0].shape[0] == 87546
+    ops = [q.GpuOp(q.GpuBasis(bb, 48), H) for bb in bases]
+    R, M = 64, 100
+    V0 = [ftlm.random_sign_block(bb.shape[0], R, seed=77 + k) for k, bb in enumerate(bases)]
+    ftlm.autotune(ops, R, reps=1)
+    al, be, _ = ftlm.lanczos_block(ops, [torch.from_numpy(v).cuda() for v in V0], M,
+                                   use_graph=True)
+    for s in range(2):
+        ob = orc.Basis(bases[s])
+        D = bases[s].shape[0]
+        for c in (0, 21, 42, 63):
+            oa, obb = orc.lanczos_coeffs(ob, H, V0[s][:, c], 25)
+            scale = np.max(np.abs(oa))
+            assert np.max(np.abs(al[s, c, :25] - oa)) <= 1e-10 * scale, (s, c)
+            assert np.max(np.abs(be[s, c, :25] - obb)) <= 1e-10 * scale, (s, c)
+        nodes = ftlm.quadrature(al[s], be[s])
+        for c in range(R):
+            r = V0[s][:, c]
+            h1 = orc.evaluate_add(ob, H, r, np.zeros(D))
+            h2 = orc.evaluate_add(ob, H, h1, np.zeros(D))
+            mom = [r @ r, r @ h1, h1 @ h1, h1 @ h2]
+            e, w = nodes[c]
+            got = [np.sum(w * e ** p) for p in range(4)]
+            for p in range(4):
+                assert abs(got[p] - mom[p]) <= 1e-10 * max(1.0, abs(mom[p])), (s, c, p)
