@@ -144,9 +144,10 @@ int qsi_op_n_diag(qsi_op_t op);
  * connected rows search a 64-state window staged in LDS.  QSI_APPLY_CACHED:
  * gathers through the cached connectivity (qsi_op_cache_connectivity; setting
  * it requires the cache, dropping the cache resets it to AUTO).
- * QSI_APPLY_SLOT{2,4,8}: as LANE*, through the slot index under each bucket
- * (qsi_basis_build_slots; setting one requires the index; if the index is
- * dropped later the kernel falls back to LANE of the same depth).  AUTO =
+ * QSI_APPLY_SLOT{2,3,4}: as LANE*, through the slot index under each bucket
+ * (qsi_basis_build_slots), 2/3/4 lookups in flight per lane; setting one
+ * requires the index; if the index is dropped later the apply falls back to
+ * LANE4 (SLOT4) or LANE2.  AUTO =
  * CACHED when the connectivity is cached, else SLOT4 when the basis has a slot
  * index, else LANE4.  All give bit-identical y. */
 enum {
@@ -161,7 +162,7 @@ enum {
   QSI_APPLY_CACHED = 8,
   QSI_APPLY_SLOT2 = 9,
   QSI_APPLY_SLOT4 = 10,
-  QSI_APPLY_SLOT8 = 11
+  QSI_APPLY_SLOT3 = 11
 };
 int qsi_op_set_variant(qsi_op_t op, int variant);
 int qsi_op_get_variant(qsi_op_t op);
@@ -324,6 +325,29 @@ int qsi_lanczos_block(const qsi_op_t *ops, int n_sectors, int R,
                       const double *const *d_V0, const double *const *d_coeffs,
                       int M, double *h_alpha, double *h_beta, int use_graph,
                       double *elapsed_ms);
+
+/* One FTLM sample with operator expectations, the vector loop of
+ * ftlm_computer::evolve (include/ftlm.hpp:56-91) for one start vector, with
+ * textbook semantics (the reference's loop does not build: its krylov_expv
+ * is undefined and the sign of its step is reversed).  From d_r (D doubles,
+ * device; normalised internally, not modified) M steps of the 3-term
+ * recurrence give T_n (n = *n_steps <= M, fewer when a beta underflows) =
+ * Y diag(e) Y^T; for every beta_b (h_betas, n_beta values)
+ *   psi_b = exp(-beta_b (H - e_shift) / 2) r = V Y exp(-beta_b (e - e_shift)/2) Y^T e_1
+ * is built on the device by a second, identical pass over the Lanczos vectors
+ * V, and
+ *   h_norm[b] = <psi_b|psi_b>,  h_expect[l * n_beta + b] = <psi_b|O_l|psi_b>
+ * for the n_ops operators ops[l] (on the same basis size as H).  Summed over
+ * samples these estimate Tr(exp(-beta (H - e_shift)) O) and the shifted
+ * partition function; a complete orthonormal start set gives them exactly.
+ * e_shift: a finite value, or NaN for the sample's lowest Ritz value; *e_used
+ * (optional) receives the shift used.  h_alpha, h_beta (M doubles) receive the
+ * coefficients as qsi_lanczos_coeffs gives them.  Device memory: 3 D + n_beta
+ * D doubles.  Synchronises. */
+int qsi_ftlm_sample(qsi_op_t H, const qsi_op_t *ops, int n_ops, const double *d_r, int M,
+                    const double *h_betas, int n_beta, double e_shift, double *e_used,
+                    double *h_norm, double *h_expect, double *h_alpha, double *h_beta,
+                    int *n_steps);
 
 /* Eigenvalues (ascending) of the symmetric tridiagonal (d: n, e: n-1) and,
  * if first_row != NULL, the first component of each unit eigenvector (the

@@ -22,11 +22,13 @@
  *                         (include/matrix_diag_bits.hpp:51-95) with the Krylov
  *                         loop on the device (thick-restart Lanczos, lowest
  *                         eigenvalues).
- *   qsi::FtlmComputer     the method names of ftlm_computer (include/ftlm.hpp:
- *                         22-122) over textbook FTLM: each random start vector
- *                         runs krylov_dim Lanczos steps, the Ritz values and
- *                         the squared first components of the Ritz vectors are
- *                         the quadrature of Z(beta), E(beta), C(beta).
+ *   qsi::FtlmComputer     ftlm_computer (include/ftlm.hpp:22-122):
+ *                         (ops, H, T_min, T_max, n_Ts), evolve(rng),
+ *                         set_numerical_params, get_beta_grid, write_to_h5,
+ *                         over textbook FTLM: each random start vector runs
+ *                         krylov_dim Lanczos steps, whose Ritz pairs give
+ *                         Z(beta), E(beta), C(beta) and, per operator,
+ *                         sums of <psi|O|psi> with psi = exp(-beta H / 2) r.
  *
  * Every failing call throws qsi::Error (a std::runtime_error carrying
  * qsi_last_error()), as the reference's operator code throws
@@ -287,12 +289,32 @@ inline EigshResult eigsh(const GpuOp& op, int nev, int ncv, double tol, int max_
 // far) keeps the exponentials finite.  Over n samples of D-dimensional
 // random vectors Z = (D / n) sum_r Z_r exp(-beta e_ref); E = sum E_r /
 // sum Z_r; C = beta^2 (sum E2_r / sum Z_r - E^2).  A complete orthonormal
-// start set (n = D) gives the exact traces.  The reference's per-operator
-// sums (write_to_h5) come from its e^{-beta H / 2} evolution, which this does
-// not restate; tol is kept for the interface and not used.
+// start set (n = D) gives the exact traces.
+//
+// With operators (the reference's ops_to_evaluate), each sample also builds
+// psi_r(beta) = exp(-beta (H - e_ref) / 2) r on the device (qsi_ftlm_sample:
+// the Lanczos vectors of a second, identical pass combined with the
+// coefficients of exp(-beta T_M / 2) e_1), and accumulates per operator O and
+// beta
+//   op_sum[O](beta)   = sum_r <psi_r|O|psi_r>
+//   op_sum_2[O](beta) = sum_r <psi_r|O|psi_r>^2
+//   norm_sum(beta)    = sum_r <psi_r|psi_r>
+// (all with the same shift e_ref), so <O>(beta) = op_sum / norm_sum; with a
+// complete orthonormal start set op_sum = Tr(exp(-beta (H - e_ref)) O)
+// exactly.  write_to_h5 writes op_sum and op_sum_2 as the datasets
+// "<name>_sum" and "<name>_sum_2" (ftlm.hpp:113-124; available when
+// <hdf5.h> is included before this header).  The reference's own loop
+// (ftlm.hpp:56-91) does not build and divides its running sums by each
+// sample's norm; these are the sums its comments describe.  tol is kept for
+// the interface and not used.
 class FtlmComputer {
  public:
-  FtlmComputer(const GpuOp& H, double T_min, double T_max, int n_Ts) : op_(H), H_(H.handle()) {
+  FtlmComputer(const GpuOp& H, double T_min, double T_max, int n_Ts)
+      : FtlmComputer(std::vector<GpuOp>{}, H, T_min, T_max, n_Ts) {}
+
+  FtlmComputer(const std::vector<GpuOp>& ops, const GpuOp& H, double T_min, double T_max,
+               int n_Ts)
+      : op_(H), H_(H.handle()), ops_(ops) {
     if (T_min >= T_max) throw std::runtime_error("Invalid spec: T_max must be greater than T_min");
     if (n_Ts < 2) throw std::runtime_error("Invalid spec: n_Ts must be at least 2");
     const double b0 = 1.0 / T_max, b1 = 1.0 / T_min, db = (b1 - b0) / (n_Ts - 1);
@@ -300,6 +322,14 @@ class FtlmComputer {
     z_.assign(beta_.size(), 0.0);
     e_.assign(beta_.size(), 0.0);
     e2_.assign(beta_.size(), 0.0);
+    norm_.assign(beta_.size(), 0.0);
+    for (const GpuOp& o : ops_) {
+      if (o.rows() != H.rows())
+        throw std::runtime_error("ftlm_computer: operator and Hamiltonian sizes differ");
+      op_h_.push_back(o.handle());
+    }
+    osum_.assign(ops_.size() * beta_.size(), 0.0);
+    osum2_.assign(ops_.size() * beta_.size(), 0.0);
   }
 
   void set_numerical_params(int krylov_dim = 30, double tol = 1e-10) {
@@ -312,7 +342,8 @@ class FtlmComputer {
   // D draws, then normalised).
   void evolve(std::mt19937& rng) { evolve_block(rng, 1); }
 
-  // R samples (R <= 64) from rng in turn, run side by side in one launch.
+  // R samples (R <= 64) from rng in turn; without operators they run side by
+  // side in one launch, with operators one after another.
   void evolve_block(std::mt19937& rng, int R) {
     const int64_t D = qsi_op_dim(H_);
     std::vector<double> V(static_cast<std::size_t>(D) * R);
@@ -334,20 +365,52 @@ class FtlmComputer {
   void evolve_vectors(const double* h_V0, int R) {
     const int64_t D = qsi_op_dim(H_);
     const int M = krylov_dim_;
-    DeviceVector V0(h_V0, static_cast<std::size_t>(D) * R);
-    std::vector<double> al(static_cast<std::size_t>(R) * M), be(static_cast<std::size_t>(R) * M);
-    const double* v0 = V0.data();
-    throw_if(qsi_lanczos_block(&H_, 1, R, &v0, nullptr, M, al.data(), be.data(), 1, nullptr),
-             "ftlm evolve");
     std::vector<double> ev(M), fr(M);
-    for (int c = 0; c < R; ++c) {
-      const double* a = al.data() + static_cast<std::size_t>(c) * M;
-      const double* b = be.data() + static_cast<std::size_t>(c) * M;
-      // a beta that underflows ends the sample's recurrence
-      int m = 1;
-      while (m < M && b[m - 1] != 0.0) ++m;
-      throw_if(qsi_tridiag_eig(a, m > 1 ? b : nullptr, m, ev.data(), fr.data()), "ftlm quadrature");
-      add_sample(ev.data(), fr.data(), m);
+    if (!ops_.empty()) {
+      std::vector<double> col(static_cast<std::size_t>(D)), al(M), be(M);
+      const std::size_t nb = beta_.size(), no = ops_.size();
+      std::vector<double> nrm(nb), ex(no * nb);
+      for (int c = 0; c < R; ++c) {
+        for (int64_t i = 0; i < D; ++i) col[static_cast<std::size_t>(i)] = h_V0[i * R + c];
+        DeviceVector r(col.data(), col.size());
+        int m = 0;
+        double used = 0.0;
+        throw_if(qsi_ftlm_sample(H_, op_h_.data(), static_cast<int>(no), r.data(), M,
+                                 beta_.data(), static_cast<int>(nb),
+                                 std::numeric_limits<double>::quiet_NaN(), &used, nrm.data(),
+                                 ex.data(), al.data(), be.data(), &m),
+                 "ftlm evolve");
+        throw_if(qsi_tridiag_eig(al.data(), m > 1 ? be.data() : nullptr, m, ev.data(), fr.data()),
+                 "ftlm quadrature");
+        // the sample's shift is its lowest Ritz value; bring both to the lower
+        shift_to(std::min(e_ref_, used));
+        for (std::size_t j = 0; j < nb; ++j) {
+          const double f = std::exp(-beta_[j] * (used - e_ref_));  // <= 1
+          norm_[j] += f * nrm[j];
+          for (std::size_t l = 0; l < no; ++l) {
+            const double o = f * ex[l * nb + j];
+            osum_[l * nb + j] += o;
+            osum2_[l * nb + j] += o * o;
+          }
+        }
+        add_sample(ev.data(), fr.data(), m);
+      }
+    } else {
+      DeviceVector V0(h_V0, static_cast<std::size_t>(D) * R);
+      std::vector<double> al(static_cast<std::size_t>(R) * M), be(static_cast<std::size_t>(R) * M);
+      const double* v0 = V0.data();
+      throw_if(qsi_lanczos_block(&H_, 1, R, &v0, nullptr, M, al.data(), be.data(), 1, nullptr),
+               "ftlm evolve");
+      for (int c = 0; c < R; ++c) {
+        const double* a = al.data() + static_cast<std::size_t>(c) * M;
+        const double* b = be.data() + static_cast<std::size_t>(c) * M;
+        // a beta that underflows ends the sample's recurrence
+        int m = 1;
+        while (m < M && b[m - 1] != 0.0) ++m;
+        throw_if(qsi_tridiag_eig(a, m > 1 ? b : nullptr, m, ev.data(), fr.data()),
+                 "ftlm quadrature");
+        add_sample(ev.data(), fr.data(), m);
+      }
     }
     n_samples_ += R;
     dim_ = D;
@@ -383,21 +446,67 @@ class FtlmComputer {
     }
     return c;
   }
+  // Operator sums over the samples so far (shift e_ref), per beta.
+  std::vector<double> op_sum(int op_idx) const { return slice(osum_, op_idx); }
+  std::vector<double> op_sum_2(int op_idx) const { return slice(osum2_, op_idx); }
+  const std::vector<double>& norm_sum() const { return norm_; }
+  // <O>(beta) = op_sum / norm_sum
+  std::vector<double> op_expectation(int op_idx) const {
+    std::vector<double> o = op_sum(op_idx);
+    for (std::size_t j = 0; j < o.size(); ++j) o[j] /= norm_[j];
+    return o;
+  }
+
+#ifdef H5_VERS_MAJOR
+  // "<name>_sum" and "<name>_sum_2", one value per beta (ftlm.hpp:113-124).
+  void write_to_h5(hid_t file_id, int op_idx, const std::string& dset_name) const {
+    write_1d(file_id, dset_name + "_sum", op_sum(op_idx));
+    write_1d(file_id, dset_name + "_sum_2", op_sum_2(op_idx));
+  }
+#endif
 
  private:
-  void add_sample(const double* ev, const double* fr, int m) {
-    const double lo = *std::min_element(ev, ev + m);
-    if (lo < e_ref_) {
-      // move the sums already taken to the new reference
-      if (std::isfinite(e_ref_))
-        for (std::size_t j = 0; j < beta_.size(); ++j) {
-          const double f = std::exp(-beta_[j] * (e_ref_ - lo));
-          z_[j] *= f;
-          e_[j] *= f;
-          e2_[j] *= f;
+  std::vector<double> slice(const std::vector<double>& a, int op_idx) const {
+    if (op_idx < 0 || static_cast<std::size_t>(op_idx) >= ops_.size())
+      throw std::out_of_range("ftlm_computer: operator index out of range");
+    const std::size_t nb = beta_.size();
+    return std::vector<double>(a.begin() + op_idx * nb, a.begin() + (op_idx + 1) * nb);
+  }
+
+#ifdef H5_VERS_MAJOR
+  static void write_1d(hid_t fid, const std::string& name, const std::vector<double>& v) {
+    const hsize_t dims[1] = {static_cast<hsize_t>(v.size())};
+    const hid_t sp = H5Screate_simple(1, dims, nullptr);
+    const hid_t ds = H5Dcreate2(fid, name.c_str(), H5T_NATIVE_DOUBLE, sp, H5P_DEFAULT,
+                                H5P_DEFAULT, H5P_DEFAULT);
+    const herr_t rc = ds < 0 ? -1 : H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT,
+                                             v.data());
+    if (ds >= 0) H5Dclose(ds);
+    H5Sclose(sp);
+    if (rc < 0) throw std::runtime_error("ftlm_computer: cannot write dataset " + name);
+  }
+#endif
+
+  // Moves every sum to the reference energy lo (lo <= e_ref_).
+  void shift_to(double lo) {
+    if (!(lo < e_ref_)) return;
+    if (std::isfinite(e_ref_))
+      for (std::size_t j = 0; j < beta_.size(); ++j) {
+        const double f = std::exp(-beta_[j] * (e_ref_ - lo));
+        z_[j] *= f;
+        e_[j] *= f;
+        e2_[j] *= f;
+        norm_[j] *= f;
+        for (std::size_t l = 0; l < ops_.size(); ++l) {
+          osum_[l * beta_.size() + j] *= f;
+          osum2_[l * beta_.size() + j] *= f * f;
         }
-      e_ref_ = lo;
-    }
+      }
+    e_ref_ = lo;
+  }
+
+  void add_sample(const double* ev, const double* fr, int m) {
+    shift_to(*std::min_element(ev, ev + m));
     for (std::size_t j = 0; j < beta_.size(); ++j) {
       for (int k = 0; k < m; ++k) {
         const double w = fr[k] * fr[k] * std::exp(-beta_[j] * (ev[k] - e_ref_));
@@ -410,9 +519,11 @@ class FtlmComputer {
 
   GpuOp op_;  // keeps the operator alive
   qsi_op_t H_;
+  std::vector<GpuOp> ops_;
+  std::vector<qsi_op_t> op_h_;
   int krylov_dim_ = 30;
   double tol_ = 1e-10;
-  std::vector<double> beta_, z_, e_, e2_;
+  std::vector<double> beta_, z_, e_, e2_, norm_, osum_, osum2_;
   double e_ref_ = std::numeric_limits<double>::infinity();
   int64_t n_samples_ = 0;
   int64_t dim_ = 0;

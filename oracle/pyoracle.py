@@ -68,6 +68,8 @@ def load():
             "oracle_jacobi_eig": (None, [_P, C.c_int, _P, _P]),
             "oracle_tridiag_lowest": (C.c_double, [_P, _P, C.c_int64]),
             "oracle_tridiag_one": (None, [_P, _P, C.c_int64, C.POINTER(C.c_double), _P]),
+            "oracle_brute_basis": (C.c_int64, [C.c_int, C.c_int, _P, _P, C.c_int, _P, _P,
+                                               C.c_int64]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -314,3 +316,19 @@ def tridiag_one(d, e):
     v = np.empty(d.shape[0])
     load().oracle_tridiag_one(_ptr(d), _ptr(e), d.shape[0], C.byref(e0), _ptr(v))
     return e0.value, v
+
+
+def brute_basis(n_spins, tetra_members, order, n_pairs, cap=1 << 26):
+    """Every state of a <= 32-spin lattice replayed along the visiting order
+    with the basis builder's rule (brute_basis.cpp): (states ascending, count
+    of disallowed positions per state) for all states with at most n_pairs."""
+    lib = load()
+    tet = np.ascontiguousarray(tetra_members, dtype=np.int32)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    out = np.zeros(cap, dtype=np.uint64)
+    bad = np.zeros(cap, dtype=np.int8)
+    n = lib.oracle_brute_basis(int(n_spins), tet.shape[0], _ptr(tet), _ptr(order), int(n_pairs),
+                               _ptr(out), _ptr(bad), cap)
+    if n < 0 or n > cap:
+        raise ValueError("oracle_brute_basis: bad lattice or too many states")
+    return out[:n].copy(), bad[:n].copy()

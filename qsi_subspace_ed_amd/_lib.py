@@ -28,11 +28,11 @@ QSI_APPLY_HASH4 = 7
 QSI_APPLY_CACHED = 8
 QSI_APPLY_SLOT2 = 9
 QSI_APPLY_SLOT4 = 10
-QSI_APPLY_SLOT8 = 11
+QSI_APPLY_SLOT3 = 11
 VARIANTS = {"auto": QSI_APPLY_AUTO, "window": QSI_APPLY_WINDOW, "lane": QSI_APPLY_LANE,
             "lane2": QSI_APPLY_LANE2, "lane4": QSI_APPLY_LANE4, "lane8": QSI_APPLY_LANE8,
             "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4, "cached": QSI_APPLY_CACHED,
-            "slot2": QSI_APPLY_SLOT2, "slot4": QSI_APPLY_SLOT4, "slot8": QSI_APPLY_SLOT8}
+            "slot2": QSI_APPLY_SLOT2, "slot3": QSI_APPLY_SLOT3, "slot4": QSI_APPLY_SLOT4}
 
 
 class QsiError(RuntimeError):
@@ -127,6 +127,8 @@ SIGNATURES = {
                             _P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsi_lanczos_block": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P),
                                     C.POINTER(_P), C.c_int, _DP, _DP, C.c_int, _DP]),
+    "qsi_ftlm_sample": (C.c_int, [_P, C.POINTER(_P), C.c_int, _P, C.c_int, _DP, C.c_int, _D,
+                                  _DP, _DP, _DP, _DP, _DP, C.POINTER(C.c_int)]),
     "qsi_tridiag_eig": (C.c_int, [_DP, _DP, C.c_int, _DP, _DP]),
     "qsi_lanczos_ritz": (C.c_int, [_DP, _DP, C.c_int, _D, C.c_int, _DP, _DP,
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -69,17 +69,34 @@ CPP_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "wrapper_check.cpp")
 CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "wrapper_check")
 
 
+FTLM_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "ftlm_check.cpp")
+FTLM_CHECK = os.path.join(ROOT, "tests", "cpp", "ftlm_check")
+HDF5_PREFIX = os.environ.get("QSI_HDF5_PREFIX", "/opt/conda")
+
+
 def build_cpp_check():
-    """Compile the C++ face's check program (tests/cpp/wrapper_check) against
-    include/qsi_gpu.hpp and libqsi_gpu.so (found through an $ORIGIN rpath)."""
+    """Compile the C++ face's check programs against include/qsi_gpu.hpp and
+    libqsi_gpu.so (found through an $ORIGIN rpath): tests/cpp/wrapper_check,
+    and tests/cpp/ftlm_check (FtlmComputer::write_to_h5) when the HDF5 C
+    library's headers are present."""
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    common = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+              "-I" + os.path.join(rocm, "include"), "-I" + os.path.join(ROOT, "include")]
+    link = ["-L" + PKG, "-lqsi_gpu", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+            "-Wl,-rpath,$ORIGIN/../../qsi_subspace_ed_amd", "-Wl,-rpath," + os.path.join(rocm, "lib")]
     tmp = CPP_CHECK + ".tmp"
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
-          "-I" + os.path.join(rocm, "include"), "-I" + os.path.join(ROOT, "include"),
-          CPP_CHECK_SRC, "-L" + PKG, "-lqsi_gpu", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
-          "-Wl,-rpath,$ORIGIN/../../qsi_subspace_ed_amd", "-Wl,-rpath," + os.path.join(rocm, "lib"),
-          "-o", tmp])
+    _run(common + [CPP_CHECK_SRC] + link + ["-o", tmp])
     os.replace(tmp, CPP_CHECK)
+    h5inc = os.path.join(HDF5_PREFIX, "include")
+    h5lib = os.path.join(HDF5_PREFIX, "lib")
+    if os.path.exists(os.path.join(h5inc, "hdf5.h")):
+        tmp = FTLM_CHECK + ".tmp"
+        # the system C++ runtime must be found before the HDF5 prefix's older one
+        sysl = "/usr/lib/x86_64-linux-gnu"
+        _run(common + ["-I" + h5inc, FTLM_CHECK_SRC] + link +
+             [os.path.join(h5lib, "libhdf5.so"), "-Wl,-rpath," + sysl, "-Wl,-rpath," + h5lib,
+              "-o", tmp])
+        os.replace(tmp, FTLM_CHECK)
     return CPP_CHECK
 
 

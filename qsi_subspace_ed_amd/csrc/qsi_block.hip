@@ -1325,8 +1325,8 @@ extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double
     }
     if (op->basis->d_slots) {
       cand.push_back(QSI_APPLY_SLOT2);
+      cand.push_back(QSI_APPLY_SLOT3);
       cand.push_back(QSI_APPLY_SLOT4);
-      cand.push_back(QSI_APPLY_SLOT8);
     }
     // the cached kernel once per gather depth (4, 8, 16, 32)
     std::vector<int> pipes(cand.size(), 8);

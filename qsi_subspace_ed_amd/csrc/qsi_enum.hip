@@ -28,7 +28,6 @@ namespace qsi {
 namespace {
 
 constexpr size_t kFrontier = size_t{1} << 20;
-constexpr int kEnumMaxSpins = 128;
 
 struct EnumDev {
   const ulonglong2* tmask;  // tetrahedron masks in visiting order
@@ -115,9 +114,47 @@ __device__ __forceinline__ void unpermute(const EnumDev& L, uint64_t lo, uint64_
   }
 }
 
+// Per-level flags of the walk, one bit per visiting position (< 128) in two
+// words, so the walk keeps no per-level arrays (which would live in scratch
+// memory) and backtracks with one count-leading-zeros.
+struct Bits128 {
+  uint64_t w[2];
+};
+
+__device__ __forceinline__ void bset(Bits128& b, int k, bool v) {
+  const uint64_t m = 1ull << (k & 63);
+  uint64_t& x = b.w[k >> 6];
+  x = v ? (x | m) : (x & ~m);
+}
+
+__device__ __forceinline__ bool bget(const Bits128& b, int k) {
+  return (b.w[k >> 6] >> (k & 63)) & 1ull;
+}
+
+// Highest set bit below position k, or -1.
+__device__ __forceinline__ int bhigh_below(const Bits128& b, int k) {
+  uint64_t lo, hi;
+  low_mask(k, lo, hi);
+  const uint64_t h = b.w[1] & hi, l = b.w[0] & lo;
+  if (h) return 127 - __clzll(h);
+  if (l) return 63 - __clzll(l);
+  return -1;
+}
+
+// Set bits below position k.
+__device__ __forceinline__ int bcount_below(const Bits128& b, int k) {
+  uint64_t lo, hi;
+  low_mask(k, lo, hi);
+  return __popcll(b.w[0] & lo) + __popcll(b.w[1] & hi);
+}
+
 // Depth-first walk of one frontier subtree: counts its leaves (FILL = false)
-// or writes them from out[offset[i]] on (FILL = true).  The walk order is the
-// same in both passes.
+// or writes them from out[offset[i]] on (FILL = true).  Down child first,
+// then up (the order of fork_state_impl); the walk order is the same in both
+// passes.  Per level the walk remembers whether the up child is still to
+// visit (pend), whether that child spends a spinon pair (pend_pair), and
+// whether the child taken spent one (spent); a backtrack jumps straight to
+// the deepest level with a pending up child.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_enum(EnumDev L, const ENode* __restrict__ nodes,
                                               int64_t n_nodes, uint64_t* __restrict__ counts,
@@ -127,8 +164,6 @@ __global__ __launch_bounds__(256) void k_enum(EnumDev L, const ENode* __restrict
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n_nodes) return;
   const ENode root = nodes[i];
-  uint8_t opts[kEnumMaxSpins];
-  uint8_t prs[kEnumMaxSpins];
   uint64_t lo = root.lo, hi = root.hi;
   uint64_t n_leaves = 0;
   uint64_t pos = FILL ? offsets[i] : 0;
@@ -143,38 +178,55 @@ __global__ __launch_bounds__(256) void k_enum(EnumDev L, const ENode* __restrict
       ++n_leaves;
     }
   };
+  auto set_spin = [&](int d, int b) {
+    if (d < 64) lo = (lo & ~(1ull << d)) | (static_cast<uint64_t>(b) << d);
+    else hi = (hi & ~(1ull << (d - 64))) | (static_cast<uint64_t>(b) << (d - 64));
+  };
   const int d0 = root.spin;
   if (d0 == L.n) {
     emit();
   } else {
+    Bits128 pend{{0, 0}}, pend_pair{{0, 0}}, spent{{0, 0}};
     int d = d0;
-    prs[d] = static_cast<uint8_t>(root.pairs);
-    opts[d] = enum_children(L, lo, hi, d, root.pairs);
+    int pairs = root.pairs;
     while (true) {
-      const uint8_t o = opts[d];
-      if ((o & 3) == 0) {
-        if (d == d0) break;
-        --d;
-        continue;
-      }
-      const int b = (o & 1) ? 0 : 1;  // down child first
-      opts[d] = static_cast<uint8_t>(o & ~(1 << b));
-      const int np = prs[d] + ((o >> (2 + b)) & 1);
-      // spin d takes value b; bits above d are left over from other branches
-      // and masked off where they are read
-      if (d < 64) lo = (lo & ~(1ull << d)) | (static_cast<uint64_t>(b) << d);
-      else hi = (hi & ~(1ull << (d - 64))) | (static_cast<uint64_t>(b) << (d - 64));
-      if (d + 1 == L.n) {
-        uint64_t klo, khi;
-        low_mask(L.n, klo, khi);
-        lo &= klo;
-        hi &= khi;
+      bool back = false;
+      if (d == L.n) {
+        // bits above n are never set: every branch assigns bits below n only
         emit();
-        continue;
+        back = true;
+      } else {
+        // bits at and above d are left over from other branches and are
+        // masked off where they are read
+        const uint8_t o = enum_children(L, lo, hi, d, pairs);
+        if (o & 1) {
+          set_spin(d, 0);
+          const bool sp = (o >> 2) & 1;
+          bset(spent, d, sp);
+          bset(pend, d, (o & 2) != 0);
+          bset(pend_pair, d, (o >> 3) & 1);
+          pairs += sp;
+          ++d;
+        } else if (o & 2) {
+          set_spin(d, 1);
+          const bool sp = (o >> 3) & 1;
+          bset(spent, d, sp);
+          bset(pend, d, false);
+          pairs += sp;
+          ++d;
+        } else {
+          back = true;
+        }
       }
-      ++d;
-      prs[d] = static_cast<uint8_t>(np);
-      opts[d] = enum_children(L, lo, hi, d, np);
+      if (back) {
+        const int p = bhigh_below(pend, d);
+        if (p < d0) break;
+        bset(pend, p, false);
+        set_spin(p, 1);
+        bset(spent, p, bget(pend_pair, p));
+        pairs = root.pairs + bcount_below(spent, p + 1);  // no bit below d0 is set
+        d = p + 1;
+      }
     }
   }
   if constexpr (!FILL) counts[i] = n_leaves;

@@ -271,7 +271,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
 // state's home until an empty slot or a fingerprint match, whose state and x
 // are loaded together) instead of a bisection of the bucket.  G lookups per
 // lane are carried through the stages together; contributions are added in
-// term order, so y is bit-identical to the other variants.
+// term order, so y is bit-identical to the other variants.  A lookup keeps
+// only its bucket start, size and fingerprint, probe position, candidate
+// rank and term (the connected state is re-formed from the row and the
+// term's flip mask where needed), so four lookups fit the register budget of
+// seven waves per SIMD.
 template <int NW, bool HAS_Z, bool DOT, int G>
 __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
@@ -293,34 +297,31 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
         mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
       }
       while (mask) {
-        // per lookup: 0 absent, 1 probing, 2 candidate to check, 3 found
-        int tt[G], st[G];
-        State<NW> key[G];
-        uint32_t lo[G], m[G], p[G], fp[G], J[G];
+        // per lookup: stage (bits 8-9: 0 absent, 1 probing, 2 candidate,
+        // 3 found) | term offset in the chunk (bits 0-7); bucket start;
+        // bucket size << 16 | fingerprint; probe position; candidate rank
+        uint32_t tg[G], lo[G], nf[G], p[G], J[G];
         double xj[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           const bool act = mask != 0;
-          tt[g] = act ? t0 + __builtin_ctzll(mask) : t0;
+          const uint32_t tt = act ? __builtin_ctzll(mask) : 0;
           mask &= mask - 1;
-          key[g] = xor_state<NW>(s, off[tt[g]].X);
-          st[g] = 0;
+          tg[g] = tt;
           lo[g] = 0;
-          m[g] = 0;
-          xj[g] = 0.0;
+          nf[g] = 0;
+          p[g] = 0;
           J[g] = 0;
+          xj[g] = 0.0;
           if (act) {
+            const State<NW> key = xor_state<NW>(s, off[t0 + tt].X);
             uint32_t n;
-            bucket_bounds(b.table, bucket_key<NW>(b, key[g]), lo[g], n);
-            m[g] = 2 * n;
-            st[g] = n ? 1 : 0;
+            bucket_bounds(b.table, bucket_key<NW>(b, key), lo[g], n);
+            const uint64_t h = slot_hash<NW>(key);
+            nf[g] = (n << 16) | slot_fp(h);
+            p[g] = slot_home(h, 2 * n);
+            tg[g] |= (n ? 1u : 0u) << 8;
           }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const uint64_t h = slot_hash<NW>(key[g]);
-          p[g] = slot_home(h, m[g]);
-          fp[g] = slot_fp(h);
         }
         bool more = true;
         while (more) {
@@ -328,18 +329,19 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
           uint32_t e[G];
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            e[g] = st[g] == 1 ? b.slots[2 * static_cast<uint64_t>(lo[g]) + p[g]] : kSlotEmpty;
+            e[g] = (tg[g] >> 8) == 1 ? b.slots[2 * static_cast<uint64_t>(lo[g]) + p[g]]
+                                     : kSlotEmpty;
           }
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            if (st[g] == 1) {
+            if ((tg[g] >> 8) == 1) {
               if (e[g] == kSlotEmpty) {
-                st[g] = 0;
-              } else if ((e[g] >> 16) == fp[g]) {
+                tg[g] &= 0xFFu;
+              } else if ((e[g] >> 16) == (nf[g] & 0xFFFFu)) {
                 J[g] = lo[g] + (e[g] & 0xFFFFu);
-                st[g] = 2;
+                tg[g] = (tg[g] & 0xFFu) | (2u << 8);
               } else {
-                p[g] = p[g] + 1 == m[g] ? 0 : p[g] + 1;
+                p[g] = p[g] + 1 == 2 * (nf[g] >> 16) ? 0 : p[g] + 1;
                 more = true;
               }
             }
@@ -347,19 +349,20 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
           State<NW> f[G];
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            if (st[g] == 2) {
+            if ((tg[g] >> 8) == 2) {
               f[g] = load_state_cached<NW>(b.states, J[g]);
               xj[g] = x[J[g]];
             }
           }
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            if (st[g] == 2) {
-              if (st_eq<NW>(f[g], key[g])) {
-                st[g] = 3;
+            if ((tg[g] >> 8) == 2) {
+              const State<NW> key = xor_state<NW>(s, off[t0 + (tg[g] & 0xFFu)].X);
+              if (st_eq<NW>(f[g], key)) {
+                tg[g] = (tg[g] & 0xFFu) | (3u << 8);
               } else {
-                st[g] = 1;
-                p[g] = p[g] + 1 == m[g] ? 0 : p[g] + 1;
+                tg[g] = (tg[g] & 0xFFu) | (1u << 8);
+                p[g] = p[g] + 1 == 2 * (nf[g] >> 16) ? 0 : p[g] + 1;
                 more = true;
               }
             }
@@ -367,8 +370,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          if (st[g] == 3 && fabs(xj[g]) > kApplyTol) {
-            const OffTerm& T = off[tt[g]];
+          if ((tg[g] >> 8) == 3 && fabs(xj[g]) > kApplyTol) {
+            const OffTerm& T = off[t0 + (tg[g] & 0xFFu)];
             double c = T.c * T.sgn;
             if constexpr (HAS_Z) {
               if (parity_notz<NW>(s, T.Z)) c = -c;
@@ -949,6 +952,92 @@ __global__ __launch_bounds__(kVecBlock) void k_gemv_sub(
   }
 }
 
+// Second Gram-Schmidt pass fused with the check for a third: w -= V[:, 0:m] h
+// (the same sum order as k_gemv_sub), and from the same read of V the
+// per-workgroup partials of V[:, 0:m]^T w_new (columns 0..m-1) and of
+// w_new . w_new (column m), m <= MAXC.
+template <int MAXC>
+__global__ __launch_bounds__(kVecBlock) void k_gemv_sub_dots(
+    const double* __restrict__ V, int64_t ldv, int m, const double* __restrict__ h,
+    double* __restrict__ w, int64_t n, double* __restrict__ partials) {
+  __shared__ double hs[MAXC];
+  __shared__ double red[kVecBlock / kWave][MAXC + 1];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double acc[MAXC + 1];
+#pragma unroll
+  for (int j = 0; j <= MAXC; ++j) acc[j] = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    double vr[MAXC];
+    double sum = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) {
+      vr[j] = j < m ? V[j * ldv + i] : 0.0;
+      if (j < m) sum += vr[j] * hs[j];
+    }
+    const double wn = w[i] - sum;
+    w[i] = wn;
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) acc[j] += vr[j] * wn;
+    acc[MAXC] += wn * wn;
+  }
+#pragma unroll
+  for (int j = 0; j <= MAXC; ++j) {
+    const double v = wave_sum(acc[j]);
+    if (lane == 0) red[wid][j] = v;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j <= m; j += kVecBlock) {
+    const int src = j == m ? MAXC : j;
+    double r = 0.0;
+    for (int k = 0; k < kVecBlock / kWave; ++k) r += red[k][src];
+    partials[static_cast<int64_t>(blockIdx.x) * (m + 1) + j] = r;
+  }
+}
+
+// After the fused pass: g[0..m) = V^T w, g[m] = w . w.  A third pass is
+// needed when |V^T w| > 1e-12 |w| (flag[0] = 1); its correction is added to
+// the projected column hcol[0..m) and the norm follows it.  beta[0] = |w|,
+// beta[1] = 1 / |w| (0 for a vanishing w).
+__global__ void k_gs_tail(const double* __restrict__ g, int m, double* __restrict__ hcol,
+                          int* __restrict__ flag, double* __restrict__ beta) {
+  __shared__ double g2s;
+  __shared__ int need;
+  if (threadIdx.x == 0) {
+    double g2 = 0.0;
+    for (int j = 0; j < m; ++j) g2 += g[j] * g[j];
+    const double q = g[m];
+    need = (q > 0.0 && g2 > 1e-24 * q) ? 1 : 0;
+    g2s = g2;
+    const double b2 = need ? q - g2 : q;
+    const double b = b2 > 0.0 ? sqrt(b2) : 0.0;
+    beta[0] = b;
+    beta[1] = b > 0.0 ? 1.0 / b : 0.0;
+    flag[0] = need;
+  }
+  __syncthreads();
+  if (need)
+    for (int j = threadIdx.x; j < m; j += blockDim.x) hcol[j] += g[j];
+}
+
+// w -= V[:, 0:m] h when flag[0] is set (third Gram-Schmidt pass)
+__global__ __launch_bounds__(kVecBlock) void k_gemv_sub_if(
+    const double* __restrict__ V, int64_t ldv, int m, const double* __restrict__ h,
+    double* __restrict__ w, int64_t n, const int* __restrict__ flag) {
+  if (flag[0] == 0) return;
+  __shared__ double hs[kMaxCols];
+  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += V[j * ldv + i] * hs[j];
+    w[i] -= acc;
+  }
+}
+
 // out[:, c] = V[:, 0:m] Y[:, c] for c < k (Y column-major m x k, on device);
 // output columns in chunks of 8 so V is read ceil(k/8) times.
 __global__ __launch_bounds__(kVecBlock) void k_combine(
@@ -994,6 +1083,18 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev(
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
        i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
     y[i] += a * x[i];
+}
+
+// Psi[:, b] += c[b] * v for b < nb (Psi column-major, leading dimension ld):
+// one read of v feeds every column.
+__global__ __launch_bounds__(kVecBlock) void k_accum_cols(
+    const double* __restrict__ v, double* __restrict__ Psi, int64_t ld, int nb,
+    const double* __restrict__ c, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    const double vi = v[i];
+    for (int b = 0; b < nb; ++b) Psi[b * ld + i] += c[b] * vi;
+  }
 }
 
 // ============================================================================
@@ -1047,10 +1148,10 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
     variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED
             : op->basis->d_slots ? QSI_APPLY_SLOT4 : QSI_APPLY_LANE4;
   // a slot variant chosen while the basis had its slot index, which has been
-  // dropped since, falls back to the bisection of the same depth
-  if (!op->basis->d_slots && variant >= QSI_APPLY_SLOT2 && variant <= QSI_APPLY_SLOT8)
-    variant = variant == QSI_APPLY_SLOT2 ? QSI_APPLY_LANE2
-            : variant == QSI_APPLY_SLOT8 ? QSI_APPLY_LANE8 : QSI_APPLY_LANE4;
+  // dropped since, falls back to a bisection kernel (LANE4 for SLOT4, LANE2
+  // for SLOT2 and SLOT3)
+  if (!op->basis->d_slots && variant >= QSI_APPLY_SLOT2 && variant <= QSI_APPLY_SLOT3)
+    variant = variant == QSI_APPLY_SLOT4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE2;
   switch (variant) {
     case QSI_APPLY_CACHED:
       // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)
@@ -1101,8 +1202,8 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
                          y_in, yscale, yscale_dev, partials);
       break;
-    case QSI_APPLY_SLOT8:
-      hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 8>), grid, block, 0, s, v,
+    case QSI_APPLY_SLOT3:
+      hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 3>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
                          y_in, yscale, yscale_dev, partials);
       break;
@@ -1255,6 +1356,35 @@ int launch_gemv_sub(const double* V, int64_t ldv, int m, const double* h,
   return QSI_OK;
 }
 
+int launch_gs_reorth(const double* V, int64_t ldv, int m, const double* h, double* w,
+                     int64_t n, double* partials, double* g, double* hcol, int* flag,
+                     double* beta, hipStream_t s) {
+  if (m <= 0 || m > kMaxCols) {
+    set_error("launch_gs_reorth: column count out of range");
+    return QSI_E_INVALID;
+  }
+  const int nb = std::max(1, std::min(vec_blocks(n), multidot_partials() / (m + 1)));
+  if (m <= 16)
+    hipLaunchKernelGGL(k_gemv_sub_dots<16>, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, h, w, n,
+                       partials);
+  else if (m <= 32)
+    hipLaunchKernelGGL(k_gemv_sub_dots<32>, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, h, w, n,
+                       partials);
+  else
+    hipLaunchKernelGGL(k_gemv_sub_dots<64>, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, h, w, n,
+                       partials);
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize_cols, dim3(m + 1), dim3(256), 0, s, partials, nb, m + 1, g,
+                     static_cast<double*>(nullptr));
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_gs_tail, dim3(1), dim3(64), 0, s, g, m, hcol, flag, beta);
+  QSI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_gemv_sub_if, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, V, ldv, m, g, w,
+                     n, flag);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
 int launch_combine(const double* V, int64_t ldv, int m, const double* Y, int k,
                    double* out, int64_t ldo, int64_t n, hipStream_t s) {
   if (m > kMaxCols || k > kMaxCols) {
@@ -1283,6 +1413,14 @@ int launch_axpy_dev(const double* s_dev, double c, const double* x, double* y,
   return QSI_OK;
 }
 
+int launch_accum_cols(const double* v, double* Psi, int64_t ld, int nb, const double* c,
+                      int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_accum_cols, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, v, Psi, ld, nb,
+                     c, n);
+  QSI_CHECK_LAUNCH();
+  return QSI_OK;
+}
+
 }  // namespace qsi
 
 // ============================================================================
@@ -1303,15 +1441,14 @@ int qsi_device_count(void) {
   return n;
 }
 
-// K = ceil(log2 D) + 4, at most 28 (a 1 GiB table).  Ice-rule bases fill the
-// top-bit buckets very unevenly (the top spins are strongly constrained), so
-// the table is kept wide: on the 64-spin Q=0 basis K=26 leaves a weighted
-// mean bucket of ~23 states (5 bisection steps).
+// K = ceil(log2 D) + 7, at most 30 (a 4 GiB table) and at most n_spins.
+// Ice-rule bases fill buckets very unevenly, so the table is kept wide: on
+// the 64-spin Q=0 basis the 29-bit key leaves a lookup-weighted mean bucket
+// of ~3.4 states (the apply 5 % faster than with 27 bits); on the 96-spin
+// basis the 30-bit cap still leaves ~230 (6 % faster than 28 bits).
 static int auto_bucket_bits(int64_t dim, int n_spins) {
   int lg = 0;
   while ((int64_t{1} << lg) < dim) ++lg;
-  // 64 spins: 29 informative bits (a 2 GB table), the apply 5 % faster than
-  // with 27; 96 spins: capped at 30 (a 4 GB table, 6 % faster than 28 bits)
   int k = lg + 7;
   k = std::min(k, 30);
   k = std::min(k, n_spins);
@@ -1836,7 +1973,7 @@ int64_t qsi_op_dim(qsi_op_t op) { return op ? op->basis->dim : -1; }
 
 int qsi_op_set_variant(qsi_op_t op, int variant) {
   QSI_REQUIRE(op != nullptr, "qsi_op_set_variant: null operator");
-  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_SLOT8,
+  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_SLOT3,
               "qsi_op_set_variant: unknown variant");
   QSI_REQUIRE(op->basis->d_slots || variant < QSI_APPLY_SLOT2,
               "qsi_op_set_variant: the basis has no slot index (qsi_basis_build_slots)");

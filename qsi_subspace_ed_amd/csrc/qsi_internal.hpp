@@ -161,6 +161,15 @@ int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
 // w -= V[:, 0:m] h
 int launch_gemv_sub(const double* V, int64_t ldv, int m, const double* h,
                     double* w, int64_t n, hipStream_t s);
+// Second and (when needed) third classical Gram-Schmidt passes of w against
+// V[:, 0:m] after a first pass gave h = V^T w: w -= V h with g[0..m) =
+// V^T w_new and g[m] = |w_new|^2 from the same read (g: m + 1 doubles); if
+// |g[0..m)| > 1e-12 |w_new| then w -= V g, g is added to hcol[0..m) and
+// flag[0] = 1.  beta[0] = |w| at the end, beta[1] = 1 / beta[0] (0 if w
+// vanishes).  All on the device (no host synchronisation).
+int launch_gs_reorth(const double* V, int64_t ldv, int m, const double* h, double* w,
+                     int64_t n, double* partials, double* g, double* hcol, int* flag,
+                     double* beta, hipStream_t s);
 // out[:, 0:k] = V[:, 0:m] Y  (Y m x k column-major, device)
 int launch_combine(const double* V, int64_t ldv, int m, const double* Y, int k,
                    double* out, int64_t ldo, int64_t n, hipStream_t s);
@@ -170,5 +179,18 @@ int launch_copy_scaled(const double* s_dev, const double* y, double* x,
 // y += c * s_dev[0] * x
 int launch_axpy_dev(const double* s_dev, double c, const double* x, double* y,
                     int64_t n, hipStream_t s);
+
+// Psi[:, b] += c[b] * v for b < nb (c on the device; Psi column-major)
+int launch_accum_cols(const double* v, double* Psi, int64_t ld, int nb, const double* c,
+                      int64_t n, hipStream_t s);
+
+// The 3-term Lanczos coefficient run behind qsi_lanczos_coeffs on
+// caller-provided device buffers: v (D, the start vector, normalised here;
+// clobbered), u (D), partials (lanczos_coeffs_partials(op) doubles), ab
+// (3 M doubles), scal (2 doubles).  Allocates nothing; synchronises.
+int64_t lanczos_coeffs_partials(const qsi_op_s* op);
+int lanczos_coeffs_run(const qsi_op_s* op, double* v, double* u, double* partials, double* ab,
+                       double* scal, int M, double* h_alpha, double* h_beta, int* n_steps,
+                       hipStream_t s);
 
 }  // namespace qsi

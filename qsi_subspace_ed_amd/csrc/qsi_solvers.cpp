@@ -92,6 +92,50 @@ void check_convergence(const std::vector<double>& a, const std::vector<double>& 
 }
 
 }  // namespace
+
+int64_t lanczos_coeffs_partials(const qsi_op_s* op) {
+  return std::max<int64_t>(max_vec_partials(), apply_num_blocks(op));
+}
+
+int lanczos_coeffs_run(const qsi_op_s* op, double* v, double* u, double* partials, double* ab,
+                       double* scal, int M, double* h_alpha, double* h_beta, int* n_steps,
+                       hipStream_t s) {
+  const int64_t D = op->basis->dim;
+  double n2 = 0.0;
+  QSI_TRY(launch_dot(v, v, D, partials, scal, s));
+  QSI_CHECK_HIP(hipMemcpy(&n2, scal, sizeof(double), hipMemcpyDeviceToHost));
+  QSI_REQUIRE(n2 > 0.0, "qsi_lanczos_coeffs: zero start vector");
+  QSI_TRY(launch_scal(1.0 / std::sqrt(n2), v, D, s));
+  std::fill(h_alpha, h_alpha + M, 0.0);
+  std::fill(h_beta, h_beta + M, 0.0);
+  constexpr int kChunk = 64;  // coefficients copied back every kChunk steps
+  double hab[3 * kChunk];
+  int done = 0;
+  for (int it0 = 0; it0 < M; it0 += kChunk) {
+    const int it1 = std::min(M, it0 + kChunk);
+    for (int it = it0; it < it1; ++it) {
+      QSI_TRY(lanczos_step(op, v, u, it == 0, ab + 3 * (it - 1) + 1, ab + 3 * it, partials, s));
+      std::swap(v, u);
+    }
+    QSI_CHECK_HIP(hipMemcpy(hab, ab + 3 * it0, 3 * (it1 - it0) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    bool stop = false;
+    for (int it = it0; it < it1; ++it) {
+      h_alpha[it] = hab[3 * (it - it0)];
+      h_beta[it] = hab[3 * (it - it0) + 1];
+      done = it + 1;
+      if (!(h_beta[it] > 1e-300)) {  // invariant subspace: the recurrence ends
+        h_beta[it] = 0.0;
+        stop = true;
+        break;
+      }
+    }
+    if (stop) break;
+  }
+  if (n_steps) *n_steps = done;
+  return QSI_OK;
+}
+
 }  // namespace qsi
 
 using namespace qsi;
@@ -247,52 +291,106 @@ int qsi_lanczos_coeffs(qsi_op_t op, const double* d_v0, uint64_t seed, int M,
   DevBuf bv, bu, bpart, bab, bscal;
   QSI_CHECK_HIP(bv.alloc(bytes));
   QSI_CHECK_HIP(bu.alloc(bytes));
-  const int64_t n_part = std::max<int64_t>(max_vec_partials(), apply_num_blocks(op));
-  QSI_CHECK_HIP(bpart.alloc(static_cast<size_t>(n_part) * sizeof(double)));
+  QSI_CHECK_HIP(bpart.alloc(static_cast<size_t>(lanczos_coeffs_partials(op)) * sizeof(double)));
   QSI_CHECK_HIP(bab.alloc(static_cast<size_t>(3 * M) * sizeof(double)));
   QSI_CHECK_HIP(bscal.alloc(2 * sizeof(double)));
   hipStream_t s = nullptr;
+  if (d_v0) {
+    QSI_CHECK_HIP(hipMemcpyAsync(bv.d(), d_v0, bytes, hipMemcpyDeviceToDevice, s));
+  } else {
+    QSI_TRY(launch_randn(bv.d(), D, seed, s));
+  }
+  return lanczos_coeffs_run(op, bv.d(), bu.d(), bpart.d(), bab.d(), bscal.d(), M, h_alpha,
+                            h_beta, n_steps, s);
+}
+
+// One FTLM sample with operator expectations.  Pass 1 runs the 3-term
+// recurrence from r; T_n = Y diag(e) Y^T gives, for each beta, the Lanczos
+// coefficients of psi_beta = exp(-beta (H - e_shift) / 2) r,
+//   c_beta = Y (exp(-beta (e - e_shift) / 2) * Y[0, :])^T;
+// pass 2 runs the same recurrence again (same kernels, so the same vectors)
+// and adds c_beta[j] v_j into Psi[:, beta]; then <psi|psi> and <psi|O|psi>
+// per beta and operator come from device dots and one apply of O per
+// column.
+int qsi_ftlm_sample(qsi_op_t H, const qsi_op_t* ops, int n_ops, const double* d_r, int M,
+                    const double* h_betas, int n_beta, double e_shift, double* e_used,
+                    double* h_norm, double* h_expect, double* h_alpha, double* h_beta,
+                    int* n_steps) {
+  QSI_REQUIRE(H && d_r && h_betas && h_norm && h_alpha && h_beta,
+              "qsi_ftlm_sample: null argument");
+  QSI_REQUIRE(n_ops >= 0 && (n_ops == 0 || (ops && h_expect)),
+              "qsi_ftlm_sample: operators need ops and h_expect");
+  QSI_REQUIRE(M >= 1 && n_beta >= 1, "qsi_ftlm_sample: need M >= 1 and n_beta >= 1");
+  const int64_t D = H->basis->dim;
+  QSI_REQUIRE(D > 0, "qsi_ftlm_sample: empty basis");
+  for (int l = 0; l < n_ops; ++l)
+    QSI_REQUIRE(ops[l] && ops[l]->basis->dim == D,
+                "qsi_ftlm_sample: every operator must act on the Hamiltonian's basis size");
+  const size_t bytes = static_cast<size_t>(D) * sizeof(double);
+  DevBuf bv, bu, bt, bpsi, bpart, bab, bscal, bc;
+  QSI_CHECK_HIP(bv.alloc(bytes));
+  QSI_CHECK_HIP(bu.alloc(bytes));
+  QSI_CHECK_HIP(bpart.alloc(static_cast<size_t>(lanczos_coeffs_partials(H)) * sizeof(double)));
+  QSI_CHECK_HIP(bab.alloc(static_cast<size_t>(3 * M) * sizeof(double)));
+  QSI_CHECK_HIP(bscal.alloc(2 * sizeof(double)));
+  hipStream_t s = nullptr;
+  // pass 1: the coefficients
+  QSI_CHECK_HIP(hipMemcpyAsync(bv.d(), d_r, bytes, hipMemcpyDeviceToDevice, s));
+  int n = 0;
+  QSI_TRY(lanczos_coeffs_run(H, bv.d(), bu.d(), bpart.d(), bab.d(), bscal.d(), M, h_alpha,
+                             h_beta, &n, s));
+  if (n_steps) *n_steps = n;
+  // the Lanczos coefficients of psi_beta
+  std::vector<double> d(h_alpha, h_alpha + n), e(h_beta, h_beta + std::max(n - 1, 0)), ev, Y;
+  if (!hostla::tridiag_eig(d, e, ev, &Y)) {
+    set_error("qsi_ftlm_sample: no convergence of the tridiagonal eigensolver");
+    return QSI_E_NOCONV;
+  }
+  if (!std::isfinite(e_shift)) e_shift = ev[0];
+  if (e_used) *e_used = e_shift;
+  std::vector<double> C(static_cast<size_t>(n) * n_beta, 0.0);  // C[j * n_beta + b]
+  for (int b = 0; b < n_beta; ++b)
+    for (int k = 0; k < n; ++k) {
+      const double f = std::exp(-0.5 * h_betas[b] * (ev[k] - e_shift)) *
+                       Y[static_cast<size_t>(k) * n];
+      for (int j = 0; j < n; ++j)
+        C[static_cast<size_t>(j) * n_beta + b] += Y[static_cast<size_t>(k) * n + j] * f;
+    }
+  // pass 2: Psi = V C, one column per beta
+  const size_t ld = static_cast<size_t>(D);
+  QSI_CHECK_HIP(bpsi.alloc(ld * n_beta * sizeof(double)));
+  QSI_CHECK_HIP(bc.alloc(C.size() * sizeof(double)));
+  QSI_CHECK_HIP(hipMemcpy(bc.d(), C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
+  QSI_CHECK_HIP(hipMemsetAsync(bpsi.d(), 0, ld * n_beta * sizeof(double), s));
+  QSI_CHECK_HIP(hipMemcpyAsync(bv.d(), d_r, bytes, hipMemcpyDeviceToDevice, s));
+  double n2 = 0.0;
+  QSI_TRY(launch_dot(bv.d(), bv.d(), D, bpart.d(), bscal.d(), s));
+  QSI_CHECK_HIP(hipMemcpy(&n2, bscal.d(), sizeof(double), hipMemcpyDeviceToHost));
+  QSI_TRY(launch_scal(1.0 / std::sqrt(n2), bv.d(), D, s));
   double* v = bv.d();
   double* u = bu.d();
-  if (d_v0) {
-    QSI_CHECK_HIP(hipMemcpyAsync(v, d_v0, bytes, hipMemcpyDeviceToDevice, s));
-  } else {
-    QSI_TRY(launch_randn(v, D, seed, s));
-  }
-  double n2 = 0.0;
-  QSI_TRY(launch_dot(v, v, D, bpart.d(), bscal.d(), s));
-  QSI_CHECK_HIP(hipMemcpy(&n2, bscal.d(), sizeof(double), hipMemcpyDeviceToHost));
-  QSI_REQUIRE(n2 > 0.0, "qsi_lanczos_coeffs: zero start vector");
-  QSI_TRY(launch_scal(1.0 / std::sqrt(n2), v, D, s));
-  std::fill(h_alpha, h_alpha + M, 0.0);
-  std::fill(h_beta, h_beta + M, 0.0);
-  const int chunk = 64;
-  int done = 0;
-  std::vector<double> hab;
-  for (int it0 = 0; it0 < M; it0 += chunk) {
-    const int it1 = std::min(M, it0 + chunk);
-    for (int it = it0; it < it1; ++it) {
-      QSI_TRY(lanczos_step(op, v, u, it == 0, bab.d() + 3 * (it - 1) + 1, bab.d() + 3 * it,
+  for (int it = 0; it < n; ++it) {
+    QSI_TRY(launch_accum_cols(v, bpsi.d(), static_cast<int64_t>(ld), n_beta,
+                              bc.d() + static_cast<size_t>(it) * n_beta, D, s));
+    if (it + 1 < n) {
+      QSI_TRY(lanczos_step(H, v, u, it == 0, bab.d() + 3 * (it - 1) + 1, bab.d() + 3 * it,
                            bpart.d(), s));
       std::swap(v, u);
     }
-    hab.resize(static_cast<size_t>(3 * (it1 - it0)));
-    QSI_CHECK_HIP(hipMemcpy(hab.data(), bab.d() + 3 * it0, hab.size() * sizeof(double),
-                            hipMemcpyDeviceToHost));
-    bool stop = false;
-    for (int it = it0; it < it1; ++it) {
-      h_alpha[it] = hab[3 * (it - it0)];
-      h_beta[it] = hab[3 * (it - it0) + 1];
-      done = it + 1;
-      if (!(h_beta[it] > 1e-300)) {  // invariant subspace: the recurrence ends
-        h_beta[it] = 0.0;
-        stop = true;
-        break;
-      }
-    }
-    if (stop) break;
   }
-  if (n_steps) *n_steps = done;
+  // <psi|psi> and <psi|O|psi>
+  QSI_CHECK_HIP(bt.alloc(bytes));
+  for (int b = 0; b < n_beta; ++b) {
+    const double* psi = bpsi.d() + b * ld;
+    QSI_TRY(launch_dot(psi, psi, D, bpart.d(), bscal.d(), s));
+    QSI_CHECK_HIP(hipMemcpy(h_norm + b, bscal.d(), sizeof(double), hipMemcpyDeviceToHost));
+    for (int l = 0; l < n_ops; ++l) {
+      QSI_TRY(launch_apply(ops[l], psi, bt.d(), nullptr, 0.0, nullptr, nullptr, s));
+      QSI_TRY(launch_dot(psi, bt.d(), D, bpart.d(), bscal.d(), s));
+      QSI_CHECK_HIP(hipMemcpy(h_expect + static_cast<size_t>(l) * n_beta + b, bscal.d(),
+                              sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
   return QSI_OK;
 }
 
@@ -387,20 +485,24 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
   n_evecs = std::min(n_evecs, nev);
   const int64_t ld = (D + 63) / 64 * 64;
   hipStream_t s = nullptr;
-  DevBuf bV, bW, bw, bpart, bh, bscal;
+  DevBuf bV, bW, bpart, bscal;
   QSI_CHECK_HIP(bV.alloc(static_cast<size_t>(ld) * (ncv + 1) * sizeof(double)));
   QSI_CHECK_HIP(bW.alloc(static_cast<size_t>(ld) * (ncv + 1) * sizeof(double)));
-  QSI_CHECK_HIP(bw.alloc(static_cast<size_t>(D) * sizeof(double)));
   QSI_CHECK_HIP(bpart.alloc(std::max(max_vec_partials(), multidot_partials()) * sizeof(double)));
-  QSI_CHECK_HIP(bh.alloc(3 * 64 * sizeof(double)));
   QSI_CHECK_HIP(bscal.alloc(4 * sizeof(double)));
   DevBuf bY;  // Ritz coefficients of a restart (at most ncv x ncv)
   QSI_CHECK_HIP(bY.alloc(static_cast<size_t>(ncv) * ncv * sizeof(double)));
+  // projected columns (ncv + 1 per step: h and, from pass 1, |w|^2), per-step
+  // beta and 1/beta, the fused pass's sums and the third-pass flag, all
+  // device-resident so a restart cycle runs without host synchronisation
+  const int colw = ncv + 1;
+  DevBuf bT, bbeta, bg, bflag;
+  QSI_CHECK_HIP(bT.alloc(static_cast<size_t>(ncv) * colw * sizeof(double)));
+  QSI_CHECK_HIP(bbeta.alloc(static_cast<size_t>(2 * ncv) * sizeof(double)));
+  QSI_CHECK_HIP(bg.alloc(80 * sizeof(double)));
+  QSI_CHECK_HIP(bflag.alloc(sizeof(int)));
   double* V = bV.d();
   double* W = bW.d();
-  double* w = bw.d();
-  double* h = bh.d();
-  double* h2 = bh.d() + 64;
   int applies = 0;
 
   // start vector
@@ -426,24 +528,32 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
   int m = ncv;     // current basis size
   double beta_last = 0.0;
   bool exhausted = false;
-  std::vector<double> hh(64);
+  std::vector<double> hT(static_cast<size_t>(ncv) * colw), hbeta(2 * ncv);
   for (int restart = 0;; ++restart) {
     m = ncv;
+    // the cycle: steps k..ncv-1 write v_{j+1} straight into V[:, j+1]
     for (int j = k; j < ncv; ++j) {
-      double* vj = V + static_cast<int64_t>(j) * ld;
+      const double* vj = V + static_cast<int64_t>(j) * ld;
+      double* w = V + static_cast<int64_t>(j + 1) * ld;
+      double* hcol = bT.d() + static_cast<size_t>(j) * colw;
       QSI_TRY(launch_apply(op, vj, w, nullptr, 0.0, nullptr, nullptr, s));
       ++applies;
-      // two classical Gram-Schmidt passes against v_0..v_j
-      QSI_TRY(launch_multidot(V, ld, j + 1, w, D, bpart.d(), h, nullptr, s));
-      QSI_TRY(launch_gemv_sub(V, ld, j + 1, h, w, D, s));
-      QSI_TRY(launch_multidot(V, ld, j + 1, w, D, bpart.d(), h2, h, s));
-      QSI_TRY(launch_gemv_sub(V, ld, j + 1, h2, w, D, s));
-      QSI_TRY(launch_dot(w, w, D, bpart.d(), bscal.d(), s));
-      QSI_CHECK_HIP(hipMemcpyAsync(hh.data(), h, (j + 1) * sizeof(double),
-                                   hipMemcpyDeviceToHost, s));
-      double bb = 0.0;
-      QSI_CHECK_HIP(hipMemcpyAsync(&bb, bscal.d(), sizeof(double), hipMemcpyDeviceToHost, s));
-      QSI_CHECK_HIP(hipStreamSynchronize(s));
+      // pass 1: h = V[:, 0..j]^T w (and w . w: column j + 1 is w itself)
+      QSI_TRY(launch_multidot(V, ld, j + 2, w, D, bpart.d(), hcol, nullptr, s));
+      // pass 2 fused with the norm and the test for pass 3
+      QSI_TRY(launch_gs_reorth(V, ld, j + 1, hcol, w, D, bpart.d(), bg.d(), hcol,
+                               static_cast<int*>(bflag.p), bbeta.d() + 2 * j, s));
+      QSI_TRY(launch_scal_dev(bbeta.d() + 2 * j + 1, w, D, s));
+    }
+    QSI_CHECK_HIP(hipMemcpyAsync(hT.data() + static_cast<size_t>(k) * colw,
+                                 bT.d() + static_cast<size_t>(k) * colw,
+                                 static_cast<size_t>(ncv - k) * colw * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    QSI_CHECK_HIP(hipMemcpyAsync(hbeta.data(), bbeta.d(), 2 * ncv * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    QSI_CHECK_HIP(hipStreamSynchronize(s));
+    for (int j = k; j < ncv; ++j) {
+      const double* hh = hT.data() + static_cast<size_t>(j) * colw;
       for (int i = 0; i <= j; ++i) {
         if (i < k && j == k) {
           Tat(i, j) = Tat(j, i) = hh[i];  // coupling to the kept Ritz vectors
@@ -451,21 +561,18 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
           Tat(i, j) = Tat(j, i) = hh[i];
         }
       }
-      const double beta = std::sqrt(std::max(bb, 0.0));
+      const double beta = hbeta[2 * j];
       double tnorm = 0.0;
       for (int i = 0; i <= j; ++i) tnorm = std::max(tnorm, std::fabs(Tat(i, j)));
       if (beta <= 1e-13 * std::max(tnorm, 1e-300)) {
-        // invariant subspace: the projected matrix is exact
+        // invariant subspace: the projected matrix is exact (the later steps
+        // of the cycle worked on a vanishing vector and are dropped)
         m = j + 1;
         beta_last = 0.0;
         exhausted = true;
         break;
       }
       beta_last = beta;
-      double* vn = V + static_cast<int64_t>(j + 1) * ld;
-      QSI_CHECK_HIP(hipMemcpyAsync(vn, w, static_cast<size_t>(D) * sizeof(double),
-                                   hipMemcpyDeviceToDevice, s));
-      QSI_TRY(launch_scal(1.0 / beta, vn, D, s));
     }
     // Rayleigh-Ritz on the m x m projected matrix
     std::vector<double> Tm(static_cast<size_t>(m) * m);

@@ -114,3 +114,33 @@ def thermodynamics(nodes, D, temps, e_ref=None, normalise=True):
     E = E1 / Z
     Cv = (E2 / Z - E * E) / temps ** 2
     return {"T": temps, "Z_shifted": Z, "E_ref": e_ref, "E": E, "C": Cv}
+
+
+def sample_operators(H, ops, r, M, betas, e_shift=float("nan")):
+    """One sample of the operator-expectation FTLM (qsi_ftlm_sample): from the
+    CUDA float64 start vector r (normalised internally), M Lanczos steps and
+    psi_b = exp(-beta_b (H - e_shift) / 2) r for every beta_b, built from the
+    Lanczos vectors of a second identical pass.  Returns a dict with "norm"
+    (<psi_b|psi_b>), "expect" ((n_ops, n_beta) <psi_b|O|psi_b>), "e_shift"
+    (the shift used: the sample's lowest Ritz value when e_shift is NaN),
+    "alpha" and "beta" (the coefficients)."""
+    betas = np.ascontiguousarray(betas, dtype=np.float64)
+    nb = betas.size
+    no = len(ops)
+    op_arr = (C.c_void_p * max(no, 1))(*[o._h.value for o in ops])
+    norm = np.zeros(nb)
+    expect = np.zeros(max(no, 1) * nb)
+    al = np.zeros(int(M))
+    be = np.zeros(int(M))
+    used = C.c_double(0.0)
+    n = C.c_int(0)
+    P = C.POINTER(C.c_double)
+    import torch
+    torch.cuda.synchronize()
+    check(_lib.load().qsi_ftlm_sample(H._h, op_arr, no, r.data_ptr(), int(M), betas.ctypes.data_as(P),
+                                      nb, float(e_shift), C.byref(used), norm.ctypes.data_as(P),
+                                      expect.ctypes.data_as(P), al.ctypes.data_as(P),
+                                      be.ctypes.data_as(P), C.byref(n)), "qsi_ftlm_sample")
+    m = n.value
+    return {"norm": norm, "expect": expect[: no * nb].reshape(no, nb), "e_shift": used.value,
+            "alpha": al[:m], "beta": be[:m]}

@@ -136,7 +136,7 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
     x = torch.from_numpy(_rand_x(b.shape[0], 9)).cuda()
     ys = []
     for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
-              "slot8", "cached"):
+              "slot3", "cached"):
         if v == "cached":
             op.cache_connectivity(True)
         op.set_variant(v)
@@ -172,7 +172,7 @@ def test_apply_partial_basis(lattices, cuda):
     b = load_basis(32)[::3].copy()
     H = _ham(jd, 32, "ring+field")
     for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
-              "slot8"):
+              "slot3"):
         basis = q.GpuBasis(b, 32, hash_index=True)
         basis.build_slots()
         op = q.GpuOp(basis, H)
@@ -229,7 +229,7 @@ def test_apply_parity_two_words(lattices, cuda):
     x = _rand_x(len(states), 5)
     ref = orc.evaluate_add(orc.Basis(lohi), H, x, np.zeros(len(states)))
     for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
-              "slot8"):
+              "slot3"):
         op.set_variant(v)
         yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
         op.evaluate(torch.from_numpy(x).cuda(), yd)

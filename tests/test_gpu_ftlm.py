@@ -208,3 +208,42 @@ def test_c3_two_sectors_against_oracle(lattices, cuda):
             got = [np.sum(w * e ** p) for p in range(4)]
             for p in range(4):
                 assert abs(got[p] - mom[p]) <= 1e-10 * max(1.0, abs(mom[p])), (s, c, p)
+
+
+def test_sample_operators_against_dense(lattices, cuda):
+    """qsi_ftlm_sample (the operator-expectation sample) against the dense
+    propagator: psi_b = exp(-beta_b (H - s) / 2) r from the dense spectrum of
+    the oracle's H; <psi|psi> and <psi|O|psi> for a ring term O, 1e-9
+    relative; the coefficients equal qsi_lanczos_coeffs' (same recurrence)."""
+    torch = cuda
+    jd = lattices[32]
+    b = load_basis(32)
+    D = b.shape[0]
+    H = _ham(jd, [0.9, 0.35, 0.6, 0.45], [0.05, -0.02, 0.04])
+    basis = q.GpuBasis(b, 32)
+    op = q.GpuOp(basis, H)
+    ring, _, _ = q.get_ring_ops(jd)
+    O = q.SymbolicOpSum()
+    O.add_term(1.0, ring[3])
+    oop = q.GpuOp(basis, O)
+    r = np.random.default_rng(4).standard_normal(D)
+    rd = torch.from_numpy(r).cuda()
+    betas = np.array([0.2, 0.7, 1.5, 3.0])
+    res = ftlm.sample_operators(op, [oop], rd, 90, betas)
+    a2, b2 = op.lanczos_coeffs(90, v0=rd)
+    assert np.array_equal(res["alpha"], a2) and np.array_equal(res["beta"], b2)
+    ob = orc.Basis(b)
+    rr, cc, vv = orc.sparse_triplets(ob, H)
+    Md = np.zeros((D, D))
+    np.add.at(Md, (rr, cc), vv)
+    w, U = np.linalg.eigh(Md)
+    s = res["e_shift"]
+    e, _ = ftlm.tridiag_eig(res["alpha"], res["beta"])
+    assert s == e.min()
+    rn = r / np.linalg.norm(r)
+    for j, bt in enumerate(betas):
+        psi = U @ (np.exp(-0.5 * bt * (w - s)) * (U.T @ rn))
+        n_ref = psi @ psi
+        o_ref = psi @ orc.evaluate_add(ob, O, psi, np.zeros(D))
+        assert abs(res["norm"][j] - n_ref) <= 1e-9 * n_ref
+        assert abs(res["expect"][0, j] - o_ref) <= 1e-9 * max(abs(o_ref), 1e-3 * n_ref)
