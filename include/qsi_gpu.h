@@ -296,6 +296,19 @@ int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t *settings,
 int qsi_lanczos_coeffs(qsi_op_t op, const double *d_v0, uint64_t seed, int M,
                        double *h_alpha, double *h_beta, int *n_steps);
 
+/* qsi_lanczos_coeffs without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference.  The caller brings the
+ * two Lanczos vectors: d_v (D doubles) holds the start vector on entry (it is
+ * normalised in place and the run clobbers it) and d_u (D doubles) is work
+ * space.  The small buffers of the run live in a static device array of
+ * 8.1 MiB, one per GPU, held until the process ends: it limits M to 4096 and
+ * the basis to 2^28 states (larger requests fail with QSI_E_INVALID).  Same
+ * kernels and order as qsi_lanczos_coeffs, so the same h_alpha / h_beta.
+ * Calls that overlap on one GPU share that array and race.  Synchronises. */
+int qsi_lanczos_coeffs_noalloc(qsi_op_t op, double *d_v, double *d_u, int M,
+                               double *h_alpha, double *h_beta, int *n_steps);
+
 /* Lowest-nev eigenpairs with a thick-restart Lanczos (full
  * reorthogonalisation, ncv-dimensional search space), the GPU counterpart of
  * compute_spectrum_iterative<LazyOpSumProd> -> Spectra::SymEigsSolver

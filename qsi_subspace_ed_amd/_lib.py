@@ -123,6 +123,8 @@ SIGNATURES = {
                                       C.POINTER(LanczosResult)]),
     "qsi_lanczos_coeffs": (C.c_int, [_P, _P, C.c_uint64, C.c_int, _DP, _DP,
                                      C.POINTER(C.c_int)]),
+    "qsi_lanczos_coeffs_noalloc": (C.c_int, [_P, _P, _P, C.c_int, _DP, _DP,
+                                             C.POINTER(C.c_int)]),
     "qsi_eigsh": (C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, _P, C.c_uint64, _DP,
                             _P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "qsi_lanczos_block": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P),

@@ -16,7 +16,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libqsi_gpu.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
-SOURCES = ["qsi_gpu.hip", "qsi_block.hip", "qsi_serve.hip", "qsi_enum.hip", "qsi_solvers.cpp"]
+SOURCES = ["qsi_gpu.hip", "qsi_block.hip", "qsi_serve.hip", "qsi_enum.hip", "qsi_solvers.cpp",
+           "qsi_noalloc.hip"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
