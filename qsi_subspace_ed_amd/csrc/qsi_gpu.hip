@@ -266,6 +266,42 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
   }
 }
 
+// Flip masks and signed coefficients of the first kLdsTerms off-diagonal
+// terms, staged in LDS by every workgroup: lookups re-form their connected
+// state and take their coefficient from here instead of gathering each
+// lane's term from global memory.
+constexpr int kLdsTerms = 256;
+template <int NW>
+struct TermsLds {
+  uint64_t X[NW][kLdsTerms];
+  double c[kLdsTerms];  // c * sgn
+};
+
+template <int NW>
+__device__ __forceinline__ bool stage_terms(TermsLds<NW>& tl, const OffTerm* __restrict__ off,
+                                            int n_off) {
+  const bool lds = n_off <= kLdsTerms;
+  if (lds) {
+    for (int t = threadIdx.x; t < n_off; t += blockDim.x) {
+#pragma unroll
+      for (int w = 0; w < NW; ++w) tl.X[w][t] = off[t].X[w];
+      tl.c[t] = off[t].c * off[t].sgn;
+    }
+  }
+  __syncthreads();
+  return lds;
+}
+
+template <int NW>
+__device__ __forceinline__ State<NW> term_target(const TermsLds<NW>& tl, bool lds,
+                                                 const OffTerm* __restrict__ off,
+                                                 const State<NW>& s, int t) {
+  State<NW> r;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) r.w[w] = s.w[w] ^ (lds ? tl.X[w][t] : off[t].X[w]);
+  return r;
+}
+
 // Slot variant: as the lane variant, with each connected state found through
 // the slot index under its bucket (bucket bounds, then slots from the
 // state's home until an empty slot or a fingerprint match, whose state and x
@@ -284,6 +320,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
     const double* __restrict__ yscale_dev, double* __restrict__ partials) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
+  __shared__ TermsLds<NW> tl;
+  const bool lds = stage_terms<NW>(tl, off, n_off);
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
@@ -314,7 +352,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
           J[g] = 0;
           xj[g] = 0.0;
           if (act) {
-            const State<NW> key = xor_state<NW>(s, off[t0 + tt].X);
+            const State<NW> key = term_target<NW>(tl, lds, off, s, t0 + tt);
             uint32_t n;
             bucket_bounds(b.table, bucket_key<NW>(b, key), lo[g], n);
             const uint64_t h = slot_hash<NW>(key);
@@ -357,7 +395,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             if ((tg[g] >> 8) == 2) {
-              const State<NW> key = xor_state<NW>(s, off[t0 + (tg[g] & 0xFFu)].X);
+              const State<NW> key = term_target<NW>(tl, lds, off, s, t0 + (tg[g] & 0xFFu));
               if (st_eq<NW>(f[g], key)) {
                 tg[g] = (tg[g] & 0xFFu) | (3u << 8);
               } else {
@@ -371,10 +409,10 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if ((tg[g] >> 8) == 3 && fabs(xj[g]) > kApplyTol) {
-            const OffTerm& T = off[t0 + (tg[g] & 0xFFu)];
-            double c = T.c * T.sgn;
+            const int t = t0 + (tg[g] & 0xFFu);
+            double c = lds ? tl.c[t] : off[t].c * off[t].sgn;
             if constexpr (HAS_Z) {
-              if (parity_notz<NW>(s, T.Z)) c = -c;
+              if (parity_notz<NW>(s, off[t].Z)) c = -c;
             }
             acc += c * xj[g];
           }
