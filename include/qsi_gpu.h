@@ -314,7 +314,8 @@ int qsi_lanczos_coeffs_noalloc(qsi_op_t op, double *d_v, double *d_u, int M,
  * compute_spectrum_iterative<LazyOpSumProd> -> Spectra::SymEigsSolver
  * (include/matrix_diag_bits.hpp:51-95; SortRule::SmallestAlge).  tol is the
  * Spectra-style relative residual tolerance.  d_v0 (D doubles, device) is the
- * start vector or NULL for a seeded Gaussian start.  h_evals receives nev
+ * start vector or NULL for qsi_vec_randn(seed) (Spectra's init() takes no
+ * seeded vector, matrix_diag_bits.hpp:75).  h_evals receives nev
  * ascending eigenvalues, d_evecs (D x n_evecs column-major, may be NULL) the
  * eigenvectors; *nconv the number of converged pairs.  Synchronises. */
 int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,

@@ -175,15 +175,19 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
+  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
+  const double xs = xs_dev ? xs_dev[0] : 1.0;
+  const double ys = ys_dev ? ys_dev[0] : 1.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
     const State<NW> s = load_state_cached<NW>(b.states, i);
-    const double xi = x[i];
-    double acc = y_in ? yscale * y_in[i] : 0.0;
+    const double xi = x[i] * xs;
+    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
     for (int t0 = 0; t0 < n_off; t0 += 64) {
       const int tn = min(64, n_off - t0);
       // which terms connect this row (uniform loop over terms)
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
           xj[g] = 0.0;
           if (n[g] == 1) {
             f[g] = load_state_cached<NW>(b.states, lo[g]);
-            xj[g] = x[lo[g]];
+            xj[g] = x[lo[g]] * xs;
           }
         }
 #pragma unroll
@@ -317,17 +321,21 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
+  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
+  const double xs = xs_dev ? xs_dev[0] : 1.0;
+  const double ys = ys_dev ? ys_dev[0] : 1.0;
   __shared__ TermsLds<NW> tl;
   const bool lds = stage_terms<NW>(tl, off, n_off);
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
     const State<NW> s = load_state_cached<NW>(b.states, i);
-    const double xi = x[i];
-    double acc = y_in ? yscale * y_in[i] : 0.0;
+    const double xi = x[i] * xs;
+    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
     for (int t0 = 0; t0 < n_off; t0 += 64) {
       const int tn = min(64, n_off - t0);
       uint64_t mask = 0;
@@ -389,7 +397,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
           for (int g = 0; g < G; ++g) {
             if ((tg[g] >> 8) == 2) {
               f[g] = load_state_cached<NW>(b.states, J[g]);
-              xj[g] = x[J[g]];
+              xj[g] = x[J[g]] * xs;
             }
           }
 #pragma unroll
@@ -442,17 +450,21 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
     const uint64_t* __restrict__ rowptr, const uint32_t* __restrict__ cj,
     const uint8_t* __restrict__ cm, const DiagTerm* __restrict__ dg, int n_diag,
     const double* __restrict__ x, double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
+  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
+  const double xs = xs_dev ? xs_dev[0] : 1.0;
+  const double ys = ys_dev ? ys_dev[0] : 1.0;
   double cc[kMaxCoefClasses];
 #pragma unroll
   for (int k = 0; k < kMaxCoefClasses; ++k) cc[k] = k < n_cls ? cls[k].c : 0.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
-    const double xi = x[i];
-    double acc = y_in ? yscale * y_in[i] : 0.0;
+    const double xi = x[i] * xs;
+    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
     // the row's state (for the diagonal) is loaded before the gathers so its
     // latency overlaps theirs
     const State<NW> si = load_state_cached<NW>(b.states, i);
@@ -481,7 +493,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
         }
       }
 #pragma unroll
-      for (int u = 0; u < PIPE; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] : 0.0;
+      for (int u = 0; u < PIPE; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] * xs : 0.0;
 #pragma unroll
       for (int u = 0; u < PIPE; ++u) {
         if (J[u] != 0xFFFFFFFFu && fabs(xj[u]) > kApplyTol) {
@@ -513,15 +525,19 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_hash(
     BasisView b, HashView hv, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
+  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
+  const double xs = xs_dev ? xs_dev[0] : 1.0;
+  const double ys = ys_dev ? ys_dev[0] : 1.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
     const State<NW> s = load_state_cached<NW>(b.states, i);
-    const double xi = x[i];
-    double acc = y_in ? yscale * y_in[i] : 0.0;
+    const double xi = x[i] * xs;
+    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
     for (int t0 = 0; t0 < n_off; t0 += 64) {
       const int tn = min(64, n_off - t0);
       uint64_t mask = 0;
@@ -578,7 +594,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_hash(
         }
         double xj[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) xj[g] = (J[g] >= 0) ? x[J[g]] : 0.0;
+        for (int g = 0; g < G; ++g) xj[g] = (J[g] >= 0) ? x[J[g]] * xs : 0.0;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if (J[g] >= 0 && fabs(xj[g]) > kApplyTol) {
@@ -638,7 +654,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials) {
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   __shared__ uint64_t rows_lds[kWavesPerBlock][64 * NW];
   __shared__ uint64_t win_lds[kWavesPerBlock][P][64 * NW];
   __shared__ double xwin_lds[kWavesPerBlock][P][64];
@@ -646,6 +663,9 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
   const int wid = threadIdx.x >> 6;
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
+  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
+  const double xs = xs_dev ? xs_dev[0] : 1.0;
+  const double ys = ys_dev ? ys_dev[0] : 1.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   const bool row_ok = i < b.dim;
   const State<NW> s = load_state_cached<NW>(b.states, row_ok ? i : 0);
@@ -654,8 +674,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
   double acc = 0.0;
   double xi = 0.0;
   if (row_ok) {
-    xi = x[i];
-    acc = y_in ? yscale * y_in[i] : 0.0;
+    xi = x[i] * xs;
+    acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
   }
   wave_sync();
   for (int t0 = 0; t0 < n_off; t0 += 64) {
@@ -697,7 +717,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
         const int64_t idx = gr[j] + lane;
         if (gm[j] != 0 && idx < b.dim) {
           wv[j] = load_state_cached<NW>(b.states, idx);
-          xv[j] = x[idx];
+          xv[j] = x[idx] * xs;
         }
       }
 #pragma unroll
@@ -757,7 +777,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
             const int64_t J = basis_lookup<NW>(b, key);
             if (J >= 0) {
               found = true;
-              xj = x[J];
+              xj = x[J] * xs;
             }
           }
           if (found && fabs(xj) > kApplyTol) {
@@ -862,10 +882,12 @@ __global__ __launch_bounds__(kVecBlock) void k_dot_partial(
 __global__ __launch_bounds__(kVecBlock) void k_axpy(double alpha,
                                                     const double* __restrict__ x,
                                                     double* __restrict__ y,
-                                                    int64_t n) {
+                                                    int64_t n,
+                                                    const double* __restrict__ xs_dev) {
+  const double xs = xs_dev ? xs_dev[0] : 1.0;  // x is read as xs * x
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
        i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock)
-    y[i] += alpha * x[i];
+    y[i] += alpha * (x[i] * xs);
 }
 
 __global__ __launch_bounds__(kVecBlock) void k_scal(double alpha,
@@ -879,8 +901,10 @@ __global__ __launch_bounds__(kVecBlock) void k_scal(double alpha,
 // u -= (*alpha) v, partials[b] = sum u^2 over the workgroup's share
 __global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
     const double* __restrict__ alpha, const double* __restrict__ v,
-    double* __restrict__ u, int64_t n, double* __restrict__ partials) {
+    double* __restrict__ u, int64_t n, double* __restrict__ partials,
+    const double* __restrict__ vs_dev) {
   const double a = alpha[0];
+  const double vs = vs_dev ? vs_dev[0] : 1.0;  // v is read as vs * v
   double acc = 0.0;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBlock;
   int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
@@ -889,7 +913,7 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
 #pragma unroll
     for (int k = 0; k < kVecUnroll; ++k) {
       uv[k] = u[i + k * stride];
-      vv[k] = v[i + k * stride];
+      vv[k] = v[i + k * stride] * vs;
     }
 #pragma unroll
     for (int k = 0; k < kVecUnroll; ++k) {
@@ -899,7 +923,7 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
     }
   }
   for (; i < n; i += stride) {
-    const double w = u[i] - a * v[i];
+    const double w = u[i] - a * (v[i] * vs);
     u[i] = w;
     acc += w * w;
   }
@@ -1124,13 +1148,14 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev(
 }
 
 // Psi[:, b] += c[b] * v for b < nb (Psi column-major, leading dimension ld):
-// one read of v feeds every column.
+// one read of v feeds every column.  v is read as vs * v.
 __global__ __launch_bounds__(kVecBlock) void k_accum_cols(
     const double* __restrict__ v, double* __restrict__ Psi, int64_t ld, int nb,
-    const double* __restrict__ c, int64_t n) {
+    const double* __restrict__ c, int64_t n, const double* __restrict__ vs_dev) {
+  const double vs = vs_dev ? vs_dev[0] : 1.0;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
        i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
-    const double vi = v[i];
+    const double vi = v[i] * vs;
     for (int b = 0; b < nb; ++b) Psi[b * ld + i] += c[b] * vi;
   }
 }
@@ -1164,20 +1189,21 @@ int64_t apply_num_blocks(const qsi_op_s* op) {
 template <int NW, bool DOT, bool ALIGNED>
 static void launch_cached_t(const qsi_op_s* op, dim3 grid, dim3 block, const BasisView& v,
                             const double* x, double* y, const double* y_in, double yscale,
-                            const double* yscale_dev, double* partials, hipStream_t s) {
+                            const double* yscale_dev, double* partials, const double* xs_dev,
+                            const double* ys_dev, hipStream_t s) {
   auto k = op->cached_pipe == 32 ? k_apply_cached<NW, DOT, 32, ALIGNED>
          : op->cached_pipe == 4  ? k_apply_cached<NW, DOT, 4, ALIGNED>
          : op->cached_pipe == 16 ? k_apply_cached<NW, DOT, 16, ALIGNED>
                                  : k_apply_cached<NW, DOT, 8, ALIGNED>;
   hipLaunchKernelGGL(k, grid, block, 0, s, v, op->d_cls, op->n_cls, op->d_rowptr, op->d_cj,
-                     op->d_cm, op->d_diag, op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+                     op->d_cm, op->d_diag, op->n_diag, x, y, y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
 }
 
 template <int NW, bool HAS_Z, bool DOT>
 static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
                            const double* y_in, double yscale,
                            const double* yscale_dev, double* partials,
-                           hipStream_t s) {
+                           const double* xs_dev, const double* ys_dev, hipStream_t s) {
   const int64_t nb = apply_num_blocks(op);
   const dim3 grid(static_cast<uint32_t>(nb)), block(kApplyBlock);
   const auto v = op->basis->view();
@@ -1195,60 +1221,60 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
       // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)
       if (op->conn_pad == 4)
         launch_cached_t<NW, DOT, true>(op, grid, block, v, x, y, y_in, yscale, yscale_dev,
-                                       partials, s);
+                                       partials, xs_dev, ys_dev, s);
       else
         launch_cached_t<NW, DOT, false>(op, grid, block, v, x, y, y_in, yscale, yscale_dev,
-                                        partials, s);
+                                        partials, xs_dev, ys_dev, s);
       break;
     case QSI_APPLY_LANE:
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LANE2:
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 2>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_WINDOW:
       hipLaunchKernelGGL((k_apply_win<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LANE8:
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 8>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_HASH:
       hipLaunchKernelGGL((k_apply_hash<NW, HAS_Z, DOT, 1>), grid, block, 0, s, v,
                          op->basis->hview(), op->d_off, op->n_off, op->d_diag,
-                         op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+                         op->n_diag, x, y, y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_HASH4:
       hipLaunchKernelGGL((k_apply_hash<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
                          op->basis->hview(), op->d_off, op->n_off, op->d_diag,
-                         op->n_diag, x, y, y_in, yscale, yscale_dev, partials);
+                         op->n_diag, x, y, y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_SLOT2:
       hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 2>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_SLOT4:
       hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_SLOT3:
       hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 3>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     default:  // QSI_APPLY_LANE4
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials);
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
   }
 
 }
@@ -1264,11 +1290,20 @@ int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
                      const double* y_in, double yscale,
                      const double* yscale_dev, double* dot_partials,
                      double* dot_out, hipStream_t s) {
+  return launch_apply_scaled(op, x, y, y_in, yscale, yscale_dev, nullptr, nullptr, dot_partials,
+                             dot_out, s);
+}
+
+int launch_apply_scaled(const qsi_op_s* op, const double* x, double* y,
+                        const double* y_in, double yscale, const double* yscale_dev,
+                        const double* xs_dev, const double* ys_dev, double* dot_partials,
+                        double* dot_out, hipStream_t s) {
   const bool dot = dot_partials != nullptr;
   const int nw = op->basis->nw;
   const bool hz = op->has_z;
 #define QSI_LAUNCH(NW_, HZ_, D_) \
-  launch_apply_t<NW_, HZ_, D_>(op, x, y, y_in, yscale, yscale_dev, dot_partials, s)
+  launch_apply_t<NW_, HZ_, D_>(op, x, y, y_in, yscale, yscale_dev, dot_partials, xs_dev, \
+                               ys_dev, s)
   if (nw == 1) {
     if (hz) { if (dot) QSI_LAUNCH(1, true, true); else QSI_LAUNCH(1, true, false); }
     else    { if (dot) QSI_LAUNCH(1, false, true); else QSI_LAUNCH(1, false, false); }
@@ -1327,9 +1362,9 @@ int launch_dot(const double* a, const double* b, int64_t n, double* partials,
 }
 
 int launch_axpy(double alpha, const double* x, double* y, int64_t n,
-                hipStream_t s) {
+                hipStream_t s, const double* xs_dev) {
   hipLaunchKernelGGL(k_axpy, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, alpha,
-                     x, y, n);
+                     x, y, n, xs_dev);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }
@@ -1344,10 +1379,10 @@ int launch_scal(double alpha, double* x, int64_t n, hipStream_t s) {
 // u -= alpha_dev * v; nrm_out[0] = ||u||, nrm_out[1] = 1/||u||
 int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
                      int64_t n, double* partials, double* nrm_out,
-                     hipStream_t s) {
+                     hipStream_t s, const double* vs_dev) {
   const int nb = vec_blocks(n);
   hipLaunchKernelGGL(k_axpy_dev_norm, dim3(nb), dim3(kVecBlock), 0, s,
-                     alpha_dev, v, u, n, partials);
+                     alpha_dev, v, u, n, partials, vs_dev);
   QSI_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials,
                      static_cast<int64_t>(nb), nrm_out, 1);
@@ -1452,9 +1487,9 @@ int launch_axpy_dev(const double* s_dev, double c, const double* x, double* y,
 }
 
 int launch_accum_cols(const double* v, double* Psi, int64_t ld, int nb, const double* c,
-                      int64_t n, hipStream_t s) {
+                      int64_t n, hipStream_t s, const double* vs_dev) {
   hipLaunchKernelGGL(k_accum_cols, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, v, Psi, ld, nb,
-                     c, n);
+                     c, n, vs_dev);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }

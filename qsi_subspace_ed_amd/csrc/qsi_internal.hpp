@@ -138,6 +138,14 @@ int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
                      const double* y_in, double yscale,
                      const double* yscale_dev, double* dot_partials,
                      double* dot_out, hipStream_t s);
+// as launch_apply_dev, with x read as xs_dev[0] * x and y_in as
+// ys_dev[0] * y_in (either may be null: scale 1).  The products are the
+// values a separate scaling pass would have stored, so the result has the
+// same bits as applying to the scaled vectors.
+int launch_apply_scaled(const qsi_op_s* op, const double* x, double* y,
+                        const double* y_in, double yscale, const double* yscale_dev,
+                        const double* xs_dev, const double* ys_dev, double* dot_partials,
+                        double* dot_out, hipStream_t s);
 int64_t apply_num_blocks(const qsi_op_s* op);
 
 // streaming vector kernels; partial buffers need max_vec_partials() doubles
@@ -147,12 +155,13 @@ int multidot_partials();
 int launch_dot(const double* a, const double* b, int64_t n, double* partials,
                double* out, hipStream_t s);
 int launch_axpy(double alpha, const double* x, double* y, int64_t n,
-                hipStream_t s);
+                hipStream_t s, const double* xs_dev = nullptr);
 int launch_scal(double alpha, double* x, int64_t n, hipStream_t s);
-// u -= alpha_dev[0] v; nrm_out[0] = ||u||, nrm_out[1] = 1/||u||
+// u -= alpha_dev[0] v; nrm_out[0] = ||u||, nrm_out[1] = 1/||u||; v is read as
+// vs_dev[0] * v when vs_dev is given
 int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
                      int64_t n, double* partials, double* nrm_out,
-                     hipStream_t s);
+                     hipStream_t s, const double* vs_dev = nullptr);
 int launch_scal_dev(const double* s_dev, double* x, int64_t n, hipStream_t s);
 // out[j] = V[:, j] . w for j < m (V column-major, ld ldv); acc_out += out
 int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
@@ -180,9 +189,10 @@ int launch_copy_scaled(const double* s_dev, const double* y, double* x,
 int launch_axpy_dev(const double* s_dev, double c, const double* x, double* y,
                     int64_t n, hipStream_t s);
 
-// Psi[:, b] += c[b] * v for b < nb (c on the device; Psi column-major)
+// Psi[:, b] += c[b] * v for b < nb (c on the device; Psi column-major); v is
+// read as vs_dev[0] * v when vs_dev is given
 int launch_accum_cols(const double* v, double* Psi, int64_t ld, int nb, const double* c,
-                      int64_t n, hipStream_t s);
+                      int64_t n, hipStream_t s, const double* vs_dev = nullptr);
 
 // The 3-term Lanczos coefficient run behind qsi_lanczos_coeffs on
 // caller-provided device buffers: v (D, the start vector, normalised here;

@@ -56,22 +56,32 @@ std::vector<double> random_unit(int64_t n, uint32_t seed) {
   return v;
 }
 
-// One Lanczos step on the device.  On entry v holds v_j (normalised), u holds
-// v_{j-1} (ignored when first).  On exit u = beta_j^{-1} (H v_j - alpha_j v_j
-// - beta_{j-1} v_{j-1}) — the next vector — and the caller swaps v and u.
-// alpha_j -> ab[0], beta_j -> ab[1], 1/beta_j -> ab[2]; beta_{j-1} is read
-// from prev_beta (device) when !first.
-int lanczos_step(const qsi_op_s* op, const double* v, double* u, bool first,
-                 const double* prev_beta, double* ab, double* partials,
-                 hipStream_t s) {
+// Step `it` of the 3-term recurrence on the device (lanczos.cpp:111-146).
+// The vectors are kept unnormalised: the buffer v holds v_it / s_it and u
+// holds v_{it-1} / s_{it-1}, where s_j = 1/beta_{j-1} sits on the device at
+// ab[3(j-1) + 2] (s_0 = 1: the start vector is normalised in place).  The
+// apply and the axpy read the buffers times their scales, which gives the
+// values a separate normalising pass would have stored, so the coefficients
+// have the same bits as with that pass, one pass fewer per step.
+// On exit u = beta_it v_{it+1} / ... unnormalised (its scale 1/beta_it at
+// ab[3 it + 2]); alpha_it -> ab[3 it], beta_it -> ab[3 it + 1]; the caller
+// swaps v and u.
+inline const double* lanczos_scale(const double* ab, int64_t j) {
+  return j == 0 ? nullptr : ab + 3 * (j - 1) + 2;
+}
+
+int lanczos_step(const qsi_op_s* op, const double* v, double* u, int64_t it,
+                 double* ab_base, double* partials, hipStream_t s) {
   const int64_t D = op->basis->dim;
-  // u = -beta_{j-1} u + H v;  alpha = <v, u>     (lanczos.cpp:117-126)
-  QSI_TRY(launch_apply_dev(op, v, u, first ? nullptr : u, -1.0,
-                           first ? nullptr : prev_beta, partials, ab, s));
-  // u -= alpha v;  beta = ||u||                   (lanczos.cpp:129-132)
-  QSI_TRY(launch_axpy_norm(ab, v, u, D, partials, ab + 1, s));
-  // v_{j+1} = u / beta                            (lanczos.cpp:136-139)
-  QSI_TRY(launch_scal_dev(ab + 2, u, D, s));
+  double* ab = ab_base + 3 * it;
+  const double* vs = lanczos_scale(ab_base, it);
+  // u = -beta_{it-1} v_{it-1} + H v_it;  alpha = <v_it, u>   (lanczos.cpp:117-126)
+  QSI_TRY(launch_apply_scaled(op, v, u, it == 0 ? nullptr : u, -1.0,
+                              it == 0 ? nullptr : ab - 2, vs,
+                              it == 0 ? nullptr : lanczos_scale(ab_base, it - 1), partials, ab,
+                              s));
+  // u -= alpha v_it;  beta = ||u||, 1/beta                   (lanczos.cpp:129-139)
+  QSI_TRY(launch_axpy_norm(ab, v, u, D, partials, ab + 1, s, vs));
   return QSI_OK;
 }
 
@@ -114,7 +124,7 @@ int lanczos_coeffs_run(const qsi_op_s* op, double* v, double* u, double* partial
   for (int it0 = 0; it0 < M; it0 += kChunk) {
     const int it1 = std::min(M, it0 + kChunk);
     for (int it = it0; it < it1; ++it) {
-      QSI_TRY(lanczos_step(op, v, u, it == 0, ab + 3 * (it - 1) + 1, ab + 3 * it, partials, s));
+      QSI_TRY(lanczos_step(op, v, u, it, ab, partials, s));
       std::swap(v, u);
     }
     QSI_CHECK_HIP(hipMemcpy(hab, ab + 3 * it0, 3 * (it1 - it0) * sizeof(double),
@@ -180,8 +190,7 @@ int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t* st,
     for (int64_t it0 = 0; it0 < max_it && !stop; it0 += chunk) {
       const int64_t it1 = std::min(max_it, it0 + chunk);
       for (int64_t it = it0; it < it1; ++it) {
-        QSI_TRY(lanczos_step(op, v, u, it == 0, bab.d() + 3 * (it - 1) + 1,
-                             bab.d() + 3 * it, bpart.d(), s));
+        QSI_TRY(lanczos_step(op, v, u, it, bab.d(), bpart.d(), s));
         std::swap(v, u);
       }
       hab.resize(static_cast<size_t>(3 * (it1 - it0)));
@@ -253,10 +262,9 @@ int qsi_lanczos_eigval0(qsi_op_t op, const qsi_lanczos_settings_t* st,
     QSI_CHECK_HIP(hipMemsetAsync(d_eigvec, 0, bytes, s));
     for (int64_t it = 0; it <= last_acc; ++it) {
       if (it < static_cast<int64_t>(ritz.size()))
-        QSI_TRY(launch_axpy(ritz[it], v, d_eigvec, D, s));
+        QSI_TRY(launch_axpy(ritz[it], v, d_eigvec, D, s, lanczos_scale(bab.d(), it)));
       if (it == last_acc) break;
-      QSI_TRY(lanczos_step(op, v, u, it == 0, bab.d() + 3 * (it - 1) + 1,
-                           bab.d() + 3 * it, bpart.d(), s));
+      QSI_TRY(lanczos_step(op, v, u, it, bab.d(), bpart.d(), s));
       std::swap(v, u);
     }
     QSI_CHECK_HIP(hipStreamSynchronize(s));
@@ -371,10 +379,10 @@ int qsi_ftlm_sample(qsi_op_t H, const qsi_op_t* ops, int n_ops, const double* d_
   double* u = bu.d();
   for (int it = 0; it < n; ++it) {
     QSI_TRY(launch_accum_cols(v, bpsi.d(), static_cast<int64_t>(ld), n_beta,
-                              bc.d() + static_cast<size_t>(it) * n_beta, D, s));
+                              bc.d() + static_cast<size_t>(it) * n_beta, D, s,
+                              lanczos_scale(bab.d(), it)));
     if (it + 1 < n) {
-      QSI_TRY(lanczos_step(H, v, u, it == 0, bab.d() + 3 * (it - 1) + 1, bab.d() + 3 * it,
-                           bpart.d(), s));
+      QSI_TRY(lanczos_step(H, v, u, it, bab.d(), bpart.d(), s));
       std::swap(v, u);
     }
   }
@@ -510,9 +518,9 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
     QSI_CHECK_HIP(hipMemcpyAsync(V, d_v0, static_cast<size_t>(D) * sizeof(double),
                                  hipMemcpyDeviceToDevice, s));
   } else {
-    const std::vector<double> v0 = random_unit(D, static_cast<uint32_t>(seed));
-    QSI_CHECK_HIP(hipMemcpy(V, v0.data(), static_cast<size_t>(D) * sizeof(double),
-                            hipMemcpyHostToDevice));
+    // Spectra's solver takes no seeded start vector (matrix_diag_bits.hpp:75),
+    // so none needs reproducing: a counter-based Gaussian made on the device
+    QSI_TRY(launch_randn(V, D, seed, s));
   }
   double nrm2 = 0.0;
   QSI_TRY(launch_dot(V, V, D, bpart.d(), bscal.d(), s));
