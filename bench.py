@@ -35,13 +35,13 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_64spin_pmc.json")
 PMC_SUMMARY_96 = os.path.join(ROOT, "profiles", "apply_96spin_pmc.json")
 PMC_SUMMARY_96_CACHED = os.path.join(ROOT, "profiles", "apply_96spin_cached_pmc.json")
 PMC_SUMMARY_CACHED = os.path.join(ROOT, "profiles", "apply_64spin_cached_pmc.json")
-CALIB_BYTES = 64 << 20  # the calibration copy reads and writes this many bytes
 
 
 def measured_traffic(path=PMC_SUMMARY):
-    """HBM bytes per apply from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE of the apply kernel, each divided by what the
-    same counter reads for a copy of known size), or None."""
+    """Bytes per apply that cross the L2/fabric interface (to MALL or HBM),
+    from the committed rocprofv3 counter summary of the apply kernel: read
+    requests counted by size, 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B,
+    plus writes, 64*WRREQ_64B + 32*(WRREQ - WRREQ_64B); None if absent."""
     try:
         with open(path) as f:
             return float(json.load(f)["traffic_bytes_per_apply"])
@@ -520,13 +520,6 @@ def bench_96(args):
         "enumeration_s": t_enum,
         "algorithmic_bytes_per_apply": int(bytes_apply),
     }
-    if args.calib_copy:
-        a = torch.full((CALIB_BYTES // 8,), 1.0, dtype=torch.float64, device="cuda")
-        c = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=c)
-        torch.cuda.synchronize()
-        del a, c
     # parity and CPU baseline on a bounded sample of source rows [0, n): the
     # oracle pushes those rows; the device pulls with x zero outside them
     if not args.no_cpu:
@@ -672,14 +665,6 @@ def bench_64(args):
     S = 8  # bytes per state (<= 64 spins)
     bytes_apply = D * (S + 8 + 8) + 8 * nnz
     achieved = bytes_apply / kern_s / 1e9
-
-    if args.calib_copy:
-        a = torch.full((CALIB_BYTES // 8,), 1.0, dtype=torch.float64, device="cuda")
-        c = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=c)
-        torch.cuda.synchronize()
-        del a, c
 
     lanczos = None
     eigsh = None
@@ -827,8 +812,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="also run a 64 MiB device copy (read+write) for counter calibration")
     ap.add_argument("--no-solvers", action="store_true", help="skip the Lanczos/eigsh extras")
     ap.add_argument("--variant", default="auto",
                     help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")

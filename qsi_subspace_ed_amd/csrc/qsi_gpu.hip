@@ -179,9 +179,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
     const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
-  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
-  const double xs = xs_dev ? xs_dev[0] : 1.0;
-  const double ys = ys_dev ? ys_dev[0] : 1.0;
+  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
+  // instances (the Lanczos step) take scales; elsewhere they are the
+  // constant 1 and the multiplies fold away.
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
@@ -242,11 +244,15 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
           xj[g] = 0.0;
           if (n[g] == 1) {
             f[g] = load_state_cached<NW>(b.states, lo[g]);
-            xj[g] = x[lo[g]] * xs;
+            xj[g] = x[lo[g]];
           }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) hit[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]);
+        // the scale is applied where the values are consumed, after every
+        // gather of the group has been issued
+#pragma unroll
+        for (int g = 0; g < G; ++g) xj[g] *= xs;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if (hit[g] && fabs(xj[g]) > kApplyTol) {
@@ -325,9 +331,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
     const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
-  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
-  const double xs = xs_dev ? xs_dev[0] : 1.0;
-  const double ys = ys_dev ? ys_dev[0] : 1.0;
+  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
+  // instances (the Lanczos step) take scales; elsewhere they are the
+  // constant 1 and the multiplies fold away.
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
   __shared__ TermsLds<NW> tl;
   const bool lds = stage_terms<NW>(tl, off, n_off);
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
           for (int g = 0; g < G; ++g) {
             if ((tg[g] >> 8) == 2) {
               f[g] = load_state_cached<NW>(b.states, J[g]);
-              xj[g] = x[J[g]] * xs;
+              xj[g] = x[J[g]];
             }
           }
 #pragma unroll
@@ -414,6 +422,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_slot(
             }
           }
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g) xj[g] *= xs;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if ((tg[g] >> 8) == 3 && fabs(xj[g]) > kApplyTol) {
@@ -454,9 +464,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
     const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
-  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
-  const double xs = xs_dev ? xs_dev[0] : 1.0;
-  const double ys = ys_dev ? ys_dev[0] : 1.0;
+  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
+  // instances (the Lanczos step) take scales; elsewhere they are the
+  // constant 1 and the multiplies fold away.
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
   double cc[kMaxCoefClasses];
 #pragma unroll
   for (int k = 0; k < kMaxCoefClasses; ++k) cc[k] = k < n_cls ? cls[k].c : 0.0;
@@ -493,7 +505,10 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_cached(
         }
       }
 #pragma unroll
-      for (int u = 0; u < PIPE; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] * xs : 0.0;
+      for (int u = 0; u < PIPE; ++u) xj[u] = J[u] != 0xFFFFFFFFu ? x[J[u]] : 0.0;
+      // the scale is applied after every gather of the group has been issued
+#pragma unroll
+      for (int u = 0; u < PIPE; ++u) xj[u] *= xs;
 #pragma unroll
       for (int u = 0; u < PIPE; ++u) {
         if (J[u] != 0xFFFFFFFFu && fabs(xj[u]) > kApplyTol) {
@@ -529,9 +544,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_hash(
     const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
-  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
-  const double xs = xs_dev ? xs_dev[0] : 1.0;
-  const double ys = ys_dev ? ys_dev[0] : 1.0;
+  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
+  // instances (the Lanczos step) take scales; elsewhere they are the
+  // constant 1 and the multiplies fold away.
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
@@ -594,7 +611,9 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_hash(
         }
         double xj[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) xj[g] = (J[g] >= 0) ? x[J[g]] * xs : 0.0;
+        for (int g = 0; g < G; ++g) xj[g] = (J[g] >= 0) ? x[J[g]] : 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) xj[g] *= xs;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if (J[g] >= 0 && fabs(xj[g]) > kApplyTol) {
@@ -663,9 +682,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_win(
   const int wid = threadIdx.x >> 6;
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   if (yscale_dev) yscale *= yscale_dev[0];
-  // x and y_in are read as xs * x and ys * y_in (exact when the scales are 1)
-  const double xs = xs_dev ? xs_dev[0] : 1.0;
-  const double ys = ys_dev ? ys_dev[0] : 1.0;
+  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
+  // instances (the Lanczos step) take scales; elsewhere they are the
+  // constant 1 and the multiplies fold away.
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   const bool row_ok = i < b.dim;
   const State<NW> s = load_state_cached<NW>(b.states, row_ok ? i : 0);
@@ -1294,11 +1315,33 @@ int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
                              dot_out, s);
 }
 
+// Sums n per-workgroup partials into out (mode as in k_finalize_sum) in a
+// fixed order.  Arrays longer than 8 chunks are first reduced chunk by chunk
+// in place (k_sum_chunks), so a single workgroup never streams millions of
+// partials; the partials are clobbered.
+static void launch_finalize(double* partials, int64_t n, double* out, int mode,
+                            hipStream_t s) {
+  constexpr int64_t kChunk = 4096;
+  if (n > 8 * kChunk) {
+    const int64_t g = (n + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_sum_chunks<256>, dim3(static_cast<uint32_t>(g)), dim3(256), 0, s,
+                       partials, n, kChunk);
+    hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials, g, out, mode,
+                       kChunk);
+  } else {
+    hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials, n, out, mode,
+                       int64_t{1});
+  }
+}
+
 int launch_apply_scaled(const qsi_op_s* op, const double* x, double* y,
                         const double* y_in, double yscale, const double* yscale_dev,
                         const double* xs_dev, const double* ys_dev, double* dot_partials,
                         double* dot_out, hipStream_t s) {
   const bool dot = dot_partials != nullptr;
+  // the kernels honour the scales only in their dot-product instances
+  QSI_REQUIRE(dot || (xs_dev == nullptr && ys_dev == nullptr),
+              "launch_apply_scaled: scales need the dot product");
   const int nw = op->basis->nw;
   const bool hz = op->has_z;
 #define QSI_LAUNCH(NW_, HZ_, D_) \
@@ -1314,8 +1357,7 @@ int launch_apply_scaled(const qsi_op_s* op, const double* x, double* y,
 #undef QSI_LAUNCH
   QSI_CHECK_LAUNCH();
   if (dot) {
-    hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s,
-                       dot_partials, apply_num_blocks(op), dot_out, 0);
+    launch_finalize(dot_partials, apply_num_blocks(op), dot_out, 0, s);
     QSI_CHECK_LAUNCH();
   }
   return QSI_OK;
@@ -1355,8 +1397,7 @@ int launch_dot(const double* a, const double* b, int64_t n, double* partials,
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kVecBlock), 0, s, a, b, n,
                      partials);
   QSI_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials,
-                     static_cast<int64_t>(nb), out, 0);
+  launch_finalize(partials, nb, out, 0, s);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }
@@ -1384,8 +1425,7 @@ int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
   hipLaunchKernelGGL(k_axpy_dev_norm, dim3(nb), dim3(kVecBlock), 0, s,
                      alpha_dev, v, u, n, partials, vs_dev);
   QSI_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_finalize_sum<256>, dim3(1), dim3(256), 0, s, partials,
-                     static_cast<int64_t>(nb), nrm_out, 1);
+  launch_finalize(partials, nb, nrm_out, 1, s);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }

@@ -141,7 +141,8 @@ int launch_apply_dev(const qsi_op_s* op, const double* x, double* y,
 // as launch_apply_dev, with x read as xs_dev[0] * x and y_in as
 // ys_dev[0] * y_in (either may be null: scale 1).  The products are the
 // values a separate scaling pass would have stored, so the result has the
-// same bits as applying to the scaled vectors.
+// same bits as applying to the scaled vectors.  Scales need dot_partials
+// (only the dot-product kernel instances read them).
 int launch_apply_scaled(const qsi_op_s* op, const double* x, double* y,
                         const double* y_in, double yscale, const double* yscale_dev,
                         const double* xs_dev, const double* ys_dev, double* dot_partials,
