@@ -455,6 +455,11 @@ def bench_96(args):
         basis.build_slots()
         print(f"[bench] slot index built in {time.perf_counter() - ts:.1f} s", file=sys.stderr,
               flush=True)
+    if not args.no_lowkeys:
+        ts = time.perf_counter()
+        basis.build_lowkeys()
+        print(f"[bench] low keys built in {time.perf_counter() - ts:.1f} s", file=sys.stderr,
+              flush=True)
     op = q.GpuOp(basis, H)
     nnz = C.c_int64(0)
     _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
@@ -464,7 +469,8 @@ def bench_96(args):
     x /= torch.linalg.norm(x)
     y = torch.zeros_like(x)
     variants_ms = time_variants(op, x, y, ["lane4"] + ([] if args.no_slots else
-                                                       ["slot2", "slot4", "slot3"]))
+                                                       ["slot2", "slot4", "slot3"])
+                                + ([] if args.no_lowkeys else ["low2", "low4", "low8"]))
     tuned = op.autotune(0, reps=1, x=x, y=y) if args.variant == "auto" else None
     if tuned is None:
         op.set_variant(args.variant)
@@ -517,6 +523,7 @@ def bench_96(args):
         "variants_ms": variants_ms,
         "bucket_bits": basis.bucket_bits(),
         "slot_index": basis.has_slots(),
+        "low_keys": basis.has_lowkeys(),
         "enumeration_s": t_enum,
         "algorithmic_bytes_per_apply": int(bytes_apply),
     }
@@ -553,10 +560,12 @@ def bench_96(args):
         out["speedup_vs_cpu"] = out["value"] / rate
         torch.cuda.empty_cache()
     # the same apply through the cached connectivity, when it fits in HBM
-    # (the slot index is dropped first: the cached kernel does not search)
+    # (the slot index and low keys are dropped first: the cached kernel does
+    # not search)
     cached = None
     if not args.no_cached:
         basis.build_slots(False)
+        basis.build_lowkeys(False)
         torch.cuda.empty_cache()
     free, _ = torch.cuda.mem_get_info()
     need = 5 * nnz + 8 * (D + 1)
@@ -630,6 +639,8 @@ def bench_64(args):
                        hash_index=args.variant.startswith("hash"))
     if not args.no_slots:
         basis.build_slots()
+    if not args.no_lowkeys:
+        basis.build_lowkeys()
     op = q.GpuOp(basis, H)
     tuned = None
     if args.variant == "auto":
@@ -825,6 +836,8 @@ def main():
                          "apply and its extras alone")
     ap.add_argument("--no-extras", action="store_true",
                     help="96spin: skip the 64-spin, FTLM and served sections")
+    ap.add_argument("--no-lowkeys", action="store_true",
+                    help="do not build the low keys (no QSI_APPLY_LOW* variants)")
     ap.add_argument("--no-slots", action="store_true",
                     help="do not build the slot index (search by bucket bisection only)")
     args = ap.parse_args()

@@ -105,6 +105,17 @@ int qsi_basis_has_hash(qsi_basis_t b);
  * reference (its buckets are searched by bisection, operator.cpp:38-70). */
 int qsi_basis_build_slots(qsi_basis_t b, int enable);
 int qsi_basis_has_slots(qsi_basis_t b);
+/* Builds (enable = 1) or drops (enable = 0) the basis's low keys: for every
+ * state, the informative positions below the bucket key's lowest bit (the
+ * positions at which neighbouring basis states first differ) packed into one
+ * uint32 (4 bytes per state).  Within a bucket the low keys are distinct and
+ * ordered like the states, so the QSI_APPLY_LOW* kernels bisect a bucket on
+ * them and load one candidate state per lookup.  Fails (QSI_E_INVALID) if
+ * more than 32 informative positions lie below the key.  Dropping
+ * synchronises the device.  No counterpart in the reference (its buckets are
+ * bisected on whole states, operator.cpp:38-70). */
+int qsi_basis_build_lowkeys(qsi_basis_t b, int enable);
+int qsi_basis_has_lowkeys(qsi_basis_t b);
 int qsi_basis_destroy(qsi_basis_t b);
 int64_t qsi_basis_dim(qsi_basis_t b);
 int qsi_basis_n_spins(qsi_basis_t b);
@@ -147,9 +158,11 @@ int qsi_op_n_diag(qsi_op_t op);
  * QSI_APPLY_SLOT{2,3,4}: as LANE*, through the slot index under each bucket
  * (qsi_basis_build_slots), 2/3/4 lookups in flight per lane; setting one
  * requires the index; if the index is dropped later the apply falls back to
- * LANE4 (SLOT4) or LANE2.  AUTO =
- * CACHED when the connectivity is cached, else SLOT4 when the basis has a slot
- * index, else LANE4.  All give bit-identical y. */
+ * LANE4 (SLOT4) or LANE2.  QSI_APPLY_LOW{2,4,8}: as LANE*, bisecting each
+ * bucket on the basis's low keys (qsi_basis_build_lowkeys); setting one
+ * requires them, and without them later the apply falls back to LANE* of the
+ * same depth.  AUTO = CACHED when the connectivity is cached, else LANE4.
+ * All give bit-identical y. */
 enum {
   QSI_APPLY_AUTO = 0,
   QSI_APPLY_WINDOW = 1,
@@ -162,7 +175,10 @@ enum {
   QSI_APPLY_CACHED = 8,
   QSI_APPLY_SLOT2 = 9,
   QSI_APPLY_SLOT4 = 10,
-  QSI_APPLY_SLOT3 = 11
+  QSI_APPLY_SLOT3 = 11,
+  QSI_APPLY_LOW2 = 12,
+  QSI_APPLY_LOW4 = 13,
+  QSI_APPLY_LOW8 = 14
 };
 int qsi_op_set_variant(qsi_op_t op, int variant);
 int qsi_op_get_variant(qsi_op_t op);

@@ -29,10 +29,14 @@ QSI_APPLY_CACHED = 8
 QSI_APPLY_SLOT2 = 9
 QSI_APPLY_SLOT4 = 10
 QSI_APPLY_SLOT3 = 11
+QSI_APPLY_LOW2 = 12
+QSI_APPLY_LOW4 = 13
+QSI_APPLY_LOW8 = 14
 VARIANTS = {"auto": QSI_APPLY_AUTO, "window": QSI_APPLY_WINDOW, "lane": QSI_APPLY_LANE,
             "lane2": QSI_APPLY_LANE2, "lane4": QSI_APPLY_LANE4, "lane8": QSI_APPLY_LANE8,
             "hash": QSI_APPLY_HASH, "hash4": QSI_APPLY_HASH4, "cached": QSI_APPLY_CACHED,
-            "slot2": QSI_APPLY_SLOT2, "slot3": QSI_APPLY_SLOT3, "slot4": QSI_APPLY_SLOT4}
+            "slot2": QSI_APPLY_SLOT2, "slot3": QSI_APPLY_SLOT3, "slot4": QSI_APPLY_SLOT4,
+            "low2": QSI_APPLY_LOW2, "low4": QSI_APPLY_LOW4, "low8": QSI_APPLY_LOW8}
 
 
 class QsiError(RuntimeError):
@@ -93,6 +97,8 @@ SIGNATURES = {
     "qsi_basis_has_hash": (C.c_int, [_P]),
     "qsi_basis_build_slots": (C.c_int, [_P, C.c_int]),
     "qsi_basis_has_slots": (C.c_int, [_P]),
+    "qsi_basis_build_lowkeys": (C.c_int, [_P, C.c_int]),
+    "qsi_basis_has_lowkeys": (C.c_int, [_P]),
     "qsi_basis_destroy": (C.c_int, [_P]),
     "qsi_basis_dim": (_I64, [_P]),
     "qsi_basis_n_spins": (C.c_int, [_P]),

@@ -1328,6 +1328,11 @@ extern "C" int qsi_op_autotune(qsi_op_t op, int R, int reps, double* d_x, double
       cand.push_back(QSI_APPLY_SLOT3);
       cand.push_back(QSI_APPLY_SLOT4);
     }
+    if (op->basis->d_lowkeys) {
+      cand.push_back(QSI_APPLY_LOW2);
+      cand.push_back(QSI_APPLY_LOW4);
+      cand.push_back(QSI_APPLY_LOW8);
+    }
     // the cached kernel once per gather depth (4, 8, 16, 32)
     std::vector<int> pipes(cand.size(), 8);
     if (op->conn_nnz >= 0) {

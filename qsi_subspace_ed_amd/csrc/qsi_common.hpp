@@ -106,17 +106,32 @@ __device__ __forceinline__ bool st_eq(const State<NW>& a, const State<NW>& b) {
 // positions the key is the top K bits.
 constexpr int kKeySegs = 4;
 
-// Device view of a basis: states + bucket table (+ slot index).
+// Low keys.  The informative positions below the bucket key's lowest bit,
+// packed highest first into one 32-bit word per state (at most kLowBits of
+// them, in at most kLowSegs runs of adjacent positions).  Two basis states
+// always differ at some informative position, so within one bucket (equal
+// key bits) the low keys are distinct and ordered like the states: a lookup
+// can bisect a bucket on 4-byte low keys instead of whole states, and only
+// the candidate's state needs loading (a target outside the basis may share
+// its informative bits with a basis state and differ at a fixed position).
+constexpr int kLowSegs = 32;
+constexpr int kLowBits = 32;
+
+// Device view of a basis: states + bucket table (+ slot index, low keys).
 struct BasisView {
   const uint64_t* states;  // D * NW words
   const uint32_t* table;   // 2^K + 1 entries: table[k] = #states with key < k
   const uint32_t* slots;   // slot index (2 D entries) or null
+  const uint32_t* lowkeys; // D low keys or null
   int64_t dim;
   int kbits;               // K = sum of the field widths
   int n_spins;
   int n_seg;
   uint8_t seg_lo[kKeySegs];
   uint8_t seg_w[kKeySegs];
+  int n_low;               // low-key runs, highest first
+  uint8_t low_lo[kLowSegs];
+  uint8_t low_w[kLowSegs];
 };
 
 template <int NW>
@@ -139,6 +154,14 @@ __host__ __device__ __forceinline__ uint32_t bucket_key(const BasisView& b, cons
   for (int q = 0; q < kKeySegs; ++q) {
     if (q < b.n_seg) k = (k << b.seg_w[q]) | state_field<NW>(s, b.seg_lo[q], b.seg_w[q]);
   }
+  return k;
+}
+
+template <int NW>
+__host__ __device__ __forceinline__ uint32_t low_key(const BasisView& b, const State<NW>& s) {
+  uint32_t k = 0;
+  for (int q = 0; q < b.n_low; ++q)
+    k = (k << b.low_w[q]) | state_field<NW>(s, b.low_lo[q], b.low_w[q]);
   return k;
 }
 

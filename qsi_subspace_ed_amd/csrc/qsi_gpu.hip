@@ -125,6 +125,14 @@ __global__ void k_build_hash(const uint64_t* __restrict__ states, int64_t dim,
 // kSlotEmpty): linear probing from the state's home, a slot claimed by a
 // compare-and-swap.  A bucket above kSlotMaxBucket states sets *too_big and
 // places nothing.
+// lowkeys[i] = low key of state i (qsi_basis_build_lowkeys)
+template <int NW>
+__global__ void k_build_lowkeys(BasisView b, uint32_t* __restrict__ lowkeys) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.dim) return;
+  lowkeys[i] = low_key<NW>(b, load_state_cached<NW>(b.states, i));
+}
+
 template <int NW>
 __global__ void k_build_slots(BasisView b, uint32_t* __restrict__ slots,
                               unsigned int* __restrict__ too_big) {
@@ -251,6 +259,111 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
         for (int g = 0; g < G; ++g) hit[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]);
         // the scale is applied where the values are consumed, after every
         // gather of the group has been issued
+#pragma unroll
+        for (int g = 0; g < G; ++g) xj[g] *= xs;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (hit[g] && fabs(xj[g]) > kApplyTol) {
+            const OffTerm& T = off[tt[g]];
+            double c = T.c * T.sgn;
+            if constexpr (HAS_Z) {
+              if (parity_notz<NW>(s, T.Z)) c = -c;
+            }
+            acc += c * xj[g];
+          }
+        }
+      }
+    }
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
+}
+
+// Low-key variant (QSI_APPLY_LOW*): as the lane variant, but the bisection
+// of a bucket probes the 4-byte low keys (qsi_basis_build_lowkeys) instead of
+// whole states — four times as many candidates per cache line, and the state
+// array is touched once per lookup, for the candidate.  The candidate is the
+// last state of the bucket whose low key is <= the target's, the same state
+// the lane variant's bisection ends on, so y is bit-identical.
+template <int NW, bool HAS_Z, bool DOT, int G>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_low(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  // scales as in k_apply_lane (dot-product instances only)
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  double prod = 0.0;
+  if (i < b.dim) {
+    const State<NW> s = load_state_cached<NW>(b.states, i);
+    const double xi = x[i] * xs;
+    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
+    for (int t0 = 0; t0 < n_off; t0 += 64) {
+      const int tn = min(64, n_off - t0);
+      uint64_t mask = 0;
+      for (int t = 0; t < tn; ++t) {
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
+      }
+      while (mask) {
+        int tt[G];
+        bool act[G];
+        State<NW> key[G];
+        uint32_t lo[G], n[G], r[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          act[g] = mask != 0;
+          tt[g] = act[g] ? t0 + __builtin_ctzll(mask) : t0;
+          mask &= mask - 1;
+          key[g] = xor_state<NW>(s, off[tt[g]].X);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          lo[g] = 0;
+          n[g] = 0;
+          r[g] = 0;
+          if (act[g]) {
+            bucket_bounds(b.table, bucket_key<NW>(b, key[g]), lo[g], n[g]);
+            r[g] = low_key<NW>(b, key[g]);
+          }
+        }
+        // interleaved bisection on the low keys down to one candidate
+        bool more = true;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (n[g] > 1) {
+              const uint32_t half = n[g] >> 1;
+              const uint32_t m = b.lowkeys[lo[g] + half];
+              lo[g] = m <= r[g] ? lo[g] + half : lo[g];
+              n[g] -= half;
+              more |= n[g] > 1;
+            }
+          }
+        }
+        double xj[G];
+        bool hit[G];
+        State<NW> f[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          xj[g] = 0.0;
+          if (n[g] == 1) {
+            f[g] = load_state_cached<NW>(b.states, lo[g]);
+            xj[g] = x[lo[g]];
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) hit[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]);
 #pragma unroll
         for (int g = 0; g < G; ++g) xj[g] *= xs;
 #pragma unroll
@@ -1199,6 +1312,7 @@ void basis_release(qsi_basis_s* b) {
     if (b->d_table) (void)hipFree(b->d_table);
     if (b->d_hash) (void)hipFree(b->d_hash);
     if (b->d_slots) (void)hipFree(b->d_slots);
+    if (b->d_lowkeys) (void)hipFree(b->d_lowkeys);
     delete b;
   }
 }
@@ -1229,14 +1343,19 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   const dim3 grid(static_cast<uint32_t>(nb)), block(kApplyBlock);
   const auto v = op->basis->view();
   int variant = op->variant;
+  // AUTO: the cached gathers when the connectivity is cached, else LANE4 (the
+  // fastest search kernel measured on the 64- and 96-spin bases, with or
+  // without the slot index and low keys; the autotuner times them all)
   if (variant == QSI_APPLY_AUTO)
-    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED
-            : op->basis->d_slots ? QSI_APPLY_SLOT4 : QSI_APPLY_LANE4;
-  // a slot variant chosen while the basis had its slot index, which has been
-  // dropped since, falls back to a bisection kernel (LANE4 for SLOT4, LANE2
-  // for SLOT2 and SLOT3)
+    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE4;
+  // a slot or low-key variant chosen while the basis had that index, which
+  // has been dropped since, falls back to the lane kernel (LANE4 for SLOT4,
+  // LANE2 for SLOT2 and SLOT3; the same depth for LOW*)
   if (!op->basis->d_slots && variant >= QSI_APPLY_SLOT2 && variant <= QSI_APPLY_SLOT3)
     variant = variant == QSI_APPLY_SLOT4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE2;
+  if (!op->basis->d_lowkeys && variant >= QSI_APPLY_LOW2 && variant <= QSI_APPLY_LOW8)
+    variant = variant == QSI_APPLY_LOW2 ? QSI_APPLY_LANE2
+            : variant == QSI_APPLY_LOW4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE8;
   switch (variant) {
     case QSI_APPLY_CACHED:
       // gathers in flight per lane: op->cached_pipe (4, 8, 16 or 32; autotuned)
@@ -1289,6 +1408,21 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
       break;
     case QSI_APPLY_SLOT3:
       hipLaunchKernelGGL((k_apply_slot<NW, HAS_Z, DOT, 3>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      break;
+    case QSI_APPLY_LOW2:
+      hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      break;
+    case QSI_APPLY_LOW4:
+      hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
+                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      break;
+    case QSI_APPLY_LOW8:
+      hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8>), grid, block, 0, s, v,
                          op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
                          y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
@@ -1596,6 +1730,8 @@ static hipError_t basis_build_index(qsi_basis_s* b, int bucket_bits) {
     (void)hipFree(dmask);
     if (e != hipSuccess) return e;
   }
+  b->infmask[0] = hmask[0];
+  b->infmask[1] = hmask[1];
   // informative positions from the top; a run starts a new field, and once
   // kKeySegs fields exist the last one is extended down over the gap (the
   // fixed positions it then covers do not break the ordering)
@@ -1793,6 +1929,69 @@ int qsi_basis_build_slots(qsi_basis_t b, int enable) {
 }
 
 int qsi_basis_has_slots(qsi_basis_t b) { return b && b->d_slots ? 1 : 0; }
+
+int qsi_basis_build_lowkeys(qsi_basis_t b, int enable) {
+  QSI_REQUIRE(b != nullptr, "qsi_basis_build_lowkeys: null basis");
+  QSI_REQUIRE(enable == 0 || enable == 1, "qsi_basis_build_lowkeys: enable must be 0 or 1");
+  if (!enable || b->d_lowkeys) {
+    if (!enable && b->d_lowkeys) {
+      QSI_CHECK_HIP(hipDeviceSynchronize());  // no apply may still read it
+      QSI_CHECK_HIP(hipFree(b->d_lowkeys));
+      b->d_lowkeys = nullptr;
+    }
+    return QSI_OK;
+  }
+  if (b->dim == 0) return QSI_OK;
+  // the informative positions below the key's lowest bit, as runs from the top
+  int key_lo = b->n_spins;
+  for (int q = 0; q < b->n_seg; ++q) key_lo = std::min<int>(key_lo, b->seg_lo[q]);
+  int n_low = 0, bits = 0;
+  uint8_t lo[qsi::kLowSegs] = {}, w[qsi::kLowSegs] = {};
+  for (int p = key_lo - 1; p >= 0; --p) {
+    if (!((b->infmask[p >> 6] >> (p & 63)) & 1ull)) continue;
+    ++bits;
+    if (n_low > 0 && lo[n_low - 1] == p + 1) {
+      lo[n_low - 1] = static_cast<uint8_t>(p);
+      w[n_low - 1] += 1;
+    } else {
+      if (n_low == qsi::kLowSegs) { n_low = qsi::kLowSegs + 1; break; }
+      lo[n_low] = static_cast<uint8_t>(p);
+      w[n_low] = 1;
+      ++n_low;
+    }
+  }
+  QSI_REQUIRE(bits <= qsi::kLowBits && n_low <= qsi::kLowSegs,
+              "qsi_basis_build_lowkeys: more than 32 informative positions lie below the "
+              "bucket key (a wider key is needed)");
+  uint32_t* keys = nullptr;
+  hipError_t e = hipMalloc(&keys, static_cast<size_t>(b->dim) * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_basis_build_lowkeys: hipMalloc: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? QSI_E_NOMEM : QSI_E_HIP;
+  }
+  b->n_low = n_low;
+  for (int q = 0; q < qsi::kLowSegs; ++q) {
+    b->low_lo[q] = lo[q];
+    b->low_w[q] = w[q];
+  }
+  const qsi::BasisView v = b->view();
+  const uint32_t grid = static_cast<uint32_t>((b->dim + 255) / 256);
+  if (b->nw == 1)
+    hipLaunchKernelGGL(qsi::k_build_lowkeys<1>, dim3(grid), dim3(256), 0, 0, v, keys);
+  else
+    hipLaunchKernelGGL(qsi::k_build_lowkeys<2>, dim3(grid), dim3(256), 0, 0, v, keys);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(keys);
+    set_error(std::string("qsi_basis_build_lowkeys: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  b->d_lowkeys = keys;
+  return QSI_OK;
+}
+
+int qsi_basis_has_lowkeys(qsi_basis_t b) { return b && b->d_lowkeys ? 1 : 0; }
 
 }  // extern "C"
 
@@ -2086,10 +2285,12 @@ int64_t qsi_op_dim(qsi_op_t op) { return op ? op->basis->dim : -1; }
 
 int qsi_op_set_variant(qsi_op_t op, int variant) {
   QSI_REQUIRE(op != nullptr, "qsi_op_set_variant: null operator");
-  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_SLOT3,
+  QSI_REQUIRE(variant >= QSI_APPLY_AUTO && variant <= QSI_APPLY_LOW8,
               "qsi_op_set_variant: unknown variant");
-  QSI_REQUIRE(op->basis->d_slots || variant < QSI_APPLY_SLOT2,
+  QSI_REQUIRE(op->basis->d_slots || variant < QSI_APPLY_SLOT2 || variant > QSI_APPLY_SLOT3,
               "qsi_op_set_variant: the basis has no slot index (qsi_basis_build_slots)");
+  QSI_REQUIRE(op->basis->d_lowkeys || variant < QSI_APPLY_LOW2,
+              "qsi_op_set_variant: the basis has no low keys (qsi_basis_build_lowkeys)");
   QSI_REQUIRE(variant != QSI_APPLY_CACHED || op->conn_nnz >= 0,
               "qsi_op_set_variant: no cached connectivity (qsi_op_cache_connectivity)");
   QSI_REQUIRE(op->basis->d_hash || (variant != QSI_APPLY_HASH && variant != QSI_APPLY_HASH4),

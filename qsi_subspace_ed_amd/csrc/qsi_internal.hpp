@@ -64,6 +64,11 @@ struct qsi_basis_s {
   uint32_t* d_table = nullptr;   // 2^kbits + 1
   ulonglong2* d_hash = nullptr;  // hash index (null if not built)
   uint32_t* d_slots = nullptr;   // slot index, 2 * dim entries (null if not built)
+  uint32_t* d_lowkeys = nullptr; // low keys, dim entries (null if not built)
+  int n_low = 0;                 // low-key runs (see BasisView)
+  uint8_t low_lo[qsi::kLowSegs] = {};
+  uint8_t low_w[qsi::kLowSegs] = {};
+  uint64_t infmask[2] = {0, 0};  // informative positions (bit p: spin p)
   int hbits = 0;
   std::atomic<int> refs{1};
 
@@ -80,6 +85,7 @@ struct qsi_basis_s {
     v.states = d_states;
     v.table = d_table;
     v.slots = d_slots;
+    v.lowkeys = d_lowkeys;
     v.dim = dim;
     v.kbits = kbits;
     v.n_spins = n_spins;
@@ -87,6 +93,11 @@ struct qsi_basis_s {
     for (int q = 0; q < qsi::kKeySegs; ++q) {
       v.seg_lo[q] = seg_lo[q];
       v.seg_w[q] = seg_w[q];
+    }
+    v.n_low = n_low;
+    for (int q = 0; q < qsi::kLowSegs; ++q) {
+      v.low_lo[q] = low_lo[q];
+      v.low_w[q] = low_w[q];
     }
     return v;
   }

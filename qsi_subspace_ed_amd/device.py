@@ -104,6 +104,15 @@ class GpuBasis:
     def has_slots(self):
         return bool(_lib.load().qsi_basis_has_slots(self._h))
 
+    def build_lowkeys(self, enable=True):
+        """Build (or drop) the low keys (qsi_basis_build_lowkeys): 4 bytes
+        per state that the QSI_APPLY_LOW* kernels bisect a bucket on."""
+        check(_lib.load().qsi_basis_build_lowkeys(self._h, int(bool(enable))),
+              "qsi_basis_build_lowkeys")
+
+    def has_lowkeys(self):
+        return bool(_lib.load().qsi_basis_has_lowkeys(self._h))
+
     def key_fields(self):
         """Bucket key fields [(lo, width), ...], highest first."""
         n = C.c_int(0)

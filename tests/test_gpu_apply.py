@@ -11,6 +11,7 @@ import pytest
 from conftest import load_basis
 from oracle import pyoracle as orc
 import qsi_subspace_ed_amd as q
+from qsi_subspace_ed_amd._lib import QsiError
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +64,8 @@ def test_search_ranks(W, kbits, cuda):
     assert not basis.has_slots()
 
 
-@pytest.mark.parametrize("variant", ["auto", "window", "lane", "lane8", "hash", "slot4", "cached"])
+@pytest.mark.parametrize("variant", ["auto", "window", "lane", "lane8", "hash", "slot4", "low4",
+                                     "cached"])
 @pytest.mark.parametrize("W,kind", [(8, "ring"), (32, "ring"), (32, "ring+field"),
                                     (48, "ring+field"), (48, "jpm")])
 def test_apply_parity(W, kind, variant, lattices, cuda):
@@ -75,6 +77,8 @@ def test_apply_parity(W, kind, variant, lattices, cuda):
     gb = q.GpuBasis(b, W, hash_index=True)
     if variant == "slot4":
         gb.build_slots()
+    if variant == "low4":
+        gb.build_lowkeys()
     op = q.GpuOp(gb, H)
     if variant == "cached":
         op.cache_connectivity(True)
@@ -132,11 +136,12 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
     H = _ham(jd, W, "jpm" if W == 48 else "ring+field")
     basis = q.GpuBasis(b, W, bucket_bits=kbits, hash_index=True)
     basis.build_slots()
+    basis.build_lowkeys()
     op = q.GpuOp(basis, H)
     x = torch.from_numpy(_rand_x(b.shape[0], 9)).cuda()
     ys = []
     for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
-              "slot3", "cached"):
+              "slot3", "low2", "low4", "low8", "cached"):
         if v == "cached":
             op.cache_connectivity(True)
         op.set_variant(v)
@@ -172,9 +177,10 @@ def test_apply_partial_basis(lattices, cuda):
     b = load_basis(32)[::3].copy()
     H = _ham(jd, 32, "ring+field")
     for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
-              "slot3"):
+              "slot3", "low2", "low4", "low8"):
         basis = q.GpuBasis(b, 32, hash_index=True)
         basis.build_slots()
+        basis.build_lowkeys()
         op = q.GpuOp(basis, H)
         op.set_variant(v)
         x = _rand_x(b.shape[0], 4, tiny=False)
@@ -225,11 +231,12 @@ def test_apply_parity_two_words(lattices, cuda):
     H.add_term(0.021, q.SymbolicPMROperator.from_ops("zz", [f(3), f(40)]))
     gb = q.GpuBasis(lohi, W2, hash_index=True)
     gb.build_slots()
+    gb.build_lowkeys()
     op = q.GpuOp(gb, H)
     x = _rand_x(len(states), 5)
     ref = orc.evaluate_add(orc.Basis(lohi), H, x, np.zeros(len(states)))
     for v in ("window", "lane", "lane2", "lane4", "lane8", "hash", "hash4", "slot2", "slot4",
-              "slot3"):
+              "slot3", "low2", "low4", "low8"):
         op.set_variant(v)
         yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
         op.evaluate(torch.from_numpy(x).cuda(), yd)
@@ -244,6 +251,15 @@ def test_apply_parity_two_words(lattices, cuda):
     yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
     op.evaluate(torch.from_numpy(x).cuda(), yd)
     assert np.linalg.norm(yd.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref)
+    # and a low-key variant once the low keys are dropped
+    op.set_variant("low8")
+    gb.build_lowkeys(False)
+    assert not gb.has_lowkeys()
+    yd = torch.zeros(len(states), dtype=torch.float64, device="cuda")
+    op.evaluate(torch.from_numpy(x).cuda(), yd)
+    assert np.linalg.norm(yd.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref)
+    with pytest.raises(QsiError, match="no low keys"):
+        op.set_variant("low4")
 
 
 @pytest.mark.parametrize("R", [1, 3, 16, 64])

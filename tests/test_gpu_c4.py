@@ -41,6 +41,7 @@ def c4(cuda):
     H = q.build_hamiltonian_g(jd, [-v for v in g])
     q.add_local_field(H, jd, B)
     basis.build_slots()
+    basis.build_lowkeys()
     op = q.GpuOp(basis, H)
     states = basis.states()
     yield {"jd": jd, "basis": basis, "op": op, "H": H, "states": states}
@@ -97,7 +98,7 @@ def test_c4_apply_parity(c4, cuda):
     assert nz > N_ROWS  # the pushes land far outside the source rows
     yd = torch.empty(D, dtype=torch.float64, device="cuda")
     prev = None
-    for v in ("slot4", "lane4"):  # slot index and bucket bisection
+    for v in ("slot4", "lane4", "low4"):  # slot index, bisection on states, on low keys
         c4["op"].set_variant(v)
         yd.fill_(3.0)
         c4["op"].evaluate(xd, yd)
