@@ -116,17 +116,40 @@ def cpu_eigs_reference(g, B, D):
 
 
 def host_cpu():
-    """The host's CPU model and logical core count (for the CPU baseline)."""
-    model = "unknown"
+    """The host's CPU (model, sockets, cores, threads, clock), logical core
+    count and memory, for the CPU baseline: lscpu when present, else
+    /proc/cpuinfo; MemTotal from /proc/meminfo."""
+    info = {"cpu_model": "unknown", "nproc": os.cpu_count()}
+    import subprocess
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "cpu_model", "Socket(s)": "sockets",
+                "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core",
+                "CPU max MHz": "cpu_max_mhz", "L3 cache": "l3_cache"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and v.strip():
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count()}
+    if info["cpu_model"] == "unknown":
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["cpu_model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    info["host_mem_gib"] = round(int(line.split()[1]) / 2**20, 1)
+                    break
+    except (OSError, ValueError):
+        pass
+    return info
 
 
 def cpu_baseline(states, H, x, budget_s=12.0, threads=None):

@@ -986,6 +986,16 @@ inline int vec_blocks(int64_t n) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(nb, 8192)));
 }
 
+// Workgroups of the projection passes (multi-dots over up to 64 columns):
+// about 16 rows per thread, but at least 1024 workgroups while there are 4
+// rows per thread to give them, so each workgroup's column sums (one block
+// reduction per column) are spread over more rows than in a plain vector
+// kernel and the chip still fills.
+inline int proj_blocks(int64_t n) {
+  const int64_t want = (n + kVecBlock * 16 - 1) / (kVecBlock * 16);
+  return static_cast<int>(std::min<int64_t>(vec_blocks(n), std::max<int64_t>(want, 1024)));
+}
+
 // The grid-stride dot, axpy-norm and device-scalar scale kernels take
 // kVecUnroll of a thread's elements per round, all loads first, and
 // accumulate in element order (the sums are the ones a one-element loop
@@ -1099,10 +1109,15 @@ __global__ __launch_bounds__(kVecBlock) void k_multidot(
     for (int j = 0; j < 16; ++j) acc[j] = 0.0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
          i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+      // every load of the row first, then unpredicated updates (absent
+      // columns load as 0 and leave their unused sums unchanged), so a lane
+      // keeps all its column loads in flight
       const double wi = w[i];
+      double v[16];
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (j < jn) acc[j] += V[(j0 + j) * ldv + i] * wi;
+      for (int j = 0; j < 16; ++j) v[j] = j < jn ? V[(j0 + j) * ldv + i] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] += v[j] * wi;
     }
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -1159,21 +1174,24 @@ __global__ __launch_bounds__(kVecBlock) void k_gemv_sub_dots(
   __shared__ double hs[MAXC];
   __shared__ double red[kVecBlock / kWave][MAXC + 1];
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  if (threadIdx.x < MAXC) hs[threadIdx.x] = threadIdx.x < m ? h[threadIdx.x] : 0.0;
   __syncthreads();
   double acc[MAXC + 1];
 #pragma unroll
   for (int j = 0; j <= MAXC; ++j) acc[j] = 0.0;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
        i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+    // every load of the row first (absent columns load as 0, and their h is
+    // 0, so the sum and the dots are unchanged), then the sums in column
+    // order, so a lane keeps all its column loads in flight
     double vr[MAXC];
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) vr[j] = j < m ? V[j * ldv + i] : 0.0;
+    const double wo = w[i];
     double sum = 0.0;
 #pragma unroll
-    for (int j = 0; j < MAXC; ++j) {
-      vr[j] = j < m ? V[j * ldv + i] : 0.0;
-      if (j < m) sum += vr[j] * hs[j];
-    }
-    const double wn = w[i] - sum;
+    for (int j = 0; j < MAXC; ++j) sum += vr[j] * hs[j];
+    const double wn = wo - sum;
     w[i] = wn;
 #pragma unroll
     for (int j = 0; j < MAXC; ++j) acc[j] += vr[j] * wn;
@@ -1584,7 +1602,7 @@ int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
   }
   // one partial per (workgroup, column): the caller's buffer holds
   // multidot_partials() doubles
-  const int nb = std::max(1, std::min(vec_blocks(n), multidot_partials() / std::max(m, 1)));
+  const int nb = std::max(1, std::min(proj_blocks(n), multidot_partials() / std::max(m, 1)));
   hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, w,
                      n, partials);
   QSI_CHECK_LAUNCH();
@@ -1610,7 +1628,7 @@ int launch_gs_reorth(const double* V, int64_t ldv, int m, const double* h, doubl
     set_error("launch_gs_reorth: column count out of range");
     return QSI_E_INVALID;
   }
-  const int nb = std::max(1, std::min(vec_blocks(n), multidot_partials() / (m + 1)));
+  const int nb = std::max(1, std::min(proj_blocks(n), multidot_partials() / (m + 1)));
   if (m <= 16)
     hipLaunchKernelGGL(k_gemv_sub_dots<16>, dim3(nb), dim3(kVecBlock), 0, s, V, ldv, m, h, w, n,
                        partials);

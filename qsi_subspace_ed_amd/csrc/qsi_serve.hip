@@ -337,7 +337,9 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
   if (!flag[c]) return;
   const int m = colj[c] + 1;
   __shared__ double hs[MMAX];
-  if (threadIdx.x < m) hs[threadIdx.x] = h[c * kHMax + threadIdx.x] * vs[c * kHMax + threadIdx.x];
+  if (threadIdx.x < MMAX)
+    hs[threadIdx.x] =
+        threadIdx.x < m ? h[c * kHMax + threadIdx.x] * vs[c * kHMax + threadIdx.x] : 0.0;
   __syncthreads();
   const double* V = Vb + static_cast<int64_t>(c) * ncv1 * ld;
   double* u = Vb + (static_cast<int64_t>(c) * ncv1 + m) * ld;
@@ -346,9 +348,19 @@ __global__ __launch_bounds__(kSrvThreads) void k_srv_gemv_norm(
   double nrm = 0.0;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kSrvThreads) {
     double a = 0.0;
+    if constexpr (MMAX <= 32) {
+      // all loads first, then the sum in column order (absent columns load
+      // as 0 against a zero h), so a lane keeps its column loads in flight
+      double v[MMAX];
 #pragma unroll
-    for (int i = 0; i < MMAX; ++i)
-      if (i < m) a += V[i * ld + r] * hs[i];
+      for (int i = 0; i < MMAX; ++i) v[i] = i < m ? V[i * ld + r] : 0.0;
+#pragma unroll
+      for (int i = 0; i < MMAX; ++i) a += v[i] * hs[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < MMAX; ++i)
+        if (i < m) a += V[i * ld + r] * hs[i];
+    }
     const double ur = u[r] - a;
     u[r] = ur;
     nrm += ur * ur;
