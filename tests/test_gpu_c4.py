@@ -10,9 +10,11 @@ benchmark does).
   source rows [0, 2^24) of the full basis; the device apply of x zeroed
   outside those rows must equal it over the whole output vector, to 1e-13
   relative 2-norm (the push over a set of sources is the pull of the masked
-  vector, targets anywhere in the basis).
+  vector, targets anywhere in the basis);
+* the first Lanczos coefficients (the apply's dot product summed in two
+  stages over 7.8 M workgroup partials) against torch's dot and norm.
 
-The three tests share one enumeration (module fixture, ~1.5 min).
+The tests share one enumeration (module fixture).
 """
 import numpy as np
 import pytest
@@ -109,3 +111,36 @@ def test_c4_apply_parity(c4, cuda):
         if prev is not None:
             assert torch.equal(prev, yd)
         prev = yd.clone()
+
+
+def test_c4_lanczos_two_stage_sums(c4, cuda):
+    """The first Lanczos coefficients at 96 spins, where the apply's dot
+    product leaves 7.8 M per-workgroup partials (summed in two fixed-order
+    stages), against the same recurrence written with torch's dot and norm:
+    alpha_0 = v0.Hv0, beta_0 = |Hv0 - alpha_0 v0|, alpha_1 = v1.(Hv1 - beta_0 v0),
+    to 1e-11 of |alpha_0| + beta_0 (two different summation orders over 2e9
+    terms)."""
+    torch = cuda
+    op = c4["op"]
+    op.set_variant("lane4")
+    D = c4["basis"].dim()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    v0 = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
+    al, be = op.lanczos_coeffs(2, v0=v0)
+    v0 /= torch.linalg.norm(v0)
+    y = torch.empty_like(v0)
+    op.evaluate(v0, y)
+    a0 = float(torch.dot(v0, y))
+    y.add_(v0, alpha=-a0)
+    b0 = float(torch.linalg.norm(y))
+    y /= b0  # v1
+    w = torch.empty_like(v0)
+    op.evaluate(y, w)
+    w.add_(v0, alpha=-b0)
+    a1 = float(torch.dot(y, w))
+    del w, y, v0
+    torch.cuda.empty_cache()
+    scale = abs(a0) + b0
+    assert abs(al[0] - a0) <= 1e-11 * scale, (al[0], a0)
+    assert abs(be[0] - b0) <= 1e-11 * scale, (be[0], b0)
+    assert abs(al[1] - a1) <= 1e-11 * scale, (al[1], a1)
