@@ -662,8 +662,6 @@ def bench_64(args):
                        hash_index=args.variant.startswith("hash"))
     if not args.no_slots:
         basis.build_slots()
-    if not args.no_lowkeys:
-        basis.build_lowkeys()
     op = q.GpuOp(basis, H)
     tuned = None
     if args.variant == "auto":
@@ -860,7 +858,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="96spin: skip the 64-spin, FTLM and served sections")
     ap.add_argument("--no-lowkeys", action="store_true",
-                    help="do not build the low keys (no QSI_APPLY_LOW* variants)")
+                    help="96 spins: do not build the low keys (no QSI_APPLY_LOW* variants)")
     ap.add_argument("--no-slots", action="store_true",
                     help="do not build the slot index (search by bucket bisection only)")
     args = ap.parse_args()
