@@ -1075,22 +1075,6 @@ __global__ __launch_bounds__(kVecBlock) void k_axpy_dev_norm(
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
-// x *= s[0] (s on the device)
-__global__ __launch_bounds__(kVecBlock) void k_scal_dev(
-    const double* __restrict__ s, double* __restrict__ x, int64_t n) {
-  const double a = s[0];
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBlock;
-  int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
-  for (; i + (kVecUnroll - 1) * stride < n; i += kVecUnroll * stride) {
-    double xv[kVecUnroll];
-#pragma unroll
-    for (int k = 0; k < kVecUnroll; ++k) xv[k] = x[i + k * stride];
-#pragma unroll
-    for (int k = 0; k < kVecUnroll; ++k) x[i + k * stride] = xv[k] * a;
-  }
-  for (; i < n; i += stride) x[i] *= a;
-}
-
 // ---- tall-skinny kernels for full reorthogonalisation ----------------------
 // V is column-major with leading dimension ldv (column j at V + j*ldv).
 
@@ -1217,14 +1201,12 @@ __global__ __launch_bounds__(kVecBlock) void k_gemv_sub_dots(
 // beta[1] = 1 / |w| (0 for a vanishing w).
 __global__ void k_gs_tail(const double* __restrict__ g, int m, double* __restrict__ hcol,
                           int* __restrict__ flag, double* __restrict__ beta) {
-  __shared__ double g2s;
   __shared__ int need;
   if (threadIdx.x == 0) {
     double g2 = 0.0;
     for (int j = 0; j < m; ++j) g2 += g[j] * g[j];
     const double q = g[m];
     need = (q > 0.0 && g2 > 1e-24 * q) ? 1 : 0;
-    g2s = g2;
     const double b2 = need ? q - g2 : q;
     const double b = b2 > 0.0 ? sqrt(b2) : 0.0;
     beta[0] = b;
@@ -1236,19 +1218,35 @@ __global__ void k_gs_tail(const double* __restrict__ g, int m, double* __restric
     for (int j = threadIdx.x; j < m; j += blockDim.x) hcol[j] += g[j];
 }
 
-// w -= V[:, 0:m] h when flag[0] is set (third Gram-Schmidt pass)
-__global__ __launch_bounds__(kVecBlock) void k_gemv_sub_if(
+// The conditional third pass fused with the normalisation: w -= V[:, 0:m] h
+// when flag[0] is set (a plain column-order sum), then w *= a = beta[1], the
+// scale k_gs_tail computed from the pass-2 sums, in one pass over w.
+__global__ __launch_bounds__(kVecBlock) void k_gemv_sub_scal(
     const double* __restrict__ V, int64_t ldv, int m, const double* __restrict__ h,
-    double* __restrict__ w, int64_t n, const int* __restrict__ flag) {
-  if (flag[0] == 0) return;
+    double* __restrict__ w, int64_t n, const int* __restrict__ flag,
+    const double* __restrict__ beta) {
+  const bool need = flag[0] != 0;
+  const double a = beta[1];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBlock;
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
+  if (!need) {
+    for (; i + (kVecUnroll - 1) * stride < n; i += kVecUnroll * stride) {
+      double xv[kVecUnroll];
+#pragma unroll
+      for (int k = 0; k < kVecUnroll; ++k) xv[k] = w[i + k * stride];
+#pragma unroll
+      for (int k = 0; k < kVecUnroll; ++k) w[i + k * stride] = xv[k] * a;
+    }
+    for (; i < n; i += stride) w[i] *= a;
+    return;
+  }
   __shared__ double hs[kMaxCols];
   if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
   __syncthreads();
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kVecBlock + threadIdx.x;
-       i < n; i += static_cast<int64_t>(gridDim.x) * kVecBlock) {
+  for (; i < n; i += stride) {
     double acc = 0.0;
     for (int j = 0; j < m; ++j) acc += V[j * ldv + i] * hs[j];
-    w[i] -= acc;
+    w[i] = (w[i] - acc) * a;
   }
 }
 
@@ -1582,13 +1580,6 @@ int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
   return QSI_OK;
 }
 
-int launch_scal_dev(const double* s_dev, double* x, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_scal_dev, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s,
-                     s_dev, x, n);
-  QSI_CHECK_LAUNCH();
-  return QSI_OK;
-}
-
 int max_vec_partials() { return 8192; }
 int multidot_partials() { return 1 << 17; }
 
@@ -1644,8 +1635,8 @@ int launch_gs_reorth(const double* V, int64_t ldv, int m, const double* h, doubl
   QSI_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_gs_tail, dim3(1), dim3(64), 0, s, g, m, hcol, flag, beta);
   QSI_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gemv_sub_if, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, V, ldv, m, g, w,
-                     n, flag);
+  hipLaunchKernelGGL(k_gemv_sub_scal, dim3(vec_blocks(n)), dim3(kVecBlock), 0, s, V, ldv, m, g,
+                     w, n, flag, beta);
   QSI_CHECK_LAUNCH();
   return QSI_OK;
 }

@@ -174,7 +174,6 @@ int launch_scal(double alpha, double* x, int64_t n, hipStream_t s);
 int launch_axpy_norm(const double* alpha_dev, const double* v, double* u,
                      int64_t n, double* partials, double* nrm_out,
                      hipStream_t s, const double* vs_dev = nullptr);
-int launch_scal_dev(const double* s_dev, double* x, int64_t n, hipStream_t s);
 // out[j] = V[:, j] . w for j < m (V column-major, ld ldv); acc_out += out
 int launch_multidot(const double* V, int64_t ldv, int m, const double* w,
                     int64_t n, double* partials, double* out, double* acc_out,
@@ -187,7 +186,8 @@ int launch_gemv_sub(const double* V, int64_t ldv, int m, const double* h,
 // V^T w_new and g[m] = |w_new|^2 from the same read (g: m + 1 doubles); if
 // |g[0..m)| > 1e-12 |w_new| then w -= V g, g is added to hcol[0..m) and
 // flag[0] = 1.  beta[0] = |w| at the end, beta[1] = 1 / beta[0] (0 if w
-// vanishes).  All on the device (no host synchronisation).
+// vanishes), and w is scaled by beta[1] in the same final pass.  All on the
+// device (no host synchronisation).
 int launch_gs_reorth(const double* V, int64_t ldv, int m, const double* h, double* w,
                      int64_t n, double* partials, double* g, double* hcol, int* flag,
                      double* beta, hipStream_t s);

@@ -548,10 +548,10 @@ int qsi_eigsh(qsi_op_t op, int nev, int ncv, double tol, int max_restarts,
       ++applies;
       // pass 1: h = V[:, 0..j]^T w (and w . w: column j + 1 is w itself)
       QSI_TRY(launch_multidot(V, ld, j + 2, w, D, bpart.d(), hcol, nullptr, s));
-      // pass 2 fused with the norm and the test for pass 3
+      // pass 2 fused with the norm and the test for pass 3; the conditional
+      // third pass and the normalisation share one final pass
       QSI_TRY(launch_gs_reorth(V, ld, j + 1, hcol, w, D, bpart.d(), bg.d(), hcol,
                                static_cast<int*>(bflag.p), bbeta.d() + 2 * j, s));
-      QSI_TRY(launch_scal_dev(bbeta.d() + 2 * j + 1, w, D, s));
     }
     QSI_CHECK_HIP(hipMemcpyAsync(hT.data() + static_cast<size_t>(k) * colw,
                                  bT.d() + static_cast<size_t>(k) * colw,
