@@ -76,3 +76,38 @@ def test_inputs_below_apply_tolerance(variant, lattices, cuda):
     y = y.cpu().numpy()
     ref = orc.evaluate_add(orc.Basis(b), H, x, np.zeros(D))
     assert np.all(np.abs(y - ref) <= 1e-12 * np.abs(ref) + 1e-15 * np.abs(x) + 1e-28)
+
+
+def test_lowkeys_limits(cuda):
+    """Low keys hold at most 32 informative positions below the bucket key.
+    States 2^k: neighbours first differ at bit k + 1, so every bit from 1 up
+    is informative.  With a 1-bit key, 2^0..2^33 leave 32 positions below it
+    (accepted: the LOW kernels give the lane kernel's bits for S+_i S-_j
+    flips, each target 2^j -> 2^i a basis state); 2^0..2^63 leave 62
+    (refused with an error, the basis unchanged)."""
+    torch = cuda
+    H = q.SymbolicOpSum()
+    for i, j in ((33, 0), (5, 17), (20, 21), (0, 33), (31, 2)):
+        H.add_term(0.5 + 0.01 * i, q.SymbolicPMROperator.from_ops("+-", [i, j]))
+    st = np.array([[1 << k, 0] for k in range(34)], dtype=np.uint64)
+    basis = q.GpuBasis(st, 34, bucket_bits=1)
+    basis.build_lowkeys()
+    assert basis.has_lowkeys()
+    op = q.GpuOp(basis, H)
+    x = torch.from_numpy(np.random.default_rng(3).standard_normal(34)).cuda()
+    ys = []
+    for v in ("lane4", "low2", "low4", "low8"):
+        op.set_variant(v)
+        y = torch.zeros(34, dtype=torch.float64, device="cuda")
+        op.evaluate(x, y)
+        ys.append(y)
+    ref = orc.evaluate_add(orc.Basis(st), H, x.cpu().numpy(), np.zeros(34))
+    assert np.linalg.norm(ys[0].cpu().numpy() - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert np.count_nonzero(ref) == 5
+    for y in ys[1:]:
+        assert torch.equal(ys[0], y)
+    wide = np.array([[1 << k, 0] for k in range(64)], dtype=np.uint64)
+    b64 = q.GpuBasis(wide, 64, bucket_bits=1)
+    with pytest.raises(QsiError, match="more than 32 informative"):
+        b64.build_lowkeys()
+    assert not b64.has_lowkeys()
