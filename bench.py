@@ -33,20 +33,39 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_64spin_pmc.json")
 PMC_SUMMARY_96 = os.path.join(ROOT, "profiles", "apply_96spin_pmc.json")
-PMC_SUMMARY_96_CACHED = os.path.join(ROOT, "profiles", "apply_96spin_cached_pmc.json")
-PMC_SUMMARY_CACHED = os.path.join(ROOT, "profiles", "apply_64spin_cached_pmc.json")
+
+# the apply kernel each variant name launches (single vector, no Z strings in
+# the off-diagonal terms, no dot product), as rocprofv3 names it
+_VARIANT_KERNEL = {"lane": ("lane", 1), "lane2": ("lane", 2), "lane4": ("lane", 4),
+                   "lane8": ("lane", 8), "low2": ("low", 2), "low4": ("low", 4),
+                   "low8": ("low", 8), "slot2": ("slot", 2), "slot3": ("slot", 3),
+                   "slot4": ("slot", 4), "window": ("win", 4), "hash": ("hash", 1),
+                   "hash4": ("hash", 4)}
 
 
-def measured_traffic(path=PMC_SUMMARY):
-    """Bytes per apply that cross the L2/fabric interface (to MALL or HBM),
-    from the committed rocprofv3 counter summary of the apply kernel: read
-    requests counted by size, 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B,
-    plus writes, 64*WRREQ_64B + 32*(WRREQ - WRREQ_64B); None if absent."""
+def variant_kernel(variant, nw, cached_pipe=None):
+    """Kernel-name prefix of an apply variant (cached: by gather depth)."""
+    if variant == "cached":
+        return f"k_apply_cached<{nw}, false, {cached_pipe},"
+    kind, g = _VARIANT_KERNEL[variant]
+    return f"k_apply_{kind}<{nw}, false, false, {g}>"
+
+
+def measured_traffic(path, kernel):
+    """Bytes per apply that cross the L2/fabric interface (to MALL or HBM)
+    for the apply kernel whose name starts with `kernel`, from the committed
+    rocprofv3 counter summary: read requests counted by size, 32*RDREQ_32B +
+    64*RDREQ_64B + 128*RDREQ_128B, plus writes, 64*WRREQ_64B +
+    32*(WRREQ - WRREQ_64B); None if absent."""
     try:
         with open(path) as f:
-            return float(json.load(f)["traffic_bytes_per_apply"])
+            kernels = json.load(f)["kernels"]
     except (OSError, KeyError, ValueError):
         return None
+    for name, rec in kernels.items():
+        if name.startswith(kernel):
+            return float(rec["traffic_bytes_per_apply"])
+    return None
 
 
 def stream_copy_ceiling(gib=2, reps=10):
@@ -539,7 +558,8 @@ def bench_96(args):
         "hv_per_s": steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(PMC_SUMMARY_96)},
+                     "traffic": measured_traffic(PMC_SUMMARY_96, variant_kernel(
+                         tuned[0] if tuned else args.variant, 2))},
         "apply_kernel_ms": kern_s * 1e3,
         "apply_variant": tuned[0] if tuned else args.variant,
         "autotuned": tuned is not None,
@@ -618,7 +638,8 @@ def bench_96(args):
                   "algorithmic_GBps": bytes_apply / kc / 1e9,
                   "kernel_bytes_per_apply": cached_kernel_bytes(D, nnz, S),
                   "kernel_GBps": cached_kernel_bytes(D, nnz, S) / kc / 1e9,
-                  "traffic": measured_traffic(PMC_SUMMARY_96_CACHED),
+                  "traffic": measured_traffic(PMC_SUMMARY_96,
+                                              variant_kernel("cached", 2, op.cached_pipe())),
                   "bit_identical_to_search": bool(torch.equal(y, yc))}
         print(f"[bench] cached connectivity: {kc * 1e3:.1f} ms/apply (built in {t_build:.1f} s)",
               file=sys.stderr, flush=True)
@@ -662,6 +683,8 @@ def bench_64(args):
                        hash_index=args.variant.startswith("hash"))
     if not args.no_slots:
         basis.build_slots()
+    if not args.no_lowkeys:
+        basis.build_lowkeys()
     op = q.GpuOp(basis, H)
     tuned = None
     if args.variant == "auto":
@@ -762,7 +785,8 @@ def bench_64(args):
                   "algorithmic_GBps": bytes_apply / kc / 1e9,
                   "kernel_bytes_per_apply": cached_kernel_bytes(D, cnnz, S),
                   "kernel_GBps": cached_kernel_bytes(D, cnnz, S) / kc / 1e9,
-                  "traffic": measured_traffic(PMC_SUMMARY_CACHED),
+                  "traffic": measured_traffic(PMC_SUMMARY,
+                                              variant_kernel("cached", 1, op.cached_pipe())),
                   "bit_identical_to_search": bool(torch.equal(y, yc))}
         if not args.no_solvers and args.lanczos_steps > 0:
             n1, n2 = max(1, args.lanczos_steps // 4), args.lanczos_steps
@@ -814,7 +838,8 @@ def bench_64(args):
         "hv_per_s": steps / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic()},
+                     "traffic": measured_traffic(PMC_SUMMARY, variant_kernel(
+                         tuned[0] if tuned else args.variant, 1))},
         "apply_kernel_us": kern_s * 1e6,
         "apply_variant": tuned[0] if tuned else args.variant,
         "autotuned": tuned is not None,
@@ -858,7 +883,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="96spin: skip the 64-spin, FTLM and served sections")
     ap.add_argument("--no-lowkeys", action="store_true",
-                    help="96 spins: do not build the low keys (no QSI_APPLY_LOW* variants)")
+                    help="do not build the low keys (no QSI_APPLY_LOW* variants)")
     ap.add_argument("--no-slots", action="store_true",
                     help="do not build the slot index (search by bucket bisection only)")
     args = ap.parse_args()

@@ -284,111 +284,6 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
   }
 }
 
-// Low-key variant (QSI_APPLY_LOW*): as the lane variant, but the bisection
-// of a bucket probes the 4-byte low keys (qsi_basis_build_lowkeys) instead of
-// whole states — four times as many candidates per cache line, and the state
-// array is touched once per lookup, for the candidate.  The candidate is the
-// last state of the bucket whose low key is <= the target's, the same state
-// the lane variant's bisection ends on, so y is bit-identical.
-template <int NW, bool HAS_Z, bool DOT, int G>
-__global__ __launch_bounds__(kApplyBlock) void k_apply_low(
-    BasisView b, const OffTerm* __restrict__ off, int n_off,
-    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
-    double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials,
-    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
-  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  if (yscale_dev) yscale *= yscale_dev[0];
-  // scales as in k_apply_lane (dot-product instances only)
-  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
-  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
-  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
-  double prod = 0.0;
-  if (i < b.dim) {
-    const State<NW> s = load_state_cached<NW>(b.states, i);
-    const double xi = x[i] * xs;
-    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
-    for (int t0 = 0; t0 < n_off; t0 += 64) {
-      const int tn = min(64, n_off - t0);
-      uint64_t mask = 0;
-      for (int t = 0; t < tn; ++t) {
-        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
-      }
-      while (mask) {
-        int tt[G];
-        bool act[G];
-        State<NW> key[G];
-        uint32_t lo[G], n[G], r[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          act[g] = mask != 0;
-          tt[g] = act[g] ? t0 + __builtin_ctzll(mask) : t0;
-          mask &= mask - 1;
-          key[g] = xor_state<NW>(s, off[tt[g]].X);
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          lo[g] = 0;
-          n[g] = 0;
-          r[g] = 0;
-          if (act[g]) {
-            bucket_bounds(b.table, bucket_key<NW>(b, key[g]), lo[g], n[g]);
-            r[g] = low_key<NW>(b, key[g]);
-          }
-        }
-        // interleaved bisection on the low keys down to one candidate
-        bool more = true;
-        while (more) {
-          more = false;
-#pragma unroll
-          for (int g = 0; g < G; ++g) {
-            if (n[g] > 1) {
-              const uint32_t half = n[g] >> 1;
-              const uint32_t m = b.lowkeys[lo[g] + half];
-              lo[g] = m <= r[g] ? lo[g] + half : lo[g];
-              n[g] -= half;
-              more |= n[g] > 1;
-            }
-          }
-        }
-        double xj[G];
-        bool hit[G];
-        State<NW> f[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          xj[g] = 0.0;
-          if (n[g] == 1) {
-            f[g] = load_state_cached<NW>(b.states, lo[g]);
-            xj[g] = x[lo[g]];
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) hit[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]);
-#pragma unroll
-        for (int g = 0; g < G; ++g) xj[g] *= xs;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          if (hit[g] && fabs(xj[g]) > kApplyTol) {
-            const OffTerm& T = off[tt[g]];
-            double c = T.c * T.sgn;
-            if constexpr (HAS_Z) {
-              if (parity_notz<NW>(s, T.Z)) c = -c;
-            }
-            acc += c * xj[g];
-          }
-        }
-      }
-    }
-    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
-    y[i] = acc;
-    if constexpr (DOT) prod = xi * acc;
-  }
-  if constexpr (DOT) {
-    const double bs = block_sum<kApplyBlock>(prod);
-    if (threadIdx.x == 0) partials[blk] = bs;
-  }
-}
-
 // Flip masks and signed coefficients of the first kLdsTerms off-diagonal
 // terms, staged in LDS by every workgroup: lookups re-form their connected
 // state and take their coefficient from here instead of gathering each
@@ -423,6 +318,119 @@ __device__ __forceinline__ State<NW> term_target(const TermsLds<NW>& tl, bool ld
 #pragma unroll
   for (int w = 0; w < NW; ++w) r.w[w] = s.w[w] ^ (lds ? tl.X[w][t] : off[t].X[w]);
   return r;
+}
+
+// Low-key variant (QSI_APPLY_LOW*): as the lane variant, but the bisection
+// of a bucket probes the 4-byte low keys (qsi_basis_build_lowkeys) instead of
+// whole states — four times as many candidates per cache line, and the state
+// array is touched once per lookup, for the candidate.  The candidate is the
+// last state of the bucket whose low key is <= the target's, the same state
+// the lane variant's bisection ends on, so y is bit-identical.  A lookup
+// keeps only its term, bucket start, bucket size and low key while it
+// bisects (the connected state is re-formed from the row and the term's flip
+// mask, staged in LDS, for the final compare), so eight lookups per lane fit
+// the register budget the lane kernel needs for four.
+template <int NW, bool HAS_Z, bool DOT, int G>
+__global__ __launch_bounds__(kApplyBlock) void k_apply_low(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  // scales as in k_apply_lane (dot-product instances only)
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
+  __shared__ TermsLds<NW> tl;
+  const bool lds = stage_terms<NW>(tl, off, n_off);
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  double prod = 0.0;
+  if (i < b.dim) {
+    const State<NW> s = load_state_cached<NW>(b.states, i);
+    const double xi = x[i] * xs;
+    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
+    for (int t0 = 0; t0 < n_off; t0 += 64) {
+      const int tn = min(64, n_off - t0);
+      uint64_t mask = 0;
+      for (int t = 0; t < tn; ++t) {
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
+      }
+      while (mask) {
+        // per lookup: term offset in the chunk, bucket start, bucket size
+        // (0: nothing to find), the target's low key
+        uint32_t tt[G], lo[G], n[G], r[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const bool act = mask != 0;
+          tt[g] = act ? static_cast<uint32_t>(__builtin_ctzll(mask)) : 0u;
+          mask &= mask - 1;
+          lo[g] = 0;
+          n[g] = 0;
+          r[g] = 0;
+          if (act) {
+            const State<NW> key = term_target<NW>(tl, lds, off, s, t0 + tt[g]);
+            bucket_bounds(b.table, bucket_key<NW>(b, key), lo[g], n[g]);
+            r[g] = low_key<NW>(b, key);
+          }
+        }
+        // interleaved bisection on the low keys down to one candidate
+        bool more = true;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (n[g] > 1) {
+              const uint32_t half = n[g] >> 1;
+              const uint32_t m = b.lowkeys[lo[g] + half];
+              lo[g] = m <= r[g] ? lo[g] + half : lo[g];
+              n[g] -= half;
+              more |= n[g] > 1;
+            }
+          }
+        }
+        // candidates in chunks of up to four: every candidate state and x of
+        // a chunk first, then its compares and sums (in term order)
+        constexpr int CH = G % 4 == 0 ? 4 : G % 3 == 0 ? 3 : G % 2 == 0 ? 2 : 1;
+        static_assert(G % CH == 0, "candidate chunks must tile the lookups");
+#pragma unroll
+        for (int g0 = 0; g0 < G; g0 += CH) {
+          State<NW> f[CH];
+          double xj[CH];
+#pragma unroll
+          for (int u = 0; u < CH; ++u) {
+            xj[u] = 0.0;
+            if (n[g0 + u] == 1) {
+              f[u] = load_state_cached<NW>(b.states, lo[g0 + u]);
+              xj[u] = x[lo[g0 + u]];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < CH; ++u) xj[u] *= xs;
+#pragma unroll
+          for (int u = 0; u < CH; ++u) {
+            if (n[g0 + u] == 1 && fabs(xj[u]) > kApplyTol) {
+              const int t = t0 + static_cast<int>(tt[g0 + u]);
+              if (st_eq<NW>(f[u], term_target<NW>(tl, lds, off, s, t))) {
+                double c = lds ? tl.c[t] : off[t].c * off[t].sgn;
+                if constexpr (HAS_Z) {
+                  if (parity_notz<NW>(s, off[t].Z)) c = -c;
+                }
+                acc += c * xj[u];
+              }
+            }
+          }
+        }
+      }
+    }
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
 }
 
 // Slot variant: as the lane variant, with each connected state found through
