@@ -63,7 +63,7 @@ std::vector<double> random_unit(int64_t n, uint32_t seed) {
 // apply and the axpy read the buffers times their scales, which gives the
 // values a separate normalising pass would have stored, so the coefficients
 // have the same bits as with that pass, one pass fewer per step.
-// On exit u = beta_it v_{it+1} / ... unnormalised (its scale 1/beta_it at
+// On exit u holds beta_it v_{it+1}, unnormalised (its scale 1/beta_it is at
 // ab[3 it + 2]); alpha_it -> ab[3 it], beta_it -> ab[3 it + 1]; the caller
 // swaps v and u.
 inline const double* lanczos_scale(const double* ab, int64_t j) {
