@@ -503,6 +503,12 @@ def bench_96(args):
         print(f"[bench] low keys built in {time.perf_counter() - ts:.1f} s", file=sys.stderr,
               flush=True)
     op = q.GpuOp(basis, H)
+    lowkeys_exact = None
+    if basis.has_lowkeys():
+        ts = time.perf_counter()
+        lowkeys_exact = op.verify_lowkeys()
+        print(f"[bench] low keys verified exact for the operator: {lowkeys_exact} "
+              f"({time.perf_counter() - ts:.1f} s)", file=sys.stderr, flush=True)
     nnz = C.c_int64(0)
     _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
     nnz = nnz.value
@@ -567,6 +573,7 @@ def bench_96(args):
         "bucket_bits": basis.bucket_bits(),
         "slot_index": basis.has_slots(),
         "low_keys": basis.has_lowkeys(),
+        "low_keys_exact": lowkeys_exact,
         "enumeration_s": t_enum,
         "algorithmic_bytes_per_apply": int(bytes_apply),
     }
@@ -686,6 +693,7 @@ def bench_64(args):
     if not args.no_lowkeys:
         basis.build_lowkeys()
     op = q.GpuOp(basis, H)
+    lowkeys_exact = op.verify_lowkeys() if basis.has_lowkeys() else None
     tuned = None
     if args.variant == "auto":
         tuned = op.autotune(0, reps=3)  # fastest searching kernel on this workload
@@ -842,6 +850,7 @@ def bench_64(args):
                          tuned[0] if tuned else args.variant, 1))},
         "apply_kernel_us": kern_s * 1e6,
         "apply_variant": tuned[0] if tuned else args.variant,
+        "low_keys_exact": lowkeys_exact,
         "autotuned": tuned is not None,
         "bucket_bits": basis.bucket_bits(),
         "algorithmic_bytes_per_apply": int(bytes_apply),

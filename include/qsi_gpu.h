@@ -181,6 +181,16 @@ enum {
   QSI_APPLY_LOW8 = 14
 };
 int qsi_op_set_variant(qsi_op_t op, int variant);
+/* Checks, once per operator, whether every connected state its terms reach
+ * from a basis state either is the basis state its low-key lookup ends on or
+ * differs from it in bucket key or low key (*exact = 1), by one pass over the
+ * rows (about the cost of one apply; synchronises).  When it holds, the
+ * QSI_APPLY_LOW* kernels compare the candidate's low key instead of loading
+ * its state (the same candidates, so the same y).  Closed sectors (ring
+ * flips on a full spinon sector) satisfy it; a basis missing some connected
+ * states may not, and then nothing changes.  Needs the basis's low keys.  No
+ * counterpart in the reference. */
+int qsi_op_verify_lowkeys(qsi_op_t op, int *exact);
 int qsi_op_get_variant(qsi_op_t op);
 
 /* y = H x (accumulate = 0) or y += H x (accumulate = 1).

@@ -98,6 +98,7 @@ SIGNATURES = {
     "qsi_basis_build_slots": (C.c_int, [_P, C.c_int]),
     "qsi_basis_has_slots": (C.c_int, [_P]),
     "qsi_basis_build_lowkeys": (C.c_int, [_P, C.c_int]),
+    "qsi_op_verify_lowkeys": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "qsi_basis_has_lowkeys": (C.c_int, [_P]),
     "qsi_basis_destroy": (C.c_int, [_P]),
     "qsi_basis_dim": (_I64, [_P]),

@@ -125,6 +125,35 @@ __global__ void k_build_hash(const uint64_t* __restrict__ states, int64_t dim,
 // kSlotEmpty): linear probing from the state's home, a slot claimed by a
 // compare-and-swap.  A bucket above kSlotMaxBucket states sets *too_big and
 // places nothing.
+// For every row and connecting term of an operator: whether the target's
+// low-key lookup ends on a candidate with the target's bucket key and low key
+// but another state (a target outside the basis that differs from a basis
+// state only at positions fixed in the basis).  *bad |= 1 if any does
+// (qsi_op_verify_lowkeys).
+template <int NW>
+__global__ void k_verify_lowkeys(BasisView b, const OffTerm* __restrict__ off, int n_off,
+                                 unsigned int* __restrict__ bad) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= b.dim) return;
+  const State<NW> s = load_state_cached<NW>(b.states, i);
+  bool clash = false;
+  for (int t = 0; t < n_off; ++t) {
+    if (!term_ok<NW>(off[t], s)) continue;
+    const State<NW> key = xor_state<NW>(s, off[t].X);
+    uint32_t lo, n;
+    bucket_bounds(b.table, bucket_key<NW>(b, key), lo, n);
+    if (n == 0) continue;
+    const uint32_t r = low_key<NW>(b, key);
+    while (n > 1) {
+      const uint32_t half = n >> 1;
+      lo = b.lowkeys[lo + half] <= r ? lo + half : lo;
+      n -= half;
+    }
+    if (b.lowkeys[lo] == r && !st_eq<NW>(load_state_cached<NW>(b.states, lo), key)) clash = true;
+  }
+  if (clash) atomicOr(bad, 1u);
+}
+
 // lowkeys[i] = low key of state i (qsi_basis_build_lowkeys)
 template <int NW>
 __global__ void k_build_lowkeys(BasisView b, uint32_t* __restrict__ lowkeys) {
@@ -330,7 +359,12 @@ __device__ __forceinline__ State<NW> term_target(const TermsLds<NW>& tl, bool ld
 // bisects (the connected state is re-formed from the row and the term's flip
 // mask, staged in LDS, for the final compare), so eight lookups per lane fit
 // the register budget the lane kernel needs for four.
-template <int NW, bool HAS_Z, bool DOT, int G>
+//
+// TRUST (after qsi_op_verify_lowkeys found it exact for this operator): a
+// target whose bucket key and low key match the candidate's is the
+// candidate, so the candidate's low key is compared instead of its state and
+// the state array is not read at all.
+template <int NW, bool HAS_Z, bool DOT, int G, bool TRUST>
 __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
@@ -396,12 +430,14 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
 #pragma unroll
         for (int g0 = 0; g0 < G; g0 += CH) {
           State<NW> f[CH];
+          uint32_t fk[CH];
           double xj[CH];
 #pragma unroll
           for (int u = 0; u < CH; ++u) {
             xj[u] = 0.0;
             if (n[g0 + u] == 1) {
-              f[u] = load_state_cached<NW>(b.states, lo[g0 + u]);
+              if constexpr (TRUST) fk[u] = b.lowkeys[lo[g0 + u]];
+              else f[u] = load_state_cached<NW>(b.states, lo[g0 + u]);
               xj[u] = x[lo[g0 + u]];
             }
           }
@@ -411,7 +447,10 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
           for (int u = 0; u < CH; ++u) {
             if (n[g0 + u] == 1 && fabs(xj[u]) > kApplyTol) {
               const int t = t0 + static_cast<int>(tt[g0 + u]);
-              if (st_eq<NW>(f[u], term_target<NW>(tl, lds, off, s, t))) {
+              bool hit;
+              if constexpr (TRUST) hit = fk[u] == r[g0 + u];
+              else hit = st_eq<NW>(f[u], term_target<NW>(tl, lds, off, s, t));
+              if (hit) {
                 double c = lds ? tl.c[t] : off[t].c * off[t].sgn;
                 if constexpr (HAS_Z) {
                   if (parity_notz<NW>(s, off[t].Z)) c = -c;
@@ -1436,19 +1475,34 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
                          y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LOW2:
-      hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2>), grid, block, 0, s, v,
-                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      if (op->lowkeys_exact)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, true>), grid, block, 0, s, v,
+                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      else
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, false>), grid, block, 0, s, v,
+                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LOW4:
-      hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
-                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      if (op->lowkeys_exact)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, true>), grid, block, 0, s, v,
+                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      else
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, false>), grid, block, 0, s, v,
+                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LOW8:
-      hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8>), grid, block, 0, s, v,
-                         op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                         y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      if (op->lowkeys_exact)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, true>), grid, block, 0, s, v,
+                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      else
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, false>), grid, block, 0, s, v,
+                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
+                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     default:  // QSI_APPLY_LANE4
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
@@ -2313,6 +2367,37 @@ int qsi_op_set_variant(qsi_op_t op, int variant) {
   QSI_REQUIRE(op->basis->d_hash || (variant != QSI_APPLY_HASH && variant != QSI_APPLY_HASH4),
               "qsi_op_set_variant: the basis has no hash index");
   op->variant = variant;
+  return QSI_OK;
+}
+
+int qsi_op_verify_lowkeys(qsi_op_t op, int* exact) {
+  QSI_REQUIRE(op != nullptr, "qsi_op_verify_lowkeys: null operator");
+  QSI_REQUIRE(op->basis->d_lowkeys != nullptr,
+              "qsi_op_verify_lowkeys: the basis has no low keys (qsi_basis_build_lowkeys)");
+  op->lowkeys_exact = false;
+  unsigned int* bad = nullptr;
+  unsigned int h_bad = 0;
+  QSI_CHECK_HIP(hipMalloc(&bad, sizeof(unsigned int)));
+  hipError_t e = hipMemset(bad, 0, sizeof(unsigned int));
+  if (e == hipSuccess && op->basis->dim > 0 && op->n_off > 0) {
+    const qsi::BasisView v = op->basis->view();
+    const uint32_t grid = static_cast<uint32_t>((op->basis->dim + 255) / 256);
+    if (op->basis->nw == 1)
+      hipLaunchKernelGGL(qsi::k_verify_lowkeys<1>, dim3(grid), dim3(256), 0, 0, v, op->d_off,
+                         op->n_off, bad);
+    else
+      hipLaunchKernelGGL(qsi::k_verify_lowkeys<2>, dim3(grid), dim3(256), 0, 0, v, op->d_off,
+                         op->n_off, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost);
+  (void)hipFree(bad);
+  if (e != hipSuccess) {
+    set_error(std::string("qsi_op_verify_lowkeys: ") + hipGetErrorString(e));
+    return QSI_E_HIP;
+  }
+  op->lowkeys_exact = h_bad == 0;
+  if (exact) *exact = op->lowkeys_exact ? 1 : 0;
   return QSI_OK;
 }
 

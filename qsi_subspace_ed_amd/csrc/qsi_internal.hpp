@@ -128,6 +128,10 @@ struct qsi_op_s {
   int conn_pad = 1;       // rows padded to multiples of conn_pad entries (1 or 4)
   int blk_cfg = 1;        // block-apply configuration (qsi_op_autotune)
   int cached_pipe = 8;    // gathers in flight per lane of the cached apply (4-32)
+  // qsi_op_verify_lowkeys found that no connected state outside the basis
+  // shares a bucket key and low key with a basis state (a property of the
+  // terms and the basis, both fixed, so it holds for any build of the keys)
+  bool lowkeys_exact = false;
 };
 
 namespace qsi {

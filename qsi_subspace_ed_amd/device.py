@@ -212,10 +212,19 @@ class GpuOp:
         """Choose the apply kernel: 'auto' ('cached' when the connectivity is
         cached, else 'lane4'), 'cached', 'hash', 'hash4' (hash-index
         lookups; the basis needs hash_index=True), 'lane', 'lane2', 'lane4',
-        'lane8' (bucket-table lookups, 1-8 in flight per lane) or 'window'
-        (LDS-staged bucket windows per wavefront)."""
+        'lane8' (bucket-table lookups, 1-8 in flight per lane), 'window'
+        (LDS-staged bucket windows per wavefront), 'slot2/3/4' (slot index)
+        or 'low2/4/8' (low keys)."""
         v = _lib.VARIANTS[variant]
         check(_lib.load().qsi_op_set_variant(self._h, v), "qsi_op_set_variant")
+
+    def verify_lowkeys(self):
+        """qsi_op_verify_lowkeys: True when the low-key kernels may compare
+        the candidate's low key instead of its state for this operator (one
+        pass over the rows; the basis needs low keys)."""
+        ex = C.c_int(0)
+        check(_lib.load().qsi_op_verify_lowkeys(self._h, C.byref(ex)), "qsi_op_verify_lowkeys")
+        return bool(ex.value)
 
     def set_cached_pipe(self, pipe):
         """Pin the cached apply's gathers in flight per lane (4, 8, 16, 32)."""

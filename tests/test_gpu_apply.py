@@ -157,6 +157,17 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
         ys.append(y)
     for y in ys[1:]:
         assert torch.equal(ys[0], y)
+    # the low-key kernels after the verification pass (ring flips on a full
+    # sector reach only basis states, so it must hold there) give the same bits
+    exact = op.verify_lowkeys()
+    if W == 32:
+        assert exact
+    for v in ("low2", "low4", "low8"):
+        op.set_variant(v)
+        y = torch.zeros_like(x)
+        op.evaluate(x, y)
+        assert torch.equal(ys[0], y), (v, exact)
+    op.set_variant("cached")
     # y += H x and y = 0.5 y + H x through the cached kernel match too
     y0 = torch.from_numpy(_rand_x(b.shape[0], 10)).cuda()
     y1 = y0.clone()
@@ -182,6 +193,8 @@ def test_apply_partial_basis(lattices, cuda):
         basis.build_slots()
         basis.build_lowkeys()
         op = q.GpuOp(basis, H)
+        if v.startswith("low"):
+            op.verify_lowkeys()  # whatever it finds, the result must stay exact
         op.set_variant(v)
         x = _rand_x(b.shape[0], 4, tiny=False)
         y = torch.zeros(b.shape[0], dtype=torch.float64, device="cuda")

@@ -45,8 +45,10 @@ def c4(cuda):
     basis.build_slots()
     basis.build_lowkeys()
     op = q.GpuOp(basis, H)
+    # ring flips on the full Q=0 sector reach only basis states
+    exact = op.verify_lowkeys()
     states = basis.states()
-    yield {"jd": jd, "basis": basis, "op": op, "H": H, "states": states}
+    yield {"jd": jd, "basis": basis, "op": op, "H": H, "states": states, "exact": exact}
     op.close()
     basis.close()
     torch.cuda.empty_cache()
@@ -100,7 +102,9 @@ def test_c4_apply_parity(c4, cuda):
     assert nz > N_ROWS  # the pushes land far outside the source rows
     yd = torch.empty(D, dtype=torch.float64, device="cuda")
     prev = None
-    for v in ("slot4", "lane4", "low4"):  # slot index, bisection on states, on low keys
+    assert c4["exact"]
+    # slot index, bisection on states, on low keys (verified: no state loads)
+    for v in ("slot4", "lane4", "low4"):
         c4["op"].set_variant(v)
         yd.fill_(3.0)
         c4["op"].evaluate(xd, yd)

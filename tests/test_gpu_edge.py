@@ -111,3 +111,27 @@ def test_lowkeys_limits(cuda):
     with pytest.raises(QsiError, match="more than 32 informative"):
         b64.build_lowkeys()
     assert not b64.has_lowkeys()
+
+
+def test_lowkeys_verification_finds_a_clash(cuda):
+    """Basis {00, 11}: only bit 1 is informative, so with a 1-bit key the
+    state 01 (the pre-image of row 11 under S+ on spin 1) has the same key and
+    low key as 00.  The verification pass must report the clash, and the
+    low-key kernels must then still compare states: y = 0, as the oracle's
+    push (which drops the target 10, absent from the basis) gives."""
+    torch = cuda
+    st = np.array([[0, 0], [3, 0]], dtype=np.uint64)
+    H = q.SymbolicOpSum()
+    H.add_term(1.0, q.SymbolicPMROperator.from_ops("+", [1]))
+    basis = q.GpuBasis(st, 2, bucket_bits=1)
+    basis.build_lowkeys()
+    op = q.GpuOp(basis, H)
+    assert op.verify_lowkeys() is False
+    x = np.array([0.5, 0.25])
+    ref = orc.evaluate_add(orc.Basis(st), H, x, np.zeros(2))
+    assert np.all(ref == 0.0)
+    for v in ("low2", "low4", "low8", "lane4"):
+        op.set_variant(v)
+        y = torch.full((2,), 9.0, dtype=torch.float64, device="cuda")
+        op.evaluate(torch.from_numpy(x).cuda(), y)
+        assert np.array_equal(y.cpu().numpy(), ref), v
