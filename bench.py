@@ -43,11 +43,14 @@ _VARIANT_KERNEL = {"lane": ("lane", 1), "lane2": ("lane", 2), "lane4": ("lane", 
                    "hash4": ("hash", 4)}
 
 
-def variant_kernel(variant, nw, cached_pipe=None):
-    """Kernel-name prefix of an apply variant (cached: by gather depth)."""
+def variant_kernel(variant, nw, cached_pipe=None, verified=False):
+    """Kernel-name prefix of an apply variant (cached: by gather depth; low
+    keys: the instance that skips state loads when they were verified)."""
     if variant == "cached":
         return f"k_apply_cached<{nw}, false, {cached_pipe},"
     kind, g = _VARIANT_KERNEL[variant]
+    if kind == "low":
+        return f"k_apply_low<{nw}, false, false, {g}, {'true' if verified else 'false'}>"
     return f"k_apply_{kind}<{nw}, false, false, {g}>"
 
 
@@ -565,7 +568,8 @@ def bench_96(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PMC_SUMMARY_96, variant_kernel(
-                         tuned[0] if tuned else args.variant, 2))},
+                         tuned[0] if tuned else args.variant, 2,
+                         verified=bool(lowkeys_exact)))},
         "apply_kernel_ms": kern_s * 1e3,
         "apply_variant": tuned[0] if tuned else args.variant,
         "autotuned": tuned is not None,
@@ -847,7 +851,8 @@ def bench_64(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PMC_SUMMARY, variant_kernel(
-                         tuned[0] if tuned else args.variant, 1))},
+                         tuned[0] if tuned else args.variant, 1,
+                         verified=bool(lowkeys_exact)))},
         "apply_kernel_us": kern_s * 1e6,
         "apply_variant": tuned[0] if tuned else args.variant,
         "low_keys_exact": lowkeys_exact,
