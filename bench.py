@@ -43,14 +43,14 @@ _VARIANT_KERNEL = {"lane": ("lane", 1), "lane2": ("lane", 2), "lane4": ("lane", 
                    "hash4": ("hash", 4)}
 
 
-def variant_kernel(variant, nw, cached_pipe=None, verified=False):
+def variant_kernel(variant, nw, cached_pipe=None, low_mode=0):
     """Kernel-name prefix of an apply variant (cached: by gather depth; low
-    keys: the instance that skips state loads when they were verified)."""
+    keys: by how candidates are confirmed, qsi_op_verify_lowkeys' mode)."""
     if variant == "cached":
         return f"k_apply_cached<{nw}, false, {cached_pipe},"
     kind, g = _VARIANT_KERNEL[variant]
     if kind == "low":
-        return f"k_apply_low<{nw}, false, false, {g}, {'true' if verified else 'false'}>"
+        return f"k_apply_low<{nw}, false, false, {g}, {int(low_mode or 0)}>"
     return f"k_apply_{kind}<{nw}, false, false, {g}>"
 
 
@@ -506,11 +506,11 @@ def bench_96(args):
         print(f"[bench] low keys built in {time.perf_counter() - ts:.1f} s", file=sys.stderr,
               flush=True)
     op = q.GpuOp(basis, H)
-    lowkeys_exact = None
+    lowkeys_mode = None
     if basis.has_lowkeys():
         ts = time.perf_counter()
-        lowkeys_exact = op.verify_lowkeys()
-        print(f"[bench] low keys verified exact for the operator: {lowkeys_exact} "
+        lowkeys_mode = op.verify_lowkeys()
+        print(f"[bench] low-key check mode for the operator: {lowkeys_mode} "
               f"({time.perf_counter() - ts:.1f} s)", file=sys.stderr, flush=True)
     nnz = C.c_int64(0)
     _lib.check(_lib.load().qsi_op_connectivity(op._h, C.byref(nnz), None, None, None, None))
@@ -569,7 +569,7 @@ def bench_96(args):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PMC_SUMMARY_96, variant_kernel(
                          tuned[0] if tuned else args.variant, 2,
-                         verified=bool(lowkeys_exact)))},
+                         low_mode=lowkeys_mode))},
         "apply_kernel_ms": kern_s * 1e3,
         "apply_variant": tuned[0] if tuned else args.variant,
         "autotuned": tuned is not None,
@@ -577,7 +577,7 @@ def bench_96(args):
         "bucket_bits": basis.bucket_bits(),
         "slot_index": basis.has_slots(),
         "low_keys": basis.has_lowkeys(),
-        "low_keys_exact": lowkeys_exact,
+        "low_keys_mode": lowkeys_mode,
         "enumeration_s": t_enum,
         "algorithmic_bytes_per_apply": int(bytes_apply),
     }
@@ -697,7 +697,7 @@ def bench_64(args):
     if not args.no_lowkeys:
         basis.build_lowkeys()
     op = q.GpuOp(basis, H)
-    lowkeys_exact = op.verify_lowkeys() if basis.has_lowkeys() else None
+    lowkeys_mode = op.verify_lowkeys() if basis.has_lowkeys() else None
     tuned = None
     if args.variant == "auto":
         tuned = op.autotune(0, reps=3)  # fastest searching kernel on this workload
@@ -852,10 +852,10 @@ def bench_64(args):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(PMC_SUMMARY, variant_kernel(
                          tuned[0] if tuned else args.variant, 1,
-                         verified=bool(lowkeys_exact)))},
+                         low_mode=lowkeys_mode))},
         "apply_kernel_us": kern_s * 1e6,
         "apply_variant": tuned[0] if tuned else args.variant,
-        "low_keys_exact": lowkeys_exact,
+        "low_keys_mode": lowkeys_mode,
         "autotuned": tuned is not None,
         "bucket_bits": basis.bucket_bits(),
         "algorithmic_bytes_per_apply": int(bytes_apply),

@@ -181,16 +181,18 @@ enum {
   QSI_APPLY_LOW8 = 14
 };
 int qsi_op_set_variant(qsi_op_t op, int variant);
-/* Checks, once per operator, whether every connected state its terms reach
- * from a basis state either is the basis state its low-key lookup ends on or
- * differs from it in bucket key or low key (*exact = 1), by one pass over the
- * rows (about the cost of one apply; synchronises).  When it holds, the
- * QSI_APPLY_LOW* kernels compare the candidate's low key instead of loading
- * its state (the same candidates, so the same y).  Closed sectors (ring
- * flips on a full spinon sector) satisfy it; a basis missing some connected
- * states may not, and then nothing changes.  Needs the basis's low keys.  No
+/* Checks, once per operator, by one pass over the rows (about the cost of one
+ * apply; synchronises), how the QSI_APPLY_LOW* kernels may confirm the
+ * candidate a low-key lookup ends on.  *mode = 1 when every connected state
+ * its terms reach from a basis state either is that candidate or differs
+ * from it in bucket key or low key: the kernels then compare the candidate's
+ * low key instead of loading its state.  *mode = 2 when, in addition, every
+ * connected state is in the basis (a closed sector, such as ring flips on a
+ * full spinon sector): the kernels take the candidate with no compare.
+ * *mode = 0 otherwise: states are compared, as before the check.  The same
+ * candidates in every mode, so the same y.  Needs the basis's low keys.  No
  * counterpart in the reference. */
-int qsi_op_verify_lowkeys(qsi_op_t op, int *exact);
+int qsi_op_verify_lowkeys(qsi_op_t op, int *mode);
 int qsi_op_get_variant(qsi_op_t op);
 
 /* y = H x (accumulate = 0) or y += H x (accumulate = 1).

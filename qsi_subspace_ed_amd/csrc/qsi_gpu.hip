@@ -128,30 +128,34 @@ __global__ void k_build_hash(const uint64_t* __restrict__ states, int64_t dim,
 // For every row and connecting term of an operator: whether the target's
 // low-key lookup ends on a candidate with the target's bucket key and low key
 // but another state (a target outside the basis that differs from a basis
-// state only at positions fixed in the basis).  *bad |= 1 if any does
-// (qsi_op_verify_lowkeys).
+// state only at positions fixed in the basis: *bad |= 1), and whether any
+// target is outside the basis (*bad |= 2) (qsi_op_verify_lowkeys).
 template <int NW>
 __global__ void k_verify_lowkeys(BasisView b, const OffTerm* __restrict__ off, int n_off,
                                  unsigned int* __restrict__ bad) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= b.dim) return;
   const State<NW> s = load_state_cached<NW>(b.states, i);
-  bool clash = false;
+  bool clash = false, absent = false;
   for (int t = 0; t < n_off; ++t) {
     if (!term_ok<NW>(off[t], s)) continue;
     const State<NW> key = xor_state<NW>(s, off[t].X);
     uint32_t lo, n;
     bucket_bounds(b.table, bucket_key<NW>(b, key), lo, n);
-    if (n == 0) continue;
+    if (n == 0) {
+      absent = true;
+      continue;
+    }
     const uint32_t r = low_key<NW>(b, key);
     while (n > 1) {
       const uint32_t half = n >> 1;
       lo = b.lowkeys[lo + half] <= r ? lo + half : lo;
       n -= half;
     }
-    if (b.lowkeys[lo] == r && !st_eq<NW>(load_state_cached<NW>(b.states, lo), key)) clash = true;
+    if (b.lowkeys[lo] != r) absent = true;
+    else if (!st_eq<NW>(load_state_cached<NW>(b.states, lo), key)) clash = true;
   }
-  if (clash) atomicOr(bad, 1u);
+  if (clash || absent) atomicOr(bad, (clash ? 1u : 0u) | (absent ? 2u : 0u));
 }
 
 // lowkeys[i] = low key of state i (qsi_basis_build_lowkeys)
@@ -360,11 +364,14 @@ __device__ __forceinline__ State<NW> term_target(const TermsLds<NW>& tl, bool ld
 // mask, staged in LDS, for the final compare), so eight lookups per lane fit
 // the register budget the lane kernel needs for four.
 //
-// TRUST (after qsi_op_verify_lowkeys found it exact for this operator): a
-// target whose bucket key and low key match the candidate's is the
-// candidate, so the candidate's low key is compared instead of its state and
-// the state array is not read at all.
-template <int NW, bool HAS_Z, bool DOT, int G, bool TRUST>
+// CHECK says how a candidate is confirmed.  kLowCmpState: its state is
+// loaded and compared.  kLowCmpKey (qsi_op_verify_lowkeys found that a target
+// whose bucket key and low key match a basis state is that state): its low
+// key is compared, and the state array is not read.  kLowTrusted (the pass
+// also found every connected state in the basis): the candidate is the
+// target, with no compare.
+constexpr int kLowCmpState = 0, kLowCmpKey = 1, kLowTrusted = 2;
+template <int NW, bool HAS_Z, bool DOT, int G, int CHECK>
 __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
@@ -436,8 +443,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
           for (int u = 0; u < CH; ++u) {
             xj[u] = 0.0;
             if (n[g0 + u] == 1) {
-              if constexpr (TRUST) fk[u] = b.lowkeys[lo[g0 + u]];
-              else f[u] = load_state_cached<NW>(b.states, lo[g0 + u]);
+              if constexpr (CHECK == kLowCmpKey) fk[u] = b.lowkeys[lo[g0 + u]];
+              if constexpr (CHECK == kLowCmpState) f[u] = load_state_cached<NW>(b.states, lo[g0 + u]);
               xj[u] = x[lo[g0 + u]];
             }
           }
@@ -447,9 +454,10 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
           for (int u = 0; u < CH; ++u) {
             if (n[g0 + u] == 1 && fabs(xj[u]) > kApplyTol) {
               const int t = t0 + static_cast<int>(tt[g0 + u]);
-              bool hit;
-              if constexpr (TRUST) hit = fk[u] == r[g0 + u];
-              else hit = st_eq<NW>(f[u], term_target<NW>(tl, lds, off, s, t));
+              bool hit = true;
+              if constexpr (CHECK == kLowCmpKey) hit = fk[u] == r[g0 + u];
+              if constexpr (CHECK == kLowCmpState)
+                hit = st_eq<NW>(f[u], term_target<NW>(tl, lds, off, s, t));
               if (hit) {
                 double c = lds ? tl.c[t] : off[t].c * off[t].sgn;
                 if constexpr (HAS_Z) {
@@ -1475,34 +1483,46 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
                          y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LOW2:
-      if (op->lowkeys_exact)
-        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, true>), grid, block, 0, s, v,
-                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      if (op->lowkeys_check == kLowTrusted)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, kLowTrusted>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
+      else if (op->lowkeys_check == kLowCmpKey)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, kLowCmpKey>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
       else
-        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, false>), grid, block, 0, s, v,
-                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 2, kLowCmpState>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LOW4:
-      if (op->lowkeys_exact)
-        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, true>), grid, block, 0, s, v,
-                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      if (op->lowkeys_check == kLowTrusted)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, kLowTrusted>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
+      else if (op->lowkeys_check == kLowCmpKey)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, kLowCmpKey>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
       else
-        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, false>), grid, block, 0, s, v,
-                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 4, kLowCmpState>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
       break;
     case QSI_APPLY_LOW8:
-      if (op->lowkeys_exact)
-        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, true>), grid, block, 0, s, v,
-                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+      if (op->lowkeys_check == kLowTrusted)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, kLowTrusted>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
+      else if (op->lowkeys_check == kLowCmpKey)
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, kLowCmpKey>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
       else
-        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, false>), grid, block, 0, s, v,
-                           op->d_off, op->n_off, op->d_diag, op->n_diag, x, y,
-                           y_in, yscale, yscale_dev, partials, xs_dev, ys_dev);
+        hipLaunchKernelGGL((k_apply_low<NW, HAS_Z, DOT, 8, kLowCmpState>), grid, block, 0, s,
+                           v, op->d_off, op->n_off, op->d_diag, op->n_diag, x, y, y_in, yscale,
+                           yscale_dev, partials, xs_dev, ys_dev);
       break;
     default:  // QSI_APPLY_LANE4
       hipLaunchKernelGGL((k_apply_lane<NW, HAS_Z, DOT, 4>), grid, block, 0, s, v,
@@ -2370,11 +2390,11 @@ int qsi_op_set_variant(qsi_op_t op, int variant) {
   return QSI_OK;
 }
 
-int qsi_op_verify_lowkeys(qsi_op_t op, int* exact) {
+int qsi_op_verify_lowkeys(qsi_op_t op, int* mode) {
   QSI_REQUIRE(op != nullptr, "qsi_op_verify_lowkeys: null operator");
   QSI_REQUIRE(op->basis->d_lowkeys != nullptr,
               "qsi_op_verify_lowkeys: the basis has no low keys (qsi_basis_build_lowkeys)");
-  op->lowkeys_exact = false;
+  op->lowkeys_check = qsi::kLowCmpState;
   unsigned int* bad = nullptr;
   unsigned int h_bad = 0;
   QSI_CHECK_HIP(hipMalloc(&bad, sizeof(unsigned int)));
@@ -2396,8 +2416,10 @@ int qsi_op_verify_lowkeys(qsi_op_t op, int* exact) {
     set_error(std::string("qsi_op_verify_lowkeys: ") + hipGetErrorString(e));
     return QSI_E_HIP;
   }
-  op->lowkeys_exact = h_bad == 0;
-  if (exact) *exact = op->lowkeys_exact ? 1 : 0;
+  op->lowkeys_check = (h_bad & 1u) ? qsi::kLowCmpState
+                    : (h_bad & 2u) ? qsi::kLowCmpKey
+                                   : qsi::kLowTrusted;
+  if (mode) *mode = op->lowkeys_check;
   return QSI_OK;
 }
 

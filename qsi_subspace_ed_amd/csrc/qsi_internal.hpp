@@ -128,10 +128,12 @@ struct qsi_op_s {
   int conn_pad = 1;       // rows padded to multiples of conn_pad entries (1 or 4)
   int blk_cfg = 1;        // block-apply configuration (qsi_op_autotune)
   int cached_pipe = 8;    // gathers in flight per lane of the cached apply (4-32)
-  // qsi_op_verify_lowkeys found that no connected state outside the basis
-  // shares a bucket key and low key with a basis state (a property of the
-  // terms and the basis, both fixed, so it holds for any build of the keys)
-  bool lowkeys_exact = false;
+  // how the low-key kernels confirm a candidate (qsi_op_verify_lowkeys):
+  // 0 compare its state; 1 compare its low key (no connected state outside
+  // the basis shares a bucket key and low key with a basis state); 2 no
+  // compare (every connected state is in the basis).  A property of the terms
+  // and the basis, both fixed, so it holds for any build of the keys.
+  int lowkeys_check = 0;
 };
 
 namespace qsi {

@@ -219,12 +219,13 @@ class GpuOp:
         check(_lib.load().qsi_op_set_variant(self._h, v), "qsi_op_set_variant")
 
     def verify_lowkeys(self):
-        """qsi_op_verify_lowkeys: True when the low-key kernels may compare
-        the candidate's low key instead of its state for this operator (one
-        pass over the rows; the basis needs low keys)."""
-        ex = C.c_int(0)
-        check(_lib.load().qsi_op_verify_lowkeys(self._h, C.byref(ex)), "qsi_op_verify_lowkeys")
-        return bool(ex.value)
+        """qsi_op_verify_lowkeys (one pass over the rows; the basis needs low
+        keys): 0 when the low-key kernels must compare candidate states, 1
+        when they may compare low keys instead, 2 when every connected state
+        is in the basis and they need no compare."""
+        mode = C.c_int(0)
+        check(_lib.load().qsi_op_verify_lowkeys(self._h, C.byref(mode)), "qsi_op_verify_lowkeys")
+        return int(mode.value)
 
     def set_cached_pipe(self, pipe):
         """Pin the cached apply's gathers in flight per lane (4, 8, 16, 32)."""

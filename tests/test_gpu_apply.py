@@ -158,10 +158,11 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
     for y in ys[1:]:
         assert torch.equal(ys[0], y)
     # the low-key kernels after the verification pass (ring flips on a full
-    # sector reach only basis states, so it must hold there) give the same bits
+    # sector reach only basis states, so no compare is needed there) give the
+    # same bits
     exact = op.verify_lowkeys()
     if W == 32:
-        assert exact
+        assert exact == 2
     for v in ("low2", "low4", "low8"):
         op.set_variant(v)
         y = torch.zeros_like(x)

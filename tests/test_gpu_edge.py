@@ -126,7 +126,7 @@ def test_lowkeys_verification_finds_a_clash(cuda):
     basis = q.GpuBasis(st, 2, bucket_bits=1)
     basis.build_lowkeys()
     op = q.GpuOp(basis, H)
-    assert op.verify_lowkeys() is False
+    assert op.verify_lowkeys() == 0
     x = np.array([0.5, 0.25])
     ref = orc.evaluate_add(orc.Basis(st), H, x, np.zeros(2))
     assert np.all(ref == 0.0)
@@ -135,3 +135,30 @@ def test_lowkeys_verification_finds_a_clash(cuda):
         y = torch.full((2,), 9.0, dtype=torch.float64, device="cuda")
         op.evaluate(torch.from_numpy(x).cuda(), y)
         assert np.array_equal(y.cpu().numpy(), ref), v
+
+
+def test_lowkeys_verification_modes(lattices, cuda):
+    """The 32-spin basis with every third state dropped: ring flips now reach
+    states outside the basis, so the pass cannot return 2; whatever mode it
+    returns, the low-key kernels agree with the oracle and with LANE4 bit for
+    bit, and the full basis under the same terms gives 2."""
+    torch = cuda
+    jd = lattices[32]
+    H = _ham(jd)
+    for b, full in ((load_basis(32), True), (load_basis(32)[::3].copy(), False)):
+        basis = q.GpuBasis(b, 32)
+        basis.build_lowkeys()
+        op = q.GpuOp(basis, H)
+        mode = op.verify_lowkeys()
+        assert mode == 2 if full else mode in (0, 1)
+        x = np.random.default_rng(8).standard_normal(b.shape[0])
+        ref = orc.evaluate_add(orc.Basis(b), H, x, np.zeros(b.shape[0]))
+        ys = []
+        for v in ("lane4", "low2", "low4", "low8"):
+            op.set_variant(v)
+            y = torch.zeros(b.shape[0], dtype=torch.float64, device="cuda")
+            op.evaluate(torch.from_numpy(x).cuda(), y)
+            ys.append(y)
+        assert np.linalg.norm(ys[0].cpu().numpy() - ref) <= 1e-13 * np.linalg.norm(ref)
+        for y in ys[1:]:
+            assert torch.equal(ys[0], y)
