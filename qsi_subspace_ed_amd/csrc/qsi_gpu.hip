@@ -353,16 +353,45 @@ __device__ __forceinline__ State<NW> term_target(const TermsLds<NW>& tl, bool ld
   return r;
 }
 
+// Terms as the low-key kernels stage them in LDS (they run only when every
+// off-diagonal term fits): flip masks, signed coefficients, and each flip
+// mask's bucket key and low key.  Both keys select bits of a state, so they
+// commute with a flip: key(s ^ X) = key(s) ^ key(X).  A lookup forms its
+// keys from the row's own keys with two XORs instead of gathering them bit
+// field by bit field from the connected state.
+template <int NW>
+struct LowTermsLds {
+  uint64_t X[NW][kLdsTerms];
+  double c[kLdsTerms];   // c * sgn
+  uint2 key[kLdsTerms];  // bucket key, low key of X
+};
+
+template <int NW>
+__device__ __forceinline__ void stage_low_terms(LowTermsLds<NW>& tl, const BasisView& b,
+                                                const OffTerm* __restrict__ off, int n_off) {
+  for (int t = threadIdx.x; t < n_off; t += blockDim.x) {
+    State<NW> X;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      X.w[w] = off[t].X[w];
+      tl.X[w][t] = X.w[w];
+    }
+    tl.c[t] = off[t].c * off[t].sgn;
+    tl.key[t] = make_uint2(bucket_key<NW>(b, X), low_key<NW>(b, X));
+  }
+  __syncthreads();
+}
+
 // Low-key variant (QSI_APPLY_LOW*): as the lane variant, but the bisection
 // of a bucket probes the 4-byte low keys (qsi_basis_build_lowkeys) instead of
 // whole states — four times as many candidates per cache line, and the state
-// array is touched once per lookup, for the candidate.  The candidate is the
-// last state of the bucket whose low key is <= the target's, the same state
-// the lane variant's bisection ends on, so y is bit-identical.  A lookup
-// keeps only its term, bucket start, bucket size and low key while it
-// bisects (the connected state is re-formed from the row and the term's flip
-// mask, staged in LDS, for the final compare), so eight lookups per lane fit
-// the register budget the lane kernel needs for four.
+// array is touched once per lookup at most, for the candidate.  The
+// candidate is the last state of the bucket whose low key is <= the
+// target's, the same state the lane variant's bisection ends on, so y is
+// bit-identical.  A lookup keeps only its term, bucket start, bucket size and
+// low key.  Every stage issues the loads of all G lookups before it uses
+// any (a probe whose lookup has finished reads entry 0), so a lane keeps G
+// loads in flight through the table, each bisection step and the gathers.
 //
 // CHECK says how a candidate is confirmed.  kLowCmpState: its state is
 // loaded and compared.  kLowCmpKey (qsi_op_verify_lowkeys found that a target
@@ -383,14 +412,16 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
   // scales as in k_apply_lane (dot-product instances only)
   const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
   const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
-  __shared__ TermsLds<NW> tl;
-  const bool lds = stage_terms<NW>(tl, off, n_off);
+  __shared__ LowTermsLds<NW> tl;
+  stage_low_terms<NW>(tl, b, off, n_off);
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
   double prod = 0.0;
   if (i < b.dim) {
     const State<NW> s = load_state_cached<NW>(b.states, i);
     const double xi = x[i] * xs;
     double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
+    const uint32_t ks = bucket_key<NW>(b, s);
+    const uint32_t ls = low_key<NW>(b, s);
     for (int t0 = 0; t0 < n_off; t0 += 64) {
       const int tn = min(64, n_off - t0);
       uint64_t mask = 0;
@@ -398,36 +429,40 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
         mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
       }
       while (mask) {
-        // per lookup: term offset in the chunk, bucket start, bucket size
-        // (0: nothing to find), the target's low key
+        // per lookup: term, bucket start, bucket size (0: nothing to find),
+        // the target's low key; bucket bounds of all G lookups loaded first
         uint32_t tt[G], lo[G], n[G], r[G];
+        bool act[G];
+        uint2 p[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          const bool act = mask != 0;
-          tt[g] = act ? static_cast<uint32_t>(__builtin_ctzll(mask)) : 0u;
+          act[g] = mask != 0;
+          tt[g] = t0 + (act[g] ? static_cast<uint32_t>(__builtin_ctzll(mask)) : 0u);
           mask &= mask - 1;
-          lo[g] = 0;
-          n[g] = 0;
-          r[g] = 0;
-          if (act) {
-            const State<NW> key = term_target<NW>(tl, lds, off, s, t0 + tt[g]);
-            bucket_bounds(b.table, bucket_key<NW>(b, key), lo[g], n[g]);
-            r[g] = low_key<NW>(b, key);
-          }
+          const uint2 kx = tl.key[tt[g]];
+          r[g] = ls ^ kx.y;
+          __builtin_memcpy(&p[g], b.table + (act[g] ? ks ^ kx.x : 0u), sizeof(uint2));
         }
-        // interleaved bisection on the low keys down to one candidate
-        bool more = true;
+        bool more = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          lo[g] = p[g].x;
+          n[g] = act[g] ? p[g].y - p[g].x : 0u;
+          more |= n[g] > 1;
+        }
+        // interleaved bisection on the low keys down to one candidate: the G
+        // probes, then the G updates (a finished lookup has half = 0)
         while (more) {
+          uint32_t m[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g) m[g] = b.lowkeys[n[g] > 1 ? lo[g] + (n[g] >> 1) : 0u];
           more = false;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            if (n[g] > 1) {
-              const uint32_t half = n[g] >> 1;
-              const uint32_t m = b.lowkeys[lo[g] + half];
-              lo[g] = m <= r[g] ? lo[g] + half : lo[g];
-              n[g] -= half;
-              more |= n[g] > 1;
-            }
+            const uint32_t half = n[g] >> 1;
+            lo[g] = m[g] <= r[g] ? lo[g] + half : lo[g];
+            n[g] -= half;
+            more |= n[g] > 1;
           }
         }
         // candidates in chunks of up to four: every candidate state and x of
@@ -438,10 +473,11 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
         for (int g0 = 0; g0 < G; g0 += CH) {
           State<NW> f[CH];
           uint32_t fk[CH];
-          double xj[CH];
+          double xj[CH], c[CH];
 #pragma unroll
           for (int u = 0; u < CH; ++u) {
             xj[u] = 0.0;
+            c[u] = tl.c[tt[g0 + u]];
             if (n[g0 + u] == 1) {
               if constexpr (CHECK == kLowCmpKey) fk[u] = b.lowkeys[lo[g0 + u]];
               if constexpr (CHECK == kLowCmpState) f[u] = load_state_cached<NW>(b.states, lo[g0 + u]);
@@ -453,17 +489,21 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
 #pragma unroll
           for (int u = 0; u < CH; ++u) {
             if (n[g0 + u] == 1 && fabs(xj[u]) > kApplyTol) {
-              const int t = t0 + static_cast<int>(tt[g0 + u]);
+              const int t = static_cast<int>(tt[g0 + u]);
               bool hit = true;
               if constexpr (CHECK == kLowCmpKey) hit = fk[u] == r[g0 + u];
-              if constexpr (CHECK == kLowCmpState)
-                hit = st_eq<NW>(f[u], term_target<NW>(tl, lds, off, s, t));
+              if constexpr (CHECK == kLowCmpState) {
+                State<NW> key;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) key.w[w] = s.w[w] ^ tl.X[w][t];
+                hit = st_eq<NW>(f[u], key);
+              }
               if (hit) {
-                double c = lds ? tl.c[t] : off[t].c * off[t].sgn;
+                double cu = c[u];
                 if constexpr (HAS_Z) {
-                  if (parity_notz<NW>(s, off[t].Z)) c = -c;
+                  if (parity_notz<NW>(s, off[t].Z)) cu = -cu;
                 }
-                acc += c * xj[u];
+                acc += cu * xj[u];
               }
             }
           }
@@ -1421,10 +1461,12 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
     variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE4;
   // a slot or low-key variant chosen while the basis had that index, which
   // has been dropped since, falls back to the lane kernel (LANE4 for SLOT4,
-  // LANE2 for SLOT2 and SLOT3; the same depth for LOW*)
+  // LANE2 for SLOT2 and SLOT3; the same depth for LOW*, which also falls
+  // back when the terms do not all fit its LDS staging)
   if (!op->basis->d_slots && variant >= QSI_APPLY_SLOT2 && variant <= QSI_APPLY_SLOT3)
     variant = variant == QSI_APPLY_SLOT4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE2;
-  if (!op->basis->d_lowkeys && variant >= QSI_APPLY_LOW2 && variant <= QSI_APPLY_LOW8)
+  if ((!op->basis->d_lowkeys || op->n_off > kLdsTerms) && variant >= QSI_APPLY_LOW2 &&
+      variant <= QSI_APPLY_LOW8)
     variant = variant == QSI_APPLY_LOW2 ? QSI_APPLY_LANE2
             : variant == QSI_APPLY_LOW4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE8;
   switch (variant) {
