@@ -204,119 +204,6 @@ __global__ void k_search(BasisView b, const uint64_t* __restrict__ q, int64_t n,
 // Apply: one lane per output row
 // ============================================================================
 
-// y[i] = yscale * y_in[i] + sum_t H_{i,t} x[...] + d(s_i) x[i]
-// If DOT, partials[block] = sum over the block's rows of x[i] * y[i].
-// Lane variant: every lane walks its own row's connected states and runs a
-// full table lookup for each.  G lookups are carried through the bucket
-// table, the bisection and the x gather together, so a lane keeps G
-// independent memory chains in flight; contributions are still added in
-// term order.
-template <int NW, bool HAS_Z, bool DOT, int G>
-__global__ __launch_bounds__(kApplyBlock) void k_apply_lane(
-    BasisView b, const OffTerm* __restrict__ off, int n_off,
-    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
-    double* y, const double* y_in, double yscale,
-    const double* __restrict__ yscale_dev, double* __restrict__ partials,
-    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
-  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  if (yscale_dev) yscale *= yscale_dev[0];
-  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
-  // instances (the Lanczos step) take scales; elsewhere they are the
-  // constant 1 and the multiplies fold away.
-  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
-  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
-  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
-  double prod = 0.0;
-  if (i < b.dim) {
-    const State<NW> s = load_state_cached<NW>(b.states, i);
-    const double xi = x[i] * xs;
-    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
-    for (int t0 = 0; t0 < n_off; t0 += 64) {
-      const int tn = min(64, n_off - t0);
-      // which terms connect this row (uniform loop over terms)
-      uint64_t mask = 0;
-      for (int t = 0; t < tn; ++t) {
-        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
-      }
-      while (mask) {
-        int tt[G];
-        bool act[G];
-        State<NW> key[G];
-        uint32_t lo[G], n[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          act[g] = mask != 0;
-          tt[g] = act[g] ? t0 + __builtin_ctzll(mask) : t0;
-          mask &= mask - 1;
-          key[g] = xor_state<NW>(s, off[tt[g]].X);
-        }
-        // bucket bounds
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          lo[g] = 0;
-          n[g] = 0;
-          if (act[g]) {
-            const uint32_t k = bucket_key<NW>(b, key[g]);
-            bucket_bounds(b.table, k, lo[g], n[g]);
-          }
-        }
-        // interleaved bisection down to one candidate per key
-        bool more = true;
-        while (more) {
-          more = false;
-#pragma unroll
-          for (int g = 0; g < G; ++g) {
-            if (n[g] > 1) {
-              const uint32_t half = n[g] >> 1;
-              const State<NW> m = load_state_cached<NW>(b.states, lo[g] + half);
-              lo[g] = st_le<NW>(m, key[g]) ? lo[g] + half : lo[g];
-              n[g] -= half;
-              more |= n[g] > 1;
-            }
-          }
-        }
-        // the candidate's state and x are loaded together (lo < dim whenever
-        // the bucket is not empty); x counts only if the state matches
-        double xj[G];
-        bool hit[G];
-        State<NW> f[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          xj[g] = 0.0;
-          if (n[g] == 1) {
-            f[g] = load_state_cached<NW>(b.states, lo[g]);
-            xj[g] = x[lo[g]];
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) hit[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]);
-        // the scale is applied where the values are consumed, after every
-        // gather of the group has been issued
-#pragma unroll
-        for (int g = 0; g < G; ++g) xj[g] *= xs;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          if (hit[g] && fabs(xj[g]) > kApplyTol) {
-            const OffTerm& T = off[tt[g]];
-            double c = T.c * T.sgn;
-            if constexpr (HAS_Z) {
-              if (parity_notz<NW>(s, T.Z)) c = -c;
-            }
-            acc += c * xj[g];
-          }
-        }
-      }
-    }
-    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
-    y[i] = acc;
-    if constexpr (DOT) prod = xi * acc;
-  }
-  if constexpr (DOT) {
-    const double bs = block_sum<kApplyBlock>(prod);
-    if (threadIdx.x == 0) partials[blk] = bs;
-  }
-}
-
 // Flip masks and signed coefficients of the first kLdsTerms off-diagonal
 // terms, staged in LDS by every workgroup: lookups re-form their connected
 // state and take their coefficient from here instead of gathering each
@@ -353,23 +240,28 @@ __device__ __forceinline__ State<NW> term_target(const TermsLds<NW>& tl, bool ld
   return r;
 }
 
-// Terms as the low-key kernels stage them in LDS (they run only when every
-// off-diagonal term fits): flip masks, signed coefficients, and each flip
-// mask's bucket key and low key.  Both keys select bits of a state, so they
-// commute with a flip: key(s ^ X) = key(s) ^ key(X).  A lookup forms its
-// keys from the row's own keys with two XORs instead of gathering them bit
-// field by bit field from the connected state.
+// Terms as the lane and low-key kernels stage them in LDS, kLdsTerms at a
+// time: flip masks, signed coefficients, and each flip mask's bucket key and
+// low key.  Both keys select bits of a state, so they commute with a flip:
+// key(s ^ X) = key(s) ^ key(X).  A lookup forms its keys from the row's own
+// keys with an XOR instead of gathering them bit field by bit field from
+// the connected state.
 template <int NW>
 struct LowTermsLds {
   uint64_t X[NW][kLdsTerms];
   double c[kLdsTerms];   // c * sgn
-  uint2 key[kLdsTerms];  // bucket key, low key of X
+  uint2 key[kLdsTerms];  // bucket key, low key (0 without low keys) of X
 };
 
+// Stages terms [0, n) of `off` (n <= kLdsTerms).  Every thread of the
+// workgroup calls it; `reuse` (uniform) first waits until every thread is
+// done with the previous chunk.
 template <int NW>
 __device__ __forceinline__ void stage_low_terms(LowTermsLds<NW>& tl, const BasisView& b,
-                                                const OffTerm* __restrict__ off, int n_off) {
-  for (int t = threadIdx.x; t < n_off; t += blockDim.x) {
+                                                const OffTerm* __restrict__ off, int n,
+                                                bool reuse) {
+  if (reuse) __syncthreads();
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
     State<NW> X;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
@@ -377,9 +269,143 @@ __device__ __forceinline__ void stage_low_terms(LowTermsLds<NW>& tl, const Basis
       tl.X[w][t] = X.w[w];
     }
     tl.c[t] = off[t].c * off[t].sgn;
-    tl.key[t] = make_uint2(bucket_key<NW>(b, X), low_key<NW>(b, X));
+    tl.key[t] = make_uint2(bucket_key<NW>(b, X), b.lowkeys ? low_key<NW>(b, X) : 0u);
   }
   __syncthreads();
+}
+
+template <int NW>
+__device__ __forceinline__ State<NW> staged_target(const LowTermsLds<NW>& tl,
+                                                   const State<NW>& s, uint32_t t) {
+  State<NW> r;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) r.w[w] = s.w[w] ^ tl.X[w][t];
+  return r;
+}
+
+// y[i] = yscale * y_in[i] + sum_t H_{i,t} x[...] + d(s_i) x[i]
+// If DOT, partials[block] = sum over the block's rows of x[i] * y[i].
+// Lane variant: every lane walks its own row's connected states and runs a
+// full table lookup for each.  G lookups go through the bucket table, the
+// bisection and the x gather together, and every stage issues the loads of
+// all G before it uses any (a probe whose lookup has finished reads entry
+// 0), so a lane keeps G loads in flight; contributions are still added in
+// term order.
+template <int NW, bool HAS_Z, bool DOT, int G>
+__global__ __launch_bounds__(kApplyBlock)
+__attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? (HAS_Z || DOT ? 5 : 6) : 8))) void
+    k_apply_lane(
+    BasisView b, const OffTerm* __restrict__ off, int n_off,
+    const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
+    double* y, const double* y_in, double yscale,
+    const double* __restrict__ yscale_dev, double* __restrict__ partials,
+    const double* __restrict__ xs_dev, const double* __restrict__ ys_dev) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (yscale_dev) yscale *= yscale_dev[0];
+  // x and y_in are read as xs * x and ys * y_in.  Only the dot-product
+  // instances (the Lanczos step) take scales; elsewhere they are the
+  // constant 1 and the multiplies fold away.
+  const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
+  const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
+  __shared__ LowTermsLds<NW> tl;
+  const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
+  const bool live = i < b.dim;
+  State<NW> s = {};
+  double xi = 0.0, acc = 0.0, prod = 0.0;
+  uint32_t ks = 0;
+  if (live) {
+    s = load_state_cached<NW>(b.states, i);
+    xi = x[i] * xs;
+    acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
+    ks = bucket_key<NW>(b, s);
+  }
+  for (int c0 = 0; c0 < n_off; c0 += kLdsTerms) {
+    const int cn = min(kLdsTerms, n_off - c0);
+    stage_low_terms<NW>(tl, b, off + c0, cn, c0 > 0);
+    if (!live) continue;
+    for (int t0 = 0; t0 < cn; t0 += 64) {
+      const int tn = min(64, cn - t0);
+      // which terms connect this row (uniform loop over terms)
+      uint64_t mask = 0;
+      for (int t = 0; t < tn; ++t) {
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[c0 + t0 + t], s)) << t;
+      }
+      while (mask) {
+        uint32_t tt[G], lo[G], n[G];
+        bool act[G];
+        uint2 p[G];
+        // bucket bounds of all G lookups (each connected state is re-formed
+        // from the row and the staged flip mask where it is compared)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          act[g] = mask != 0;
+          tt[g] = t0 + (act[g] ? static_cast<uint32_t>(__builtin_ctzll(mask)) : 0u);
+          mask &= mask - 1;
+          __builtin_memcpy(&p[g], b.table + (act[g] ? ks ^ tl.key[tt[g]].x : 0u), sizeof(uint2));
+        }
+        bool more = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          lo[g] = p[g].x;
+          n[g] = act[g] ? p[g].y - p[g].x : 0u;
+          more |= n[g] > 1;
+        }
+        // interleaved bisection down to one candidate per key: the G probes,
+        // then the G updates (a finished lookup has half = 0)
+        while (more) {
+          State<NW> m[G];
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            m[g] = load_state_cached<NW>(b.states, n[g] > 1 ? lo[g] + (n[g] >> 1) : 0u);
+          more = false;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            const uint32_t half = n[g] >> 1;
+            lo[g] = st_le<NW>(m[g], staged_target<NW>(tl, s, tt[g])) ? lo[g] + half : lo[g];
+            n[g] -= half;
+            more |= n[g] > 1;
+          }
+        }
+        // the candidate's state and x are loaded together (lo < dim whenever
+        // the bucket is not empty); x counts only if the state matches
+        double xj[G], c[G];
+        State<NW> f[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          xj[g] = 0.0;
+          c[g] = tl.c[tt[g]];
+          if (n[g] == 1) {
+            f[g] = load_state_cached<NW>(b.states, lo[g]);
+            xj[g] = x[lo[g]];
+          }
+        }
+        // the scale is applied where the values are consumed, after every
+        // gather of the group has been issued
+#pragma unroll
+        for (int g = 0; g < G; ++g) xj[g] *= xs;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (n[g] == 1 && st_eq<NW>(f[g], staged_target<NW>(tl, s, tt[g])) &&
+              fabs(xj[g]) > kApplyTol) {
+            double cg = c[g];
+            if constexpr (HAS_Z) {
+              if (parity_notz<NW>(s, off[c0 + tt[g]].Z)) cg = -cg;
+            }
+            acc += cg * xj[g];
+          }
+        }
+      }
+    }
+  }
+  if (live) {
+    acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
+    y[i] = acc;
+    if constexpr (DOT) prod = xi * acc;
+  }
+  if constexpr (DOT) {
+    const double bs = block_sum<kApplyBlock>(prod);
+    if (threadIdx.x == 0) partials[blk] = bs;
+  }
 }
 
 // Low-key variant (QSI_APPLY_LOW*): as the lane variant, but the bisection
@@ -401,7 +427,8 @@ __device__ __forceinline__ void stage_low_terms(LowTermsLds<NW>& tl, const Basis
 // target, with no compare.
 constexpr int kLowCmpState = 0, kLowCmpKey = 1, kLowTrusted = 2;
 template <int NW, bool HAS_Z, bool DOT, int G, int CHECK>
-__global__ __launch_bounds__(kApplyBlock) void k_apply_low(
+__global__ __launch_bounds__(kApplyBlock)
+__attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_low(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
@@ -413,20 +440,27 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
   const double xs = (DOT && xs_dev) ? xs_dev[0] : 1.0;
   const double ys = (DOT && ys_dev) ? ys_dev[0] : 1.0;
   __shared__ LowTermsLds<NW> tl;
-  stage_low_terms<NW>(tl, b, off, n_off);
   const int64_t i = static_cast<int64_t>(blk) * kApplyBlock + threadIdx.x;
-  double prod = 0.0;
-  if (i < b.dim) {
-    const State<NW> s = load_state_cached<NW>(b.states, i);
-    const double xi = x[i] * xs;
-    double acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
-    const uint32_t ks = bucket_key<NW>(b, s);
-    const uint32_t ls = low_key<NW>(b, s);
-    for (int t0 = 0; t0 < n_off; t0 += 64) {
-      const int tn = min(64, n_off - t0);
+  const bool live = i < b.dim;
+  State<NW> s = {};
+  double xi = 0.0, acc = 0.0, prod = 0.0;
+  uint32_t ks = 0, ls = 0;
+  if (live) {
+    s = load_state_cached<NW>(b.states, i);
+    xi = x[i] * xs;
+    acc = y_in ? yscale * (y_in[i] * ys) : 0.0;
+    ks = bucket_key<NW>(b, s);
+    ls = low_key<NW>(b, s);
+  }
+  for (int c0 = 0; c0 < n_off; c0 += kLdsTerms) {
+    const int cn = min(kLdsTerms, n_off - c0);
+    stage_low_terms<NW>(tl, b, off + c0, cn, c0 > 0);
+    if (!live) continue;
+    for (int t0 = 0; t0 < cn; t0 += 64) {
+      const int tn = min(64, cn - t0);
       uint64_t mask = 0;
       for (int t = 0; t < tn; ++t) {
-        mask |= static_cast<uint64_t>(term_ok<NW>(off[t0 + t], s)) << t;
+        mask |= static_cast<uint64_t>(term_ok<NW>(off[c0 + t0 + t], s)) << t;
       }
       while (mask) {
         // per lookup: term, bucket start, bucket size (0: nothing to find),
@@ -493,15 +527,12 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
               bool hit = true;
               if constexpr (CHECK == kLowCmpKey) hit = fk[u] == r[g0 + u];
               if constexpr (CHECK == kLowCmpState) {
-                State<NW> key;
-#pragma unroll
-                for (int w = 0; w < NW; ++w) key.w[w] = s.w[w] ^ tl.X[w][t];
-                hit = st_eq<NW>(f[u], key);
+                hit = st_eq<NW>(f[u], staged_target<NW>(tl, s, tt[g0 + u]));
               }
               if (hit) {
                 double cu = c[u];
                 if constexpr (HAS_Z) {
-                  if (parity_notz<NW>(s, off[t].Z)) cu = -cu;
+                  if (parity_notz<NW>(s, off[c0 + t].Z)) cu = -cu;
                 }
                 acc += cu * xj[u];
               }
@@ -510,6 +541,8 @@ __global__ __launch_bounds__(kApplyBlock) void k_apply_low(
         }
       }
     }
+  }
+  if (live) {
     acc = diag_accumulate<NW>(acc, xi, s, dg, n_diag);
     y[i] = acc;
     if constexpr (DOT) prod = xi * acc;
@@ -1461,12 +1494,10 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
     variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE4;
   // a slot or low-key variant chosen while the basis had that index, which
   // has been dropped since, falls back to the lane kernel (LANE4 for SLOT4,
-  // LANE2 for SLOT2 and SLOT3; the same depth for LOW*, which also falls
-  // back when the terms do not all fit its LDS staging)
+  // LANE2 for SLOT2 and SLOT3; the same depth for LOW*)
   if (!op->basis->d_slots && variant >= QSI_APPLY_SLOT2 && variant <= QSI_APPLY_SLOT3)
     variant = variant == QSI_APPLY_SLOT4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE2;
-  if ((!op->basis->d_lowkeys || op->n_off > kLdsTerms) && variant >= QSI_APPLY_LOW2 &&
-      variant <= QSI_APPLY_LOW8)
+  if (!op->basis->d_lowkeys && variant >= QSI_APPLY_LOW2 && variant <= QSI_APPLY_LOW8)
     variant = variant == QSI_APPLY_LOW2 ? QSI_APPLY_LANE2
             : variant == QSI_APPLY_LOW4 ? QSI_APPLY_LANE4 : QSI_APPLY_LANE8;
   switch (variant) {

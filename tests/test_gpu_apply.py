@@ -12,6 +12,7 @@ from conftest import load_basis
 from oracle import pyoracle as orc
 import qsi_subspace_ed_amd as q
 from qsi_subspace_ed_amd._lib import QsiError
+from qsi_subspace_ed_amd.lattice import add_ring_terms
 
 pytestmark = pytest.mark.gpu
 
@@ -178,6 +179,57 @@ def test_variants_bit_identical(W, kbits, lattices, cuda):
     op.evaluate_add(x, y2)
     assert torch.equal(y1, y2)
     assert op.cache_connectivity(False) == -1
+
+
+def test_apply_many_terms(lattices, cuda):
+    """More off-diagonal terms than one LDS staging chunk holds (256): the
+    32-spin ring terms five times over, with two coefficient sets in turn (320
+    flips, five coefficient classes, so the connectivity can be cached), plus
+    the local field.  Every lookup kernel stages the terms in chunks and gives
+    the oracle's y, with the same bits across variants."""
+    torch = cuda
+    jd = lattices[32]
+    b = load_basis(32)
+    D = b.shape[0]
+    H = q.SymbolicOpSum()
+    for rep in range(5):
+        add_ring_terms(H, jd, [-1.0 + 0.4 * (rep % 2), -0.55, -0.3, -0.8])
+    q.add_local_field(H, jd, [0.07, -0.02, 0.05])
+    assert len(H.off_diag_terms) > 256
+    basis = q.GpuBasis(b, 32)
+    basis.build_lowkeys()
+    op = q.GpuOp(basis, H)
+    x = _rand_x(D, 17)
+    ref = orc.evaluate_add(orc.Basis(b), H, x, np.zeros(D))
+    xd = torch.from_numpy(x).cuda()
+    ys = []
+    for v in ("lane", "lane4", "lane8", "low2", "low4", "low8", "window", "slot4", "cached"):
+        if v == "slot4":
+            basis.build_slots()
+        if v == "cached":
+            op.cache_connectivity(True)
+        op.set_variant(v)
+        y = torch.zeros_like(xd)
+        op.evaluate(xd, y)
+        assert np.linalg.norm(y.cpu().numpy() - ref) <= HV_TOL * np.linalg.norm(ref), v
+        ys.append((v, y))
+    assert op.verify_lowkeys() == 2
+    for v in ("low2", "low4", "low8"):
+        op.set_variant(v)
+        y = torch.zeros_like(xd)
+        op.evaluate(xd, y)
+        ys.append((v + " verified", y))
+    for v, y in ys[1:]:
+        assert torch.equal(ys[0][1], y), v
+    # the dot-product instances (the Lanczos step) through the chunked kernels
+    op.cache_connectivity(False)
+    for v in ("lane4", "low4"):
+        op.set_variant(v)
+        a, bt = op.lanczos_coeffs(6, seed=3)
+        if v == "lane4":
+            a0, b0 = a, bt
+        else:
+            assert np.array_equal(a, a0) and np.array_equal(bt, b0)
 
 
 def test_apply_partial_basis(lattices, cuda):
