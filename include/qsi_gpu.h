@@ -161,7 +161,8 @@ int qsi_op_n_diag(qsi_op_t op);
  * LANE4 (SLOT4) or LANE2.  QSI_APPLY_LOW{2,4,8}: as LANE*, bisecting each
  * bucket on the basis's low keys (qsi_basis_build_lowkeys); setting one
  * requires them, and without them later the apply falls back to LANE* of the
- * same depth.  AUTO = CACHED when the connectivity is cached, else LANE4.
+ * same depth.  AUTO = CACHED when the connectivity is cached, else LOW4 when
+ * the basis has low keys, else LANE4.
  * All give bit-identical y. */
 enum {
   QSI_APPLY_AUTO = 0,

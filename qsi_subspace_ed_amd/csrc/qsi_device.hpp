@@ -19,6 +19,72 @@ __device__ __forceinline__ bool term_ok(const OffTerm& t, const State<NW>& s) {
   }
 }
 
+// w = 2 w + (this lane's bit of `lanes`): the lane mask of a compare
+// shifted into a per-lane word by one add with carry-in.
+__device__ __forceinline__ uint32_t shift_in_lane(uint32_t w, uint64_t lanes) {
+  uint32_t r;
+  uint64_t carry;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry) : "v"(w), "s"(lanes));
+  return r;
+}
+
+// A term's projector mask and value, read as one 32-byte block.
+struct TermMV {
+  uint64_t m[2];
+  uint64_t v[2];
+};
+template <int NW>
+__device__ __forceinline__ TermMV load_mv(const OffTerm* __restrict__ t) {
+  return *reinterpret_cast<const TermMV*>(t);
+}
+
+// Lanes of the wavefront whose s & m (u) equals v.
+template <int NW>
+__device__ __forceinline__ uint64_t match_lanes(const uint64_t* u, const uint64_t* v) {
+  uint64_t b = __builtin_amdgcn_ballot_w64(u[0] == v[0]);
+  if constexpr (NW == 2) b &= __builtin_amdgcn_ballot_w64(u[1] == v[1]);
+  return b;
+}
+
+// Which of the terms off[0, tn) (tn <= 64) connect row s: bit 63 - t of the
+// result for term t, so the terms come out in order by leading zeros.  Every
+// lane of the wavefront tests the same term at once against its masks held
+// in scalar registers; the compare's lane mask goes into a 32-bit word per
+// 32 terms with one add, and a term with the same projector mask m as the
+// one before it (a ring flip and its conjugate) reuses s & m.  Called by
+// every active lane of the wavefront together.
+template <int NW>
+__device__ __forceinline__ uint64_t connect_mask(const OffTerm* __restrict__ off, int tn,
+                                                 const State<NW>& s) {
+  uint32_t word[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int lo = 32 * h, hi = min(tn, lo + 32);
+    uint32_t w = 0;
+    for (int t = lo; t < hi;) {
+      // the projector masks of this term and the next, loaded together
+      const TermMV A = load_mv<NW>(off + t);
+      uint64_t u[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) u[q] = s.w[q] & A.m[q];
+      w = shift_in_lane(w, match_lanes<NW>(u, A.v));
+      ++t;
+      if (t < hi) {
+        const TermMV B = load_mv<NW>(off + t);
+        bool same = B.m[0] == A.m[0];
+        if constexpr (NW == 2) same = same && B.m[1] == A.m[1];
+        if (same) {
+          w = shift_in_lane(w, match_lanes<NW>(u, B.v));
+          ++t;
+        }
+      }
+    }
+    const int n = hi - lo;
+    word[h] = n > 0 ? w << (32 - n) : 0u;
+  }
+  return (static_cast<uint64_t>(word[0]) << 32) | word[1];
+}
+
 template <int NW>
 __device__ __forceinline__ int parity_notz(const State<NW>& s,
                                            const uint64_t* Z) {

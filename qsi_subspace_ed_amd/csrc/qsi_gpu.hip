@@ -288,13 +288,14 @@ __device__ __forceinline__ State<NW> staged_target(const LowTermsLds<NW>& tl,
 // Lane variant: every lane walks its own row's connected states and runs a
 // full table lookup for each.  G lookups go through the bucket table, the
 // bisection and the x gather together, and every stage issues the loads of
-// all G before it uses any (a probe whose lookup has finished reads entry
-// 0), so a lane keeps G loads in flight; contributions are still added in
-// term order.
+// all G (those of unfinished lookups) before it uses any, so a lane keeps G
+// loads in flight; contributions are still added in term order.  An empty
+// asm statement that reads each loaded value after the last load of the
+// stage keeps the compiler from moving a use next to its load, which would
+// make every load wait for the one before.
 template <int NW, bool HAS_Z, bool DOT, int G>
 __global__ __launch_bounds__(kApplyBlock)
-__attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? (HAS_Z || DOT ? 5 : 6) : 8))) void
-    k_apply_lane(
+__attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? 5 : 8))) void k_apply_lane(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
@@ -325,11 +326,8 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? (HAS_Z || DOT ? 5 : 6) 
     if (!live) continue;
     for (int t0 = 0; t0 < cn; t0 += 64) {
       const int tn = min(64, cn - t0);
-      // which terms connect this row (uniform loop over terms)
-      uint64_t mask = 0;
-      for (int t = 0; t < tn; ++t) {
-        mask |= static_cast<uint64_t>(term_ok<NW>(off[c0 + t0 + t], s)) << t;
-      }
+      // which terms connect this row (term t0 + t at bit 63 - t)
+      uint64_t mask = connect_mask<NW>(off + c0 + t0, tn, s);
       while (mask) {
         uint32_t tt[G], lo[G], n[G];
         bool act[G];
@@ -339,10 +337,14 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? (HAS_Z || DOT ? 5 : 6) 
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           act[g] = mask != 0;
-          tt[g] = t0 + (act[g] ? static_cast<uint32_t>(__builtin_ctzll(mask)) : 0u);
-          mask &= mask - 1;
-          __builtin_memcpy(&p[g], b.table + (act[g] ? ks ^ tl.key[tt[g]].x : 0u), sizeof(uint2));
+          const uint32_t z = act[g] ? static_cast<uint32_t>(__builtin_clzll(mask)) : 0u;
+          tt[g] = t0 + z;
+          mask &= ~(uint64_t{1} << 63 >> z);
+          p[g] = make_uint2(0u, 0u);
+          if (act[g]) __builtin_memcpy(&p[g], b.table + (ks ^ tl.key[tt[g]].x), sizeof(uint2));
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g) asm volatile("" : "+v"(p[g].x), "+v"(p[g].y));
         bool more = false;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -351,12 +353,19 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? (HAS_Z || DOT ? 5 : 6) 
           more |= n[g] > 1;
         }
         // interleaved bisection down to one candidate per key: the G probes,
-        // then the G updates (a finished lookup has half = 0)
+        // then the G updates (a finished lookup has half = 0: no change)
         while (more) {
           State<NW> m[G];
 #pragma unroll
+          for (int g = 0; g < G; ++g) {
+            m[g] = State<NW>{};
+            if (n[g] > 1) m[g] = load_state_cached<NW>(b.states, lo[g] + (n[g] >> 1));
+          }
+          // every probe of the group is used only after all were issued
+#pragma unroll
           for (int g = 0; g < G; ++g)
-            m[g] = load_state_cached<NW>(b.states, n[g] > 1 ? lo[g] + (n[g] >> 1) : 0u);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) asm volatile("" : "+v"(m[g].w[w]));
           more = false;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
@@ -415,8 +424,8 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? (HAS_Z || DOT ? 5 : 6) 
 // candidate is the last state of the bucket whose low key is <= the
 // target's, the same state the lane variant's bisection ends on, so y is
 // bit-identical.  A lookup keeps only its term, bucket start, bucket size and
-// low key.  Every stage issues the loads of all G lookups before it uses
-// any (a probe whose lookup has finished reads entry 0), so a lane keeps G
+// low key.  Every stage issues the loads of all G lookups (those not yet
+// finished) before it uses any, as in the lane variant, so a lane keeps G
 // loads in flight through the table, each bisection step and the gathers.
 //
 // CHECK says how a candidate is confirmed.  kLowCmpState: its state is
@@ -458,10 +467,7 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
     if (!live) continue;
     for (int t0 = 0; t0 < cn; t0 += 64) {
       const int tn = min(64, cn - t0);
-      uint64_t mask = 0;
-      for (int t = 0; t < tn; ++t) {
-        mask |= static_cast<uint64_t>(term_ok<NW>(off[c0 + t0 + t], s)) << t;
-      }
+      uint64_t mask = connect_mask<NW>(off + c0 + t0, tn, s);
       while (mask) {
         // per lookup: term, bucket start, bucket size (0: nothing to find),
         // the target's low key; bucket bounds of all G lookups loaded first
@@ -471,12 +477,16 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           act[g] = mask != 0;
-          tt[g] = t0 + (act[g] ? static_cast<uint32_t>(__builtin_ctzll(mask)) : 0u);
-          mask &= mask - 1;
+          const uint32_t z = act[g] ? static_cast<uint32_t>(__builtin_clzll(mask)) : 0u;
+          tt[g] = t0 + z;
+          mask &= ~(uint64_t{1} << 63 >> z);
           const uint2 kx = tl.key[tt[g]];
           r[g] = ls ^ kx.y;
-          __builtin_memcpy(&p[g], b.table + (act[g] ? ks ^ kx.x : 0u), sizeof(uint2));
+          p[g] = make_uint2(0u, 0u);
+          if (act[g]) __builtin_memcpy(&p[g], b.table + (ks ^ kx.x), sizeof(uint2));
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g) asm volatile("" : "+v"(p[g].x), "+v"(p[g].y));
         bool more = false;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -485,11 +495,18 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
           more |= n[g] > 1;
         }
         // interleaved bisection on the low keys down to one candidate: the G
-        // probes, then the G updates (a finished lookup has half = 0)
+        // probes, then the G updates (a finished lookup has half = 0: no
+        // change)
         while (more) {
           uint32_t m[G];
 #pragma unroll
-          for (int g = 0; g < G; ++g) m[g] = b.lowkeys[n[g] > 1 ? lo[g] + (n[g] >> 1) : 0u];
+          for (int g = 0; g < G; ++g) {
+            m[g] = 0;
+            if (n[g] > 1) m[g] = b.lowkeys[lo[g] + (n[g] >> 1)];
+          }
+          // every probe of the group is used only after all were issued
+#pragma unroll
+          for (int g = 0; g < G; ++g) asm volatile("" : "+v"(m[g]));
           more = false;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
@@ -1491,7 +1508,9 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   // fastest search kernel measured on the 64- and 96-spin bases, with or
   // without the slot index and low keys; the autotuner times them all)
   if (variant == QSI_APPLY_AUTO)
-    variant = op->conn_nnz >= 0 ? QSI_APPLY_CACHED : QSI_APPLY_LANE4;
+    variant = op->conn_nnz >= 0       ? QSI_APPLY_CACHED
+              : op->basis->d_lowkeys ? QSI_APPLY_LOW4
+                                     : QSI_APPLY_LANE4;
   // a slot or low-key variant chosen while the basis had that index, which
   // has been dropped since, falls back to the lane kernel (LANE4 for SLOT4,
   // LANE2 for SLOT2 and SLOT3; the same depth for LOW*)
