@@ -218,7 +218,7 @@ __device__ __forceinline__ int64_t basis_lookup(const BasisView& b,
 // ---- slot index ------------------------------------------------------------
 // A second level under the bucket table for bases whose buckets stay large
 // (the 96-spin ice basis spreads its states over 49 informative positions, so
-// a 30-bit key leaves ~230 states in the bucket of an average lookup).  The
+// a 31-bit key leaves ~115 states in the bucket of an average lookup).  The
 // bucket of ranks [lo, lo + n) owns the 2n slots [2 lo, 2 lo + 2n) of one
 // uint32 array (so no offsets are stored: the slots of all buckets tile
 // [0, 2 D)); each of its states sits in one slot, placed by linear probing
