@@ -60,43 +60,62 @@ __device__ __forceinline__ void blk_wave_sync() {
 }
 
 // Ranks of up to B keys with all bisections interleaved; kNoRank for
-// keys absent from the basis (or inactive slots).
+// keys absent from the basis (or inactive slots).  Every stage issues the
+// loads of all unfinished lookups before it uses any; an empty asm statement
+// that reads each loaded value after the last load keeps the compiler from
+// moving a use next to its load (which would make each load wait for the
+// one before).
 template <int NW, int B>
 __device__ __forceinline__ void lookup_batch(const BasisView& b, const State<NW>* key,
                                              const bool* act, uint32_t* rank) {
   uint32_t lo[B], n[B];
+  uint2 p[B];
 #pragma unroll
   for (int g = 0; g < B; ++g) {
-    lo[g] = 0;
-    n[g] = 0;
-    if (act[g]) {
-      const uint32_t k = bucket_key<NW>(b, key[g]);
-      lo[g] = b.table[k];
-      n[g] = b.table[k + 1] - lo[g];
-    }
+    p[g] = make_uint2(0u, 0u);
+    if (act[g]) __builtin_memcpy(&p[g], b.table + bucket_key<NW>(b, key[g]), sizeof(uint2));
   }
-  bool more = true;
+#pragma unroll
+  for (int g = 0; g < B; ++g) asm volatile("" : "+v"(p[g].x), "+v"(p[g].y));
+  bool more = false;
+#pragma unroll
+  for (int g = 0; g < B; ++g) {
+    lo[g] = p[g].x;
+    n[g] = act[g] ? p[g].y - p[g].x : 0u;
+    more |= n[g] > 1;
+  }
   while (more) {
+    State<NW> m[B];
+#pragma unroll
+    for (int g = 0; g < B; ++g) {
+      m[g] = State<NW>{};
+      if (n[g] > 1) m[g] = load_state_cached<NW>(b.states, lo[g] + (n[g] >> 1));
+    }
+#pragma unroll
+    for (int g = 0; g < B; ++g)
+#pragma unroll
+      for (int w = 0; w < NW; ++w) asm volatile("" : "+v"(m[g].w[w]));
     more = false;
 #pragma unroll
     for (int g = 0; g < B; ++g) {
-      if (n[g] > 1) {
-        const uint32_t half = n[g] >> 1;
-        const State<NW> m = load_state_cached<NW>(b.states, lo[g] + half);
-        lo[g] = st_le<NW>(m, key[g]) ? lo[g] + half : lo[g];
-        n[g] -= half;
-        more |= n[g] > 1;
-      }
+      const uint32_t half = n[g] >> 1;  // 0 once the lookup has finished
+      lo[g] = st_le<NW>(m[g], key[g]) ? lo[g] + half : lo[g];
+      n[g] -= half;
+      more |= n[g] > 1;
     }
   }
+  State<NW> f[B];
 #pragma unroll
   for (int g = 0; g < B; ++g) {
-    rank[g] = kNoRank;
-    if (n[g] == 1) {
-      const State<NW> f = load_state_cached<NW>(b.states, lo[g]);
-      if (st_eq<NW>(f, key[g])) rank[g] = lo[g];
-    }
+    f[g] = State<NW>{};
+    if (n[g] == 1) f[g] = load_state_cached<NW>(b.states, lo[g]);
   }
+#pragma unroll
+  for (int g = 0; g < B; ++g)
+#pragma unroll
+    for (int w = 0; w < NW; ++w) asm volatile("" : "+v"(f[g].w[w]));
+#pragma unroll
+  for (int g = 0; g < B; ++g) rank[g] = n[g] == 1 && st_eq<NW>(f[g], key[g]) ? lo[g] : kNoRank;
 }
 
 // Y[:, c] = H_c X[:, c] (or Y += H_c X) for R columns of row-major blocks,
