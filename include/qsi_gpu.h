@@ -81,8 +81,9 @@ int qsi_device_count(void);
  * 73-86,138-159).  The host array (D rows of [lo,hi], strictly increasing as
  * 128-bit integers) is copied to the device once; a bucket table (2^K+1
  * uint32 ranks over a K-bit key, see qsi_basis_key_fields) is built on the
- * device.  n_spins bounds the state bits (1..128).  bucket_bits = 0 picks K
- * automatically.  Synchronises. */
+ * device.  n_spins bounds the state bits (1..128); D <= 2^32 - 512.
+ * bucket_bits (K, at most 32 and at most n_spins; a 32-bit key is a 16 GiB
+ * table) = 0 picks K automatically (at most 31).  Synchronises. */
 int qsi_basis_create(const uint64_t *h_lohi, int64_t dim, int n_spins,
                      int bucket_bits, qsi_basis_t *out);
 /* As qsi_basis_create; with_hash = 1 also builds a device hash index

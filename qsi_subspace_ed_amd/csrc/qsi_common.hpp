@@ -149,20 +149,20 @@ __host__ __device__ __forceinline__ uint32_t state_field(const State<NW>& s, int
 
 template <int NW>
 __host__ __device__ __forceinline__ uint32_t bucket_key(const BasisView& b, const State<NW>& s) {
-  uint32_t k = 0;
+  uint64_t k = 0;  // 64-bit, so a 32-bit field may shift in
 #pragma unroll
   for (int q = 0; q < kKeySegs; ++q) {
     if (q < b.n_seg) k = (k << b.seg_w[q]) | state_field<NW>(s, b.seg_lo[q], b.seg_w[q]);
   }
-  return k;
+  return static_cast<uint32_t>(k);
 }
 
 template <int NW>
 __host__ __device__ __forceinline__ uint32_t low_key(const BasisView& b, const State<NW>& s) {
-  uint32_t k = 0;
+  uint64_t k = 0;  // 64-bit, so a 32-bit run may shift in
   for (int q = 0; q < b.n_low; ++q)
     k = (k << b.low_w[q]) | state_field<NW>(s, b.low_lo[q], b.low_w[q]);
-  return k;
+  return static_cast<uint32_t>(k);
 }
 
 // true if the state has a bit at or above n_spins (it is then larger than
@@ -203,7 +203,7 @@ __device__ __forceinline__ int64_t basis_lookup(const BasisView& b,
                                                 const State<NW>& key) {
   const uint32_t k = bucket_key<NW>(b, key);
   uint32_t lo = b.table[k];
-  uint32_t n = b.table[k + 1] - lo;
+  uint32_t n = b.table[static_cast<uint64_t>(k) + 1] - lo;
   if (n == 0) return -1;
   while (n > 1) {
     const uint32_t half = n >> 1;
@@ -338,7 +338,7 @@ __device__ __forceinline__ int64_t basis_lower_bound(const BasisView& b,
   };
   const uint32_t k = bucket_key<NW>(b, key);
   const int64_t lo0 = b.table[k];
-  const int64_t lo = bisect(lo0, static_cast<int64_t>(b.table[k + 1]) - lo0);
+  const int64_t lo = bisect(lo0, static_cast<int64_t>(b.table[static_cast<uint64_t>(k) + 1]) - lo0);
   const bool below_ok = lo == 0 || st_lt<NW>(load_state_cached<NW>(b.states, lo - 1), key);
   const bool above_ok = lo == b.dim || !st_lt<NW>(load_state_cached<NW>(b.states, lo), key);
   if (below_ok && above_ok) return lo;

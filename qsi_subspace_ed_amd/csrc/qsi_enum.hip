@@ -282,6 +282,8 @@ extern "C" int qsi_enumerate_basis_device(int n_spins, int n_tetra, const int32_
               "qsi_enumerate_basis_device: null lattice");
   QSI_REQUIRE(n_spinon_pairs >= 0 && n_spinon_pairs < 256,
               "qsi_enumerate_basis_device: n_spinon_pairs must be in [0, 256)");
+  QSI_REQUIRE(bucket_bits >= 0 && bucket_bits <= 32 && bucket_bits <= n_spins,
+              "qsi_enumerate_basis_device: bucket_bits must be in [0, min(32, n_spins)]");
   enumr::EnumLattice L;
   {
     const int rc = enumr::build_lattice(n_spins, n_tetra, tetra_members, order, n_spinon_pairs, L);

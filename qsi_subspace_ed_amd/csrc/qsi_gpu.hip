@@ -40,11 +40,9 @@ constexpr double kSparseTol = 1e-14;  // toSparseMatrix default tol
 // Basis: bucket table
 // ============================================================================
 
-// table[k] = first rank whose bucket key is >= k, for k in [0, 2^K].
+// First rank whose bucket key is >= k (a bisection over the basis).
 template <int NW>
-__global__ void k_build_table(BasisView b, uint32_t nkeys, uint32_t* __restrict__ table) {
-  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k > nkeys) return;
+__device__ __forceinline__ uint32_t table_entry(const BasisView& b, int64_t k) {
   int64_t lo = 0, n = b.dim;
   while (n > 0) {
     const int64_t half = n >> 1;
@@ -56,7 +54,18 @@ __global__ void k_build_table(BasisView b, uint32_t nkeys, uint32_t* __restrict_
       n = half;
     }
   }
-  table[k] = static_cast<uint32_t>(lo);
+  return static_cast<uint32_t>(lo);
+}
+
+// table[k] = first rank whose bucket key is >= k, for k in [0, 2^K].
+// Grid-stride: a dispatch holds fewer than 2^32 work-items, and a 32-bit key
+// has 2^32 + 1 entries.
+template <int NW>
+__global__ void k_build_table(BasisView b, int64_t nkeys, uint32_t* __restrict__ table) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k <= nkeys;
+       k += stride)
+    table[k] = table_entry<NW>(b, k);
 }
 
 // mask[p / 64] bit p % 64 = 1 if some neighbouring pair of the sorted basis
@@ -1952,8 +1961,8 @@ static hipError_t basis_build_index(qsi_basis_s* b, int bucket_bits) {
   const size_t nkeys = size_t{1} << b->kbits;
   e = hipMalloc(&b->d_table, (nkeys + 1) * sizeof(uint32_t));
   if (e != hipSuccess) return e;
-  const uint32_t grid = static_cast<uint32_t>((nkeys + 1 + 255) / 256);
-  const uint32_t nk = static_cast<uint32_t>(nkeys);
+  const uint32_t grid = static_cast<uint32_t>(std::min<size_t>((nkeys + 1 + 255) / 256, 1u << 20));
+  const int64_t nk = static_cast<int64_t>(nkeys);
   if (b->nw == 1)
     hipLaunchKernelGGL(k_build_table<1>, dim3(grid), dim3(256), 0, 0, b->view(), nk, b->d_table);
   else
@@ -1968,12 +1977,12 @@ int qsi_basis_create_ex(const uint64_t* h_lohi, int64_t dim, int n_spins,
   QSI_REQUIRE(out != nullptr, "qsi_basis_create: out is null");
   *out = nullptr;
   QSI_REQUIRE(h_lohi != nullptr || dim == 0, "qsi_basis_create: states are null");
-  QSI_REQUIRE(dim >= 0 && dim < (int64_t{1} << 32) - 1,
-              "qsi_basis_create: dim must be in [0, 2^32-1)");
+  QSI_REQUIRE(dim >= 0 && dim <= kMaxDim,
+              "qsi_basis_create: dim must be in [0, 2^32 - 512]");
   QSI_REQUIRE(n_spins >= 1 && n_spins <= 128,
               "qsi_basis_create: n_spins must be in [1, 128]");
-  QSI_REQUIRE(bucket_bits >= 0 && bucket_bits <= 31 && bucket_bits <= n_spins,
-              "qsi_basis_create: bucket_bits must be in [0, min(31, n_spins)]");
+  QSI_REQUIRE(bucket_bits >= 0 && bucket_bits <= 32 && bucket_bits <= n_spins,
+              "qsi_basis_create: bucket_bits must be in [0, min(32, n_spins)]");
   const int nw = n_spins <= 64 ? 1 : 2;
   // validate: in range and strictly increasing
   for (int64_t i = 0; i < dim; ++i) {
@@ -2205,6 +2214,11 @@ __global__ void k_check_sorted(const uint64_t* __restrict__ st, int64_t dim, int
 int basis_adopt_device(uint64_t* d_states, int64_t dim, int n_spins, int bucket_bits,
                        qsi_basis_s** out) {
   *out = nullptr;
+  if (dim > kMaxDim) {
+    (void)hipFree(d_states);
+    set_error("qsi_basis: more than 2^32 - 512 states");
+    return QSI_E_INVALID;
+  }
   auto* b = new (std::nothrow) qsi_basis_s();
   if (!b) {
     (void)hipFree(d_states);

@@ -8,6 +8,11 @@
 
 namespace qsi {
 
+// Largest basis dimension: ranks are uint32, and a kernel with one thread per
+// row (rounded up to whole 256-thread workgroups) must stay below the 2^32
+// work-items a dispatch can hold.
+constexpr int64_t kMaxDim = (int64_t{1} << 32) - 512;
+
 // Off-diagonal term in pull form.  For a row state s the term contributes
 //   c * sgn * (-1)^popcount(~s & Z) * x[rank(s ^ X)]
 // when (s & m) == v.  With s' = s ^ X the pre-image, this is the reference's

@@ -162,3 +162,46 @@ def test_lowkeys_verification_modes(lattices, cuda):
         assert np.linalg.norm(ys[0].cpu().numpy() - ref) <= 1e-13 * np.linalg.norm(ref)
         for y in ys[1:]:
             assert torch.equal(ys[0], y)
+
+
+def test_widest_bucket_key(cuda):
+    """A 32-bit bucket key (the widest; its table has 2^32 + 1 entries, more
+    than one dispatch of one thread per entry can hold, so the table is built
+    grid-stride) on the 64-spin Q=0 basis, whose states differ at 33
+    informative positions: every state is found at its rank, and the apply
+    gives the same bits as with the automatic key."""
+    torch = cuda
+    free, _ = torch.cuda.mem_get_info()
+    if free < 40e9:
+        pytest.skip("a 32-bit bucket table needs 16 GiB of device memory")
+    jd = q.load_lattice(q.lattice_path("pyro_2,0,0_0,2,0_0,0,4.json"))
+    H = q.build_hamiltonian_g(jd, [-1.0, -0.55, -0.3, -0.8])
+    q.add_local_field(H, jd, [0.07, -0.02, 0.05])
+    ref = q.enumerate_basis_device(jd, 0)
+    states = ref.states()
+    D = states.shape[0]
+    x = torch.from_numpy(np.random.default_rng(3).standard_normal(D)).cuda()
+    ref.build_lowkeys()
+    op_ref = q.GpuOp(ref, H)
+    op_ref.set_variant("lane4")
+    y_ref = torch.zeros_like(x)
+    op_ref.evaluate(x, y_ref)
+    wide = q.GpuBasis(states, 64, bucket_bits=32)
+    assert wide.bucket_bits() == 32
+    assert sum(w for _, w in wide.key_fields()) == 32
+    assert np.array_equal(wide.search(states), np.arange(D))
+    wide.build_lowkeys()
+    op = q.GpuOp(wide, H)
+    assert op.verify_lowkeys() == 2
+    for v in ("lane4", "low4", "low8"):
+        op.set_variant(v)
+        y = torch.zeros_like(x)
+        op.evaluate(x, y)
+        assert torch.equal(y, y_ref), v
+    op.close()
+    wide.close()
+    op_ref.close()
+    ref.close()
+    torch.cuda.empty_cache()
+    with pytest.raises(QsiError):
+        q.GpuBasis(states[:10], 64, bucket_bits=33)
