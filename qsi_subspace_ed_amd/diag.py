@@ -115,8 +115,9 @@ def diagonalise_real(op: GpuOp, opts: Options = Options()):
         op.cache_connectivity(True)
         op.set_variant("cached")
     elif saved in (_lib.QSI_APPLY_AUTO, _lib.QSI_APPLY_CACHED):
-        # matrix-free even when the operator holds a connectivity cache
-        op.set_variant("lane4")
+        # matrix-free even when the operator holds a connectivity cache (the
+        # low-key kernel when the basis has low keys)
+        op.set_variant("low4" if op.basis.has_lowkeys() else "lane4")
     try:
         evals, vecs, nconv, _ = op.eigsh(nev=n_eigvals, ncv=ncv, tol=10.0 ** opts.atol,
                                          max_restarts=opts.max_iters, seed=opts.rng_seed,
