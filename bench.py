@@ -885,7 +885,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-solvers", action="store_true", help="skip the Lanczos/eigsh extras")
     ap.add_argument("--variant", default="auto",
-                    help="apply kernel: auto, lane, lane2, lane4, lane8, window, hash, hash4")
+                    help="apply kernel: auto, lane, lane2, lane4, lane8, low2, low4, low8, "
+                         "slot2, slot3, slot4, window, hash, hash4")
     ap.add_argument("--bucket-bits", type=int, default=0, help="bucket key width K in bits (0 = automatic)")
     ap.add_argument("--no-ftlm", action="store_true", help="skip the batched FTLM comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-solves comparison")

@@ -210,8 +210,9 @@ class GpuOp:
 
     def set_variant(self, variant):
         """Choose the apply kernel: 'auto' ('cached' when the connectivity is
-        cached, else 'low4' when the basis has low keys, else 'lane4'), 'cached', 'hash', 'hash4' (hash-index
-        lookups; the basis needs hash_index=True), 'lane', 'lane2', 'lane4',
+        cached, else 'low4' when the basis has low keys, else 'lane4'),
+        'cached', 'hash', 'hash4' (hash-index lookups; the basis needs
+        hash_index=True), 'lane', 'lane2', 'lane4',
         'lane8' (bucket-table lookups, 1-8 in flight per lane), 'window'
         (LDS-staged bucket windows per wavefront), 'slot2/3/4' (slot index)
         or 'low2/4/8' (low keys)."""
