@@ -446,7 +446,9 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? 5 : 8))) void k_apply_l
 constexpr int kLowCmpState = 0, kLowCmpKey = 1, kLowTrusted = 2;
 template <int NW, bool HAS_Z, bool DOT, int G, int CHECK>
 __global__ __launch_bounds__(kApplyBlock)
-__attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_low(
+__attribute__((amdgpu_waves_per_eu(
+    G >= 8 ? ((CHECK == 2 || (CHECK == 1 && NW == 1)) && !HAS_Z ? 8 : 5) : HAS_Z ? 6 : 8)))
+void k_apply_low(
     BasisView b, const OffTerm* __restrict__ off, int n_off,
     const DiagTerm* __restrict__ dg, int n_diag, const double* __restrict__ x,
     double* y, const double* y_in, double yscale,
@@ -479,8 +481,17 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
       uint64_t mask = connect_mask<NW>(off + c0 + t0, tn, s);
       while (mask) {
         // per lookup: term, bucket start, bucket size (0: nothing to find),
-        // the target's low key; bucket bounds of all G lookups loaded first
-        uint32_t tt[G], lo[G], n[G], r[G];
+        // and below eight lookups the target's low key (with eight it is
+        // re-formed from the staged keys where it is compared, which keeps
+        // the kernel within the registers of eight waves per SIMD); bucket
+        // bounds of all G lookups loaded first
+        uint32_t tt[G], lo[G], n[G];
+        constexpr bool kKeepLow = G < 8;
+        uint32_t r[kKeepLow ? G : 1];
+        auto target_low = [&](int g) -> uint32_t {
+          if constexpr (kKeepLow) return r[g];
+          else return ls ^ tl.key[tt[g]].y;
+        };
         bool act[G];
         uint2 p[G];
 #pragma unroll
@@ -490,7 +501,7 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
           tt[g] = t0 + z;
           mask &= ~(uint64_t{1} << 63 >> z);
           const uint2 kx = tl.key[tt[g]];
-          r[g] = ls ^ kx.y;
+          if constexpr (kKeepLow) r[g] = ls ^ kx.y;
           p[g] = make_uint2(0u, 0u);
           if (act[g]) __builtin_memcpy(&p[g], b.table + (ks ^ kx.x), sizeof(uint2));
         }
@@ -520,7 +531,7 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             const uint32_t half = n[g] >> 1;
-            lo[g] = m[g] <= r[g] ? lo[g] + half : lo[g];
+            lo[g] = m[g] <= target_low(g) ? lo[g] + half : lo[g];
             n[g] -= half;
             more |= n[g] > 1;
           }
@@ -551,7 +562,7 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 5 : HAS_Z ? 6 : 8))) void k_apply_lo
             if (n[g0 + u] == 1 && fabs(xj[u]) > kApplyTol) {
               const int t = static_cast<int>(tt[g0 + u]);
               bool hit = true;
-              if constexpr (CHECK == kLowCmpKey) hit = fk[u] == r[g0 + u];
+              if constexpr (CHECK == kLowCmpKey) hit = fk[u] == target_low(g0 + u);
               if constexpr (CHECK == kLowCmpState) {
                 hit = st_eq<NW>(f[u], staged_target<NW>(tl, s, tt[g0 + u]));
               }
