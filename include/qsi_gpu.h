@@ -162,8 +162,9 @@ int qsi_op_n_diag(qsi_op_t op);
  * LANE4 (SLOT4) or LANE2.  QSI_APPLY_LOW{2,4,8}: as LANE*, bisecting each
  * bucket on the basis's low keys (qsi_basis_build_lowkeys); setting one
  * requires them, and without them later the apply falls back to LANE* of the
- * same depth.  AUTO = CACHED when the connectivity is cached, else LOW4 when
- * the basis has low keys, else LANE4.
+ * same depth.  AUTO = CACHED when the connectivity is cached, else, when the
+ * basis has low keys, LOW8 after qsi_op_verify_lowkeys found mode 2 and LOW4
+ * otherwise, else LANE4.
  * All give bit-identical y. */
 enum {
   QSI_APPLY_AUTO = 0,

@@ -1527,10 +1527,14 @@ static void launch_apply_t(const qsi_op_s* op, const double* x, double* y,
   // AUTO: the cached gathers when the connectivity is cached, else LANE4 (the
   // fastest search kernel measured on the 64- and 96-spin bases, with or
   // without the slot index and low keys; the autotuner times them all)
+  // AUTO: the cached connectivity when it exists, else the low-key kernel
+  // (eight lookups per lane once the candidates need no compare, four
+  // otherwise), else the lane kernel
   if (variant == QSI_APPLY_AUTO)
-    variant = op->conn_nnz >= 0       ? QSI_APPLY_CACHED
-              : op->basis->d_lowkeys ? QSI_APPLY_LOW4
-                                     : QSI_APPLY_LANE4;
+    variant = op->conn_nnz >= 0         ? QSI_APPLY_CACHED
+              : !op->basis->d_lowkeys   ? QSI_APPLY_LANE4
+              : op->lowkeys_check == kLowTrusted ? QSI_APPLY_LOW8
+                                                 : QSI_APPLY_LOW4;
   // a slot or low-key variant chosen while the basis had that index, which
   // has been dropped since, falls back to the lane kernel (LANE4 for SLOT4,
   // LANE2 for SLOT2 and SLOT3; the same depth for LOW*)

@@ -210,7 +210,8 @@ class GpuOp:
 
     def set_variant(self, variant):
         """Choose the apply kernel: 'auto' ('cached' when the connectivity is
-        cached, else 'low4' when the basis has low keys, else 'lane4'),
+        cached, else with low keys 'low8' after verify_lowkeys() gave 2 and
+        'low4' otherwise, else 'lane4'),
         'cached', 'hash', 'hash4' (hash-index lookups; the basis needs
         hash_index=True), 'lane', 'lane2', 'lane4',
         'lane8' (bucket-table lookups, 1-8 in flight per lane), 'window'
