@@ -221,9 +221,14 @@ def test_apply_many_terms(lattices, cuda):
         ys.append((v + " verified", y))
     for v, y in ys[1:]:
         assert torch.equal(ys[0][1], y), v
-    # the dot-product instances (the Lanczos step) through the chunked kernels
+    # AUTO without the cache takes the low-key kernel (LOW8 after a mode-2 check)
     op.cache_connectivity(False)
-    for v in ("lane4", "low4"):
+    op.set_variant("auto")
+    y = torch.zeros_like(xd)
+    op.evaluate(xd, y)
+    assert torch.equal(ys[0][1], y)
+    # the dot-product instances (the Lanczos step) through the chunked kernels
+    for v in ("lane4", "low4", "low8"):
         op.set_variant(v)
         a, bt = op.lanczos_coeffs(6, seed=3)
         if v == "lane4":
