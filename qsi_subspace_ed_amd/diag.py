@@ -85,7 +85,11 @@ def dense_matrix(op: GpuOp):
 def diagonalise_real(op: GpuOp, opts: Options = Options()):
     """(eigenvalues ascending as numpy, eigenvectors as a (D, n) CUDA float64
     tensor) by the algorithm opts.algorithm, or the reference's default for
-    the basis size."""
+    the basis size.  The matrix-free solve (mfsparse) builds the basis's low
+    keys if it has none (4 bytes per state, kept on the basis) and checks them
+    against the operator once (about one apply), so its applies run on the
+    low-key kernel; a basis whose low keys cannot be built keeps the lane
+    kernel.  Every kernel gives the same bits, so the same eigenpairs."""
     import torch
     D = op.rows()
     algo = opts.algorithm or choose_algorithm(D)
@@ -115,9 +119,17 @@ def diagonalise_real(op: GpuOp, opts: Options = Options()):
         op.cache_connectivity(True)
         op.set_variant("cached")
     elif saved in (_lib.QSI_APPLY_AUTO, _lib.QSI_APPLY_CACHED):
-        # matrix-free even when the operator holds a connectivity cache (the
-        # low-key kernel when the basis has low keys)
-        op.set_variant("low4" if op.basis.has_lowkeys() else "lane4")
+        # matrix-free even when the operator holds a connectivity cache: the
+        # low-key kernel, eight lookups per lane when candidates need no
+        # compare (check mode 2), else four; the lane kernel without low keys
+        variant = "lane4"
+        try:
+            if not op.basis.has_lowkeys():
+                op.basis.build_lowkeys()
+            variant = "low8" if op.verify_lowkeys() == 2 else "low4"
+        except _lib.QsiError:
+            pass
+        op.set_variant(variant)
     try:
         evals, vecs, nconv, _ = op.eigsh(nev=n_eigvals, ncv=ncv, tol=10.0 ** opts.atol,
                                          max_restarts=opts.max_iters, seed=opts.rng_seed,

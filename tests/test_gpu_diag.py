@@ -51,6 +51,11 @@ def test_algorithms_against_dense(W, algo, lattices, cuda):
     assert vecs.shape[1] == (1 if algo == "mfeig0" else (5 if (algo or diag.choose_algorithm(
         b.shape[0])) == "dense" else 3))
     _check(evals, vecs, w, M)
+    if algo == "mfsparse":
+        # the matrix-free solve ran on the low-key kernel, left the operator's
+        # kernel choice as it was, and the low keys stay on the basis
+        assert op.basis.has_lowkeys()
+        assert q._lib.load().qsi_op_get_variant(op.handle) == q._lib.QSI_APPLY_AUTO
 
 
 def test_errors(lattices, cuda):
