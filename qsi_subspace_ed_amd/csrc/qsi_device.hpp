@@ -4,6 +4,8 @@
 // components/operator/src/operator_matrix.cpp:65-82).
 #pragma once
 
+#include <cstddef>
+
 #include "qsi_internal.hpp"
 
 namespace qsi {
@@ -28,11 +30,14 @@ __device__ __forceinline__ uint32_t shift_in_lane(uint32_t w, uint64_t lanes) {
   return r;
 }
 
-// A term's projector mask and value, read as one 32-byte block.
+// A term's projector mask and value, read as one 32-byte block (the first
+// two fields of OffTerm).
 struct TermMV {
   uint64_t m[2];
   uint64_t v[2];
 };
+static_assert(offsetof(OffTerm, m) == 0 && offsetof(OffTerm, v) == sizeof(TermMV::m),
+              "OffTerm must start with its m and v words");
 template <int NW>
 __device__ __forceinline__ TermMV load_mv(const OffTerm* __restrict__ t) {
   return *reinterpret_cast<const TermMV*>(t);
@@ -62,7 +67,7 @@ __device__ __forceinline__ uint64_t connect_mask(const OffTerm* __restrict__ off
     const int lo = 32 * h, hi = min(tn, lo + 32);
     uint32_t w = 0;
     for (int t = lo; t < hi;) {
-      // the projector masks of this term and the next, loaded together
+      // this term's projector mask and value in one scalar load
       const TermMV A = load_mv<NW>(off + t);
       uint64_t u[NW];
 #pragma unroll
