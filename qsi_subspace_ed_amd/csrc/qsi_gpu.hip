@@ -433,9 +433,10 @@ __attribute__((amdgpu_waves_per_eu(G >= 8 ? 3 : G >= 4 ? 5 : 8))) void k_apply_l
 // candidate is the last state of the bucket whose low key is <= the
 // target's, the same state the lane variant's bisection ends on, so y is
 // bit-identical.  A lookup keeps only its term, bucket start, bucket size and
-// low key.  Every stage issues the loads of all G lookups (those not yet
-// finished) before it uses any, as in the lane variant, so a lane keeps G
-// loads in flight through the table, each bisection step and the gathers.
+// (below eight lookups per lane) its low key.  Every stage issues the loads of
+// all G lookups (those not yet finished) before it uses any, as in the lane
+// variant, so a lane keeps G loads in flight through the table, each
+// bisection step and the gathers.
 //
 // CHECK says how a candidate is confirmed.  kLowCmpState: its state is
 // loaded and compared.  kLowCmpKey (qsi_op_verify_lowkeys found that a target
