@@ -135,6 +135,13 @@ class GpuBasis {
       : GpuBasis(lohi.data(), static_cast<int64_t>(lohi.size() / 2), n_spins, bucket_bits) {}
   int64_t dim() const { return qsi_basis_dim(h_.get()); }
   int n_spins() const { return qsi_basis_n_spins(h_.get()); }
+  int bucket_bits() const { return qsi_basis_bucket_bits(h_.get()); }
+  // The packed low keys (4 bytes per state) that the QSI_APPLY_LOW* kernels
+  // bisect on (qsi_basis_build_lowkeys); false drops them.
+  void build_lowkeys(bool enable = true) const {
+    throw_if(qsi_basis_build_lowkeys(h_.get(), enable ? 1 : 0), "build_lowkeys");
+  }
+  bool has_lowkeys() const { return qsi_basis_has_lowkeys(h_.get()) != 0; }
   qsi_basis_t handle() const { return h_.get(); }
 
  private:
@@ -192,6 +199,19 @@ class GpuOp {
     int64_t nnz = -1;
     throw_if(qsi_op_cache_connectivity(h_.get(), enable ? 1 : 0, &nnz), "cache_connectivity");
     return nnz;
+  }
+  // Apply kernel (one of the QSI_APPLY_* values; AUTO by default).
+  void set_variant(int variant) const {
+    throw_if(qsi_op_set_variant(h_.get(), variant), "set_variant");
+  }
+  int variant() const { return qsi_op_get_variant(h_.get()); }
+  // One pass over the rows (qsi_op_verify_lowkeys; the basis needs low
+  // keys): 2 when the low-key kernels need no compare, 1 when they compare
+  // low keys, 0 when they compare states.
+  int verify_lowkeys() const {
+    int mode = 0;
+    throw_if(qsi_op_verify_lowkeys(h_.get(), &mode), "verify_lowkeys");
+    return mode;
   }
   // Times the apply kernels and keeps the fastest (R = 0: single vector).
   int autotune(int R = 0, int reps = 3) const {

@@ -12,7 +12,10 @@
 //                        the complete unit-vector start set, krylov_dim 80
 //        ftlm_rng.bin    [Z shifted, E] of 4 samples from mt19937(7) one at a
 //                        time, then of the same 4 as one block
-//   stdout: "one-state ok" when eigsh on the basis of the first state alone
+//   stdout: "lowkeys ok MODE" when, after GpuBasis::build_lowkeys and
+//           GpuOp::verify_lowkeys (MODE), GpuOp::evaluate_device on each
+//           low-key kernel and on AUTO gives y_device.bin's bits;
+//           "one-state ok" when eigsh on the basis of the first state alone
 //           gives that state's diagonal element;
 //           "error ok" when an unsorted basis is refused with qsi::Error
 #include <cmath>
@@ -64,10 +67,29 @@ int main(int argc, char** argv) {
   op.perform_op(x.data(), y.data());
   write_all(dir + "/y_perform.bin", y);
 
+  std::vector<double> y_dev;
   {
     qsi::DeviceVector dx(x.data(), static_cast<size_t>(D)), dy(static_cast<size_t>(D));
     op.evaluate_device(dx.data(), dy.data());
-    write_all(dir + "/y_device.bin", dy.to_host());
+    y_dev = dy.to_host();
+    write_all(dir + "/y_device.bin", y_dev);
+  }
+
+  // the low-key kernels through the C++ face: the same bits
+  {
+    basis.build_lowkeys();
+    const int mode = op.verify_lowkeys();
+    bool same = basis.has_lowkeys();
+    qsi::DeviceVector dx(x.data(), static_cast<size_t>(D)), dy(static_cast<size_t>(D));
+    for (int v : {QSI_APPLY_LOW2, QSI_APPLY_LOW4, QSI_APPLY_LOW8, QSI_APPLY_AUTO}) {
+      op.set_variant(v);
+      same = same && op.variant() == v;
+      op.evaluate_device(dx.data(), dy.data());
+      const std::vector<double> yl = dy.to_host();
+      for (int64_t i = 0; i < D && same; ++i) same = yl[i] == y_dev[i];
+    }
+    basis.build_lowkeys(false);
+    std::printf("lowkeys %s %d\n", same && !basis.has_lowkeys() ? "ok" : "bad", mode);
   }
 
   std::vector<double> y1(D, 1.0);

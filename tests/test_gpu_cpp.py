@@ -1,5 +1,5 @@
 """The C++ face (include/qsi_gpu.hpp) on the GPU: tests/cpp/wrapper_check
-(built by build()) runs qsi::GpuOp::perform_op / evaluate_device,
+(built by build()) runs qsi::GpuOp::perform_op / evaluate_device (also on the low-key kernels),
 qsi::apply_fn, qsi::eigsh and qsi::FtlmComputer on the 32-spin fixture; its outputs must equal the Python
 face's (same kernels, same bits) and the dense spectrum."""
 import ctypes as C
@@ -47,6 +47,8 @@ def test_cpp_face(lattices, cuda, tmp_path):
     assert r.returncode == 0, r.stdout
     assert "error ok" in r.stdout, r.stdout
     assert "one-state ok" in r.stdout, r.stdout
+    # ring flips and a field on the full 32-spin sector: no compare needed
+    assert "lowkeys ok 2" in r.stdout, r.stdout
 
     op = q.GpuOp(q.GpuBasis(b, 32), H)
     xd = torch.from_numpy(x).cuda()
