@@ -42,10 +42,26 @@ def test_version_and_validation():
     assert rc == _lib.QSI_E_INVALID
     rc = lib.qsi_basis_create(None, 1, 0, 0, C.byref(h))
     assert rc == _lib.QSI_E_INVALID
-    # ranks are 32-bit: 2^32 - 1 states or more are refused before any read
-    rc = lib.qsi_basis_create(big.ctypes.data_as(C.c_void_p), (1 << 32) - 1, 8, 0, C.byref(h))
+    # ranks are 32-bit and one thread per row must fit a dispatch: more than
+    # 2^32 - 512 states are refused before any read
+    for dim in ((1 << 32) - 1, (1 << 32) - 511):
+        rc = lib.qsi_basis_create(big.ctypes.data_as(C.c_void_p), dim, 8, 0, C.byref(h))
+        assert rc == _lib.QSI_E_INVALID
+        assert b"dim must be" in lib.qsi_last_error()
+    # bucket keys of at most 32 bits (and at most n_spins), on both basis paths
+    one = np.array([[1, 0]], dtype=np.uint64)
+    for bits, spins in ((33, 64), (9, 8), (-1, 8)):
+        rc = lib.qsi_basis_create(one.ctypes.data_as(C.c_void_p), 1, spins, bits, C.byref(h))
+        assert rc == _lib.QSI_E_INVALID
+        assert b"bucket_bits" in lib.qsi_last_error()
+    tet = np.zeros(4, dtype=np.int32)
+    order = np.arange(8, dtype=np.int32)
+    n = C.c_int64(0)
+    rc = lib.qsi_enumerate_basis_device(8, 1, tet.ctypes.data_as(C.c_void_p),
+                                        order.ctypes.data_as(C.c_void_p), 0, 33, C.byref(h),
+                                        C.byref(n))
     assert rc == _lib.QSI_E_INVALID
-    assert b"dim must be" in lib.qsi_last_error()
+    assert b"bucket_bits" in lib.qsi_last_error()
 
 
 def test_cpp_face_compiles():
